@@ -1,0 +1,97 @@
+/*
+ * Seam between translator-generated loop code and libops_hip.so (C ABI).
+ *
+ * It replaces the runtime calls the reference's generated HIP stubs make around each
+ * kernel launch (ops/c/include/ops_internal2.h:220-339, ops_device_rt_support.h:49-58:
+ * ops_timing_realloc, ops_H_D_exchanges_device, ops_halo_exchanges,
+ * reallocReductArrays/mvReductArraysToDevice/mvReductArraysToHost,
+ * ops_set_dirtybit_device, ops_set_halo_dirtybit3, ops_compute_transfer) and the
+ * const upload of templates/cpp/cuda/master_kernel.cpp.j2:40-67.
+ *
+ * A generated stub calls ops_hip_loop_begin(), launches its kernel with the geometry and
+ * device addresses the descriptor holds, then calls ops_hip_loop_end(). Reductions are
+ * finished on the device (per-block partials, the last block folds them into the
+ * handle's device value); only ops_reduction_result moves the value to the host.
+ */
+#ifndef OPS_AMD_HIP_RT_H
+#define OPS_AMD_HIP_RT_H
+
+#include "ops_lib_core.h"
+
+#define OPS_HIP_MAX_ARGS 24
+#define OPS_HIP_BLOCK_THREADS 256
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-launch descriptor filled by ops_hip_loop_begin. Index i follows the loop's args. */
+typedef struct ops_hip_loop {
+  int ndim;
+  int start[3];       /* first loop index per dimension */
+  int count[3];       /* iterations per dimension (1 for unused dimensions) */
+  unsigned grid[3];   /* workgroups per dimension */
+  unsigned block[3];  /* threads per workgroup (block[0]*block[1]*block[2] == 256) */
+  unsigned nblocks;   /* grid[0]*grid[1]*grid[2] */
+  void *stream;       /* hipStream_t every launch goes to */
+  /* dat args: device address of the loop's first point, and element steps per loop
+     index step in x/y/z (stencil strides and components included). */
+  char *ptr[OPS_HIP_MAX_ARGS];
+  int step[OPS_HIP_MAX_ARGS][3];
+  int xdim[OPS_HIP_MAX_ARGS];  /* size[0] (ACC row pitch, in elements of one component) */
+  int ydim[OPS_HIP_MAX_ARGS];  /* size[1] */
+  int zdim[OPS_HIP_MAX_ARGS];  /* size[2] */
+  int mdim[OPS_HIP_MAX_ARGS];  /* components per point */
+  /* reduction args: per-block partials, completion counter, running device value */
+  void *red_part[OPS_HIP_MAX_ARGS];
+  unsigned *red_count[OPS_HIP_MAX_ARGS];
+  void *red_value[OPS_HIP_MAX_ARGS];
+  int red_init[OPS_HIP_MAX_ARGS];  /* 1: overwrite the running value (first use) */
+  /* bookkeeping */
+  int kernel_id;
+  int timed;
+  double bytes;  /* ops_compute_transfer bytes of this launch */
+} ops_hip_loop;
+
+/* Registers a generated loop name once; returns its id for timing and tuning tables. */
+int ops_hip_register_kernel(const char *name);
+
+/*
+ * Prepares a launch: argument checks, host->device uploads of host-dirty dats, device
+ * addresses, launch geometry (block shape from the tuning table for kernel_id, or a
+ * default), reduction scratch. Returns 0 when the range is empty (nothing to launch; the
+ * stub must still call ops_hip_loop_end), 1 otherwise.
+ */
+int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *name, ops_block block,
+                       int dim, const int *range, ops_arg *args, int nargs);
+/* Completes bookkeeping after the launch: dirty bits, reduction state, timing. */
+void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs);
+
+/* Registers a device symbol (its host handle, HIP_SYMBOL(x)) as the home of an
+   ops_decl_const name; called from the generated translation unit's static initialiser.
+   nbytes < 0: runtime-sized, the symbol holds a device pointer the runtime allocates. */
+void ops_hip_register_const(const char *name, const void *symbol, long nbytes);
+
+/* Checks a HIP status from generated code; throws OPSException(OPS_RUNTIME_ERROR). */
+void ops_hip_check(int status, const char *what, const char *file, int line);
+/* Checks the launch just made (hipGetLastError). */
+void ops_hip_check_launch(const char *name);
+
+/* The runtime's single work stream and a host-side barrier on it. */
+void *ops_hip_get_stream(void);
+void ops_hip_device_sync(void);
+
+/* Tuning: preferred workgroup shape (bx*by*bz must be 256) for a registered kernel. */
+void ops_hip_set_block_shape(int kernel_id, int bx, int by, int bz);
+/* Per-kernel HIP-event timing: on/off, and the table the timing output is built from. */
+void ops_hip_set_timing(int on);
+int ops_hip_kernel_count(void);
+int ops_hip_kernel_stats(int kernel_id, char *name_out, int name_cap, long *calls,
+                         double *total_ms, double *bytes);
+void ops_hip_reset_stats(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPS_AMD_HIP_RT_H */

@@ -1,0 +1,21 @@
+"""ctypes declarations of the translated applications' extern "C" entry points."""
+import ctypes
+
+_i, _d, _p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+_pp = ctypes.POINTER(ctypes.c_char_p)
+
+SIGNATURES = {
+    "laplace2d": {
+        "laplace2d_init": (_i, [_i, _pp, _i, _i, _i]),
+        "laplace2d_iterate": (_i, [_i, _p]),
+        "laplace2d_fetch_full": (_i, [_p]),
+        "laplace2d_finalize": (_i, []),
+    },
+}
+
+
+def declare(name, lib):
+    for fn, (res, args) in SIGNATURES.get(name, {}).items():
+        f = getattr(lib, fn)
+        f.restype = res
+        f.argtypes = args

@@ -1,0 +1,584 @@
+// libops_hip.so device layer: HBM allocation, host<->device movement, the loop-launch
+// seam used by generated code, fused halo copies, slab copies, per-kernel event timing.
+//
+// Behaviour replaced (not ported):
+//   device management      ops/c/src/hip/ops_hip_common.cpp:50-180
+//   halo pack/unpack       ops/c/src/hip/ops_hip_rt_support_kernels.cpp:55-253 and
+//                          ops/c/src/core/ops_device_singlenode_common.cpp:262-346
+//   slab copy kernels      ops/c/src/hip/ops_hip_rt_support_kernels.cpp:256-400
+//   stub pre/post work     templates/cpp/cuda/loop_host.cpp.j2:299-782
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ops_hip_kernel.h"
+#include "ops_internal.h"
+
+#define HIPCHECK(x) ops_hip_check((int)(x), #x, __FILE__, __LINE__)
+
+namespace {
+
+struct DevState {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // pinned staging ring for small stream-ordered uploads (consts, gbl arrays)
+  char *stage = nullptr;
+  size_t stage_cap = 0, stage_off = 0;
+  // pinned bounce buffer for reduction results
+  char *pinned_small = nullptr;
+  // reduction scratch: per-block partials (one region per loop argument) + counters
+  char *red_scratch = nullptr;
+  size_t red_scratch_cap = 0;
+  unsigned *red_counters = nullptr;
+  // event timing
+  struct Pending {
+    int kid;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  bool live = false;
+};
+DevState g_dev;
+int g_timing = 0;
+
+hipEvent_t get_event() {
+  if (!g_dev.pool.empty()) {
+    hipEvent_t e = g_dev.pool.back();
+    g_dev.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIPCHECK(hipEventCreate(&e));
+  return e;
+}
+
+}  // namespace
+
+extern "C" void ops_hip_check(int status, const char *what, const char *file, int line) {
+  if (status != (int)hipSuccess) {
+    char buf[1024];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", status,
+             hipGetErrorString((hipError_t)status), what, file, line);
+    ops_throw(OPS_RUNTIME_ERROR, buf);
+  }
+}
+
+extern "C" void ops_hip_check_launch(const char *name) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP launch of %s failed: %s", name, hipGetErrorString(e));
+    ops_throw(OPS_RUNTIME_ERROR, buf);
+  }
+}
+
+void ops_dev_init(OPS_instance *inst) {
+  if (g_dev.live) return;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    ops_throw(OPS_RUNTIME_CONFIGURATION_ERROR,
+              "Error: no HIP device available (libops_hip.so needs an MI355X / gfx950 GPU)");
+  if (inst->device_id < 0 || inst->device_id >= n)
+    ops_throw(OPS_RUNTIME_CONFIGURATION_ERROR, "Error: OPS_DEVICE out of range");
+  g_dev.device = inst->device_id;
+  HIPCHECK(hipSetDevice(g_dev.device));
+  HIPCHECK(hipStreamCreate(&g_dev.stream));
+  g_dev.stage_cap = 8u << 20;
+  HIPCHECK(hipHostMalloc((void **)&g_dev.stage, g_dev.stage_cap, hipHostMallocDefault));
+  g_dev.stage_off = 0;
+  HIPCHECK(hipHostMalloc((void **)&g_dev.pinned_small, 1u << 16, hipHostMallocDefault));
+  HIPCHECK(hipMalloc((void **)&g_dev.red_counters, sizeof(unsigned) * OPS_HIP_MAX_ARGS));
+  HIPCHECK(hipMemset(g_dev.red_counters, 0, sizeof(unsigned) * OPS_HIP_MAX_ARGS));
+  g_dev.live = true;
+}
+
+void ops_dev_exit(OPS_instance *inst) {
+  (void)inst;
+  if (!g_dev.live) return;
+  ops_dev_timing_flush();
+  HIPCHECK(hipStreamSynchronize(g_dev.stream));
+  for (hipEvent_t ev : g_dev.pool) (void)hipEventDestroy(ev);
+  g_dev.pool.clear();
+  if (g_dev.red_scratch) (void)hipFree(g_dev.red_scratch);
+  g_dev.red_scratch = nullptr;
+  g_dev.red_scratch_cap = 0;
+  (void)hipFree(g_dev.red_counters);
+  (void)hipHostFree(g_dev.stage);
+  (void)hipHostFree(g_dev.pinned_small);
+  (void)hipStreamDestroy(g_dev.stream);
+  g_dev = DevState();
+}
+
+void *ops_dev_malloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  HIPCHECK(hipMalloc(&p, bytes));
+  return p;
+}
+void ops_dev_free(void *p) {
+  if (p) {
+    (void)hipStreamSynchronize(g_dev.stream);
+    (void)hipFree(p);
+  }
+}
+void ops_dev_memset(void *p, int v, size_t bytes) {
+  HIPCHECK(hipMemsetAsync(p, v, bytes, g_dev.stream));
+}
+void ops_dev_sync() {
+  if (g_dev.live) HIPCHECK(hipStreamSynchronize(g_dev.stream));
+}
+void ops_dev_h2d(void *dst, const void *src, size_t bytes) {
+  HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_dev.stream));
+  HIPCHECK(hipStreamSynchronize(g_dev.stream));
+}
+void ops_dev_d2h(void *dst, const void *src, size_t bytes) {
+  if (bytes <= (1u << 16)) {
+    HIPCHECK(hipMemcpyAsync(g_dev.pinned_small, src, bytes, hipMemcpyDeviceToHost, g_dev.stream));
+    HIPCHECK(hipStreamSynchronize(g_dev.stream));
+    memcpy(dst, g_dev.pinned_small, bytes);
+  } else {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_dev.stream));
+    HIPCHECK(hipStreamSynchronize(g_dev.stream));
+  }
+}
+void ops_dev_d2d(void *dst, const void *src, size_t bytes) {
+  HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_dev.stream));
+}
+void ops_dev_stage_h2d(void *dst, const void *src, size_t bytes) {
+  if (bytes > g_dev.stage_cap) {
+    ops_dev_h2d(dst, src, bytes);
+    return;
+  }
+  size_t off = (g_dev.stage_off + 255) & ~(size_t)255;
+  if (off + bytes > g_dev.stage_cap) {
+    // ring wraps: earlier staged copies must have been consumed
+    HIPCHECK(hipStreamSynchronize(g_dev.stream));
+    off = 0;
+  }
+  memcpy(g_dev.stage + off, src, bytes);
+  HIPCHECK(hipMemcpyAsync(dst, g_dev.stage + off, bytes, hipMemcpyHostToDevice, g_dev.stream));
+  g_dev.stage_off = off + bytes;
+}
+
+void *ops_dev_symbol_address(const void *symbol) {
+  void *p = nullptr;
+  HIPCHECK(hipGetSymbolAddress(&p, symbol));
+  return p;
+}
+
+extern "C" void *ops_hip_get_stream(void) { return (void *)g_dev.stream; }
+extern "C" void ops_hip_device_sync(void) { ops_dev_sync(); }
+
+// ------------------------------------------------------------------ timing
+
+extern "C" void ops_hip_set_timing(int on) { g_timing = on; }
+
+void ops_dev_timing_flush() {
+  if (g_dev.pending.empty()) return;
+  HIPCHECK(hipEventSynchronize(g_dev.pending.back().b));
+  auto &kt = ops_kernel_table();
+  for (auto &p : g_dev.pending) {
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    kt[p.kid].total_ms += ms;
+    g_dev.pool.push_back(p.a);
+    g_dev.pool.push_back(p.b);
+  }
+  g_dev.pending.clear();
+}
+
+extern "C" int ops_hip_register_kernel(const char *name) {
+  auto &kt = ops_kernel_table();
+  for (size_t i = 0; i < kt.size(); i++)
+    if (kt[i].name == name) return (int)i;
+  ops_kernel_entry e;
+  e.name = name;
+  kt.push_back(e);
+  return (int)kt.size() - 1;
+}
+
+extern "C" void ops_hip_set_block_shape(int kernel_id, int bx, int by, int bz) {
+  auto &kt = ops_kernel_table();
+  if (kernel_id < 0 || kernel_id >= (int)kt.size()) return;
+  if (bx * by * bz > OPS_HIP_BLOCK_THREADS || bx <= 0 || by <= 0 || bz <= 0)
+    ops_throw(OPS_INVALID_ARGUMENT, "ops_hip_set_block_shape: workgroup must have <= 256 threads");
+  kt[kernel_id].bx = bx;
+  kt[kernel_id].by = by;
+  kt[kernel_id].bz = bz;
+}
+
+extern "C" int ops_hip_kernel_count(void) { return (int)ops_kernel_table().size(); }
+
+extern "C" int ops_hip_kernel_stats(int kernel_id, char *name_out, int name_cap, long *calls,
+                                    double *total_ms, double *bytes) {
+  ops_dev_timing_flush();
+  auto &kt = ops_kernel_table();
+  if (kernel_id < 0 || kernel_id >= (int)kt.size()) return 0;
+  if (name_out && name_cap > 0) {
+    strncpy(name_out, kt[kernel_id].name.c_str(), (size_t)name_cap - 1);
+    name_out[name_cap - 1] = 0;
+  }
+  if (calls) *calls = kt[kernel_id].calls;
+  if (total_ms) *total_ms = kt[kernel_id].total_ms;
+  if (bytes) *bytes = kt[kernel_id].bytes;
+  return 1;
+}
+
+extern "C" void ops_hip_reset_stats(void) {
+  ops_dev_timing_flush();
+  for (auto &k : ops_kernel_table()) {
+    k.calls = 0;
+    k.total_ms = 0;
+    k.bytes = 0;
+    k.host_s = 0;
+  }
+}
+
+// ------------------------------------------------------------------ loop seam
+
+static unsigned pow2_at_least(int v) {
+  unsigned p = 1;
+  while ((int)p < v && p < 256) p <<= 1;
+  return p;
+}
+
+// bytes per dat argument by ops_compute_transfer's rule (ops_lib_core.cpp:1687-1697)
+static double transfer_bytes(int dims, const int *start, const int *end, const ops_arg *a) {
+  double size = 1.0;
+  for (int i = 0; i < dims; i++)
+    if (a->stencil->stride[i] != 0 && (end[i] - start[i]) > 0) size *= end[i] - start[i];
+  return size * a->dat->elem_size * ((a->acc == OPS_READ || a->acc == OPS_WRITE) ? 1.0 : 2.0);
+}
+
+extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *name,
+                                  ops_block block, int dim, const int *range, ops_arg *args,
+                                  int nargs) {
+  OPS_instance *in = ops_instance_get();
+  (void)name;
+  if (nargs > OPS_HIP_MAX_ARGS) ops_throw(OPS_NOT_IMPLEMENTED, "too many loop arguments");
+  lp->kernel_id = kernel_id;
+  lp->ndim = dim;
+  lp->stream = (void *)g_dev.stream;
+  lp->timed = 0;
+  lp->bytes = 0;
+  int start[OPS_MAX_DIM] = {0, 0, 0, 0, 0}, end[OPS_MAX_DIM] = {1, 1, 1, 1, 1};
+  long total = 1;
+  for (int d = 0; d < 3; d++) {
+    lp->start[d] = 0;
+    lp->count[d] = 1;
+  }
+  if (dim > 3) ops_throw(OPS_NOT_IMPLEMENTED, "ops_par_loop over more than 3 dimensions");
+  for (int d = 0; d < dim; d++) {
+    start[d] = range[2 * d];
+    end[d] = range[2 * d + 1];
+    int c = end[d] - start[d];
+    if (c < 0) c = 0;
+    lp->start[d] = start[d];
+    lp->count[d] = c;
+    total *= c;
+  }
+  for (int i = 0; i < nargs; i++) {
+    ops_arg &a = args[i];
+    lp->ptr[i] = nullptr;
+    lp->red_part[i] = nullptr;
+    lp->red_value[i] = nullptr;
+    lp->red_count[i] = nullptr;
+    lp->red_init[i] = 0;
+    if (a.argtype == OPS_ARG_DAT && a.opt) {
+      ops_dat dat = a.dat;
+      if (dat->locked_hd != 0)
+        ops_throw(OPS_RUNTIME_ERROR,
+                  std::string("Error: ops_par_loop called with dat ") + dat->name +
+                      " that is locked by ops_dat_get_raw_pointer");
+      if (dat->dirty_hd == 1) ops_put_data(dat);
+      const int *sst = a.stencil->stride;
+      size_t esz = in->soa ? (size_t)dat->type_size : (size_t)dat->elem_size;
+      long long off = 0, cum = 1;
+      for (int d = 0; d < dim; d++) {
+        int s = (d < a.stencil->dims) ? sst[d] : 1;
+        off += (long long)esz * cum * ((long long)start[d] * s - dat->base[d] - dat->d_m[d]);
+        cum *= dat->size[d];
+      }
+      lp->ptr[i] = dat->data_d + off;
+      int comp = in->soa ? 1 : dat->dim;
+      long long cumd = 1;
+      for (int d = 0; d < 3; d++) {
+        int s = (d < dim && d < a.stencil->dims) ? sst[d] : 0;
+        lp->step[i][d] = (int)(s * cumd * comp);
+        cumd *= dat->size[d];
+      }
+      lp->xdim[i] = dat->size[0];
+      lp->ydim[i] = dat->size[1];
+      lp->zdim[i] = dat->size[2];
+      lp->mdim[i] = dat->dim;
+      if (total > 0) lp->bytes += transfer_bytes(dim, start, end, &a);
+    }
+  }
+  if (total == 0) {
+    lp->nblocks = 0;
+    return 0;
+  }
+  // workgroup shape: tuned, else 64 lanes along x (coalesced, one wavefront per row
+  // segment) and the remaining threads stacked along y/z
+  auto &kt = ops_kernel_table();
+  unsigned bx, by, bz;
+  if (kernel_id >= 0 && kernel_id < (int)kt.size() && kt[kernel_id].bx > 0) {
+    bx = kt[kernel_id].bx;
+    by = kt[kernel_id].by;
+    bz = kt[kernel_id].bz;
+  } else {
+    bx = pow2_at_least(lp->count[0]);
+    if (bx > 64) bx = 64;
+    if (in->block_size_x > 0) bx = in->block_size_x;
+    by = pow2_at_least(lp->count[1]);
+    if (by > OPS_HIP_BLOCK_THREADS / bx) by = OPS_HIP_BLOCK_THREADS / bx;
+    if (in->block_size_y > 0) by = in->block_size_y;
+    bz = pow2_at_least(lp->count[2]);
+    if (bz > OPS_HIP_BLOCK_THREADS / (bx * by)) bz = OPS_HIP_BLOCK_THREADS / (bx * by);
+    if (bz < 1) bz = 1;
+  }
+  lp->block[0] = bx;
+  lp->block[1] = by;
+  lp->block[2] = bz;
+  lp->grid[0] = (lp->count[0] + bx - 1) / bx;
+  lp->grid[1] = (lp->count[1] + by - 1) / by;
+  lp->grid[2] = (lp->count[2] + bz - 1) / bz;
+  lp->nblocks = lp->grid[0] * lp->grid[1] * lp->grid[2];
+  // reduction scratch
+  size_t need = 0;
+  for (int i = 0; i < nargs; i++)
+    if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) {
+      ops_reduction h = (ops_reduction)args[i].data;
+      need += ((size_t)lp->nblocks * (size_t)h->size + 255) & ~(size_t)255;
+    }
+  if (need > g_dev.red_scratch_cap) {
+    if (g_dev.red_scratch) {
+      HIPCHECK(hipStreamSynchronize(g_dev.stream));
+      HIPCHECK(hipFree(g_dev.red_scratch));
+    }
+    size_t cap = need * 2;
+    HIPCHECK(hipMalloc((void **)&g_dev.red_scratch, cap));
+    g_dev.red_scratch_cap = cap;
+  }
+  size_t off = 0;
+  for (int i = 0; i < nargs; i++)
+    if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) {
+      ops_reduction h = (ops_reduction)args[i].data;
+      lp->red_part[i] = g_dev.red_scratch + off;
+      off += ((size_t)lp->nblocks * (size_t)h->size + 255) & ~(size_t)255;
+      lp->red_count[i] = g_dev.red_counters + i;
+      lp->red_value[i] = h->data_d;
+      lp->red_init[i] = h->dirty_d ? 0 : 1;
+    }
+  if (g_timing) {
+    hipEvent_t a = get_event(), b = get_event();
+    HIPCHECK(hipEventRecord(a, g_dev.stream));
+    g_dev.pending.push_back({kernel_id, a, b});
+    lp->timed = 1;
+  }
+  return 1;
+}
+
+extern "C" void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs) {
+  if (lp->timed) {
+    HIPCHECK(hipEventRecord(g_dev.pending.back().b, g_dev.stream));
+    if (g_dev.pending.size() >= 8192) ops_dev_timing_flush();
+  }
+  auto &kt = ops_kernel_table();
+  if (lp->kernel_id >= 0 && lp->kernel_id < (int)kt.size()) {
+    kt[lp->kernel_id].calls++;
+    kt[lp->kernel_id].bytes += lp->bytes;
+  }
+  for (int i = 0; i < nargs; i++) {
+    ops_arg &a = args[i];
+    if (a.argtype == OPS_ARG_DAT && a.opt && a.acc != OPS_READ) a.dat->dirty_hd = 2;
+    if (a.argtype == OPS_ARG_GBL && a.acc != OPS_READ && lp->nblocks > 0)
+      ((ops_reduction)a.data)->dirty_d = 1;
+  }
+}
+
+// ------------------------------------------------------------------ affine copies
+// One thread per (point, component) of an up-to-5D box. Source and destination
+// addresses are affine in the box coordinates (negative strides express flips and
+// permutations), so a halo transfer is a single direct copy with no staging buffer.
+
+struct AffineCopy {
+  int ndim;
+  int count[OPS_MAX_DIM];
+  long long sstride[OPS_MAX_DIM];  // in source elements
+  long long dstride[OPS_MAX_DIM];  // in destination elements
+  long long sbase, dbase;
+  int ncomp;
+  long long scomp, dcomp;  // component strides
+  long long total;         // points * ncomp
+};
+
+template <typename S, typename M, typename D>
+__global__ __launch_bounds__(256) void affine_copy_kernel(const S *__restrict__ src,
+                                                          D *__restrict__ dst, AffineCopy c) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.total) return;
+  int comp = (int)(t % c.ncomp);
+  long long p = t / c.ncomp;
+  long long so = c.sbase + comp * c.scomp, dof = c.dbase + comp * c.dcomp;
+  for (int d = 0; d < c.ndim; d++) {
+    long long q = p % c.count[d];
+    p /= c.count[d];
+    so += q * c.sstride[d];
+    dof += q * c.dstride[d];
+  }
+  dst[dof] = (D)(M)src[so];
+}
+
+template <typename S, typename M, typename D>
+static void launch_affine(const void *src, void *dst, const AffineCopy &c) {
+  if (c.total <= 0) return;
+  unsigned blocks = (unsigned)((c.total + 255) / 256);
+  hipLaunchKernelGGL((affine_copy_kernel<S, M, D>), dim3(blocks), dim3(256), 0, g_dev.stream,
+                     (const S *)src, (D *)dst, c);
+  ops_hip_check_launch("affine_copy");
+}
+
+static int type_code(const char *t) {
+  if (!strcmp(t, "double")) return 8;
+  if (!strcmp(t, "float")) return 4;
+  return 0;
+}
+
+static void launch_by_size(int size, const void *src, void *dst, const AffineCopy &c) {
+  switch (size) {
+    case 1: launch_affine<unsigned char, unsigned char, unsigned char>(src, dst, c); break;
+    case 2: launch_affine<unsigned short, unsigned short, unsigned short>(src, dst, c); break;
+    case 4: launch_affine<unsigned, unsigned, unsigned>(src, dst, c); break;
+    case 8: launch_affine<unsigned long long, unsigned long long, unsigned long long>(src, dst, c); break;
+    default: ops_throw(OPS_NOT_IMPLEMENTED, "copy of unsupported element size");
+  }
+}
+
+// Halo region mapping of ops_halo_transfer (ops_device_singlenode_common.cpp:286-336):
+// dat dimension i walks halo axis |dir[i]|-1, forwards from base-d_m-base_dat when
+// dir > 0, backwards from the end of that window when dir < 0.
+static void halo_side(ops_dat dat, const int *hbase, const int *dir, const int *iter,
+                      long long *stride_per_axis, long long *base_elem) {
+  long long cum = 1;
+  *base_elem = 0;
+  for (int a = 0; a < OPS_MAX_DIM; a++) stride_per_axis[a] = 0;
+  for (int i = 0; i < OPS_MAX_DIM; i++) {
+    int ax = abs(dir[i]) - 1;
+    long long first, step;
+    if (dir[i] > 0) {
+      first = hbase[i] - dat->d_m[i] - dat->base[i];
+      step = 1;
+    } else {
+      first = (long long)(hbase[i] - 1 - dat->d_m[i] - dat->base[i]) + iter[ax];
+      step = -1;
+    }
+    *base_elem += first * cum;
+    stride_per_axis[ax] += step * cum;
+    cum *= dat->size[i];
+  }
+}
+
+void ops_dev_halo_copy(ops_halo h) {
+  ops_dat f = h->from, t = h->to;
+  AffineCopy c;
+  c.ndim = OPS_MAX_DIM;
+  long long pts = 1;
+  for (int a = 0; a < OPS_MAX_DIM; a++) {
+    c.count[a] = h->iter_size[a];
+    pts *= h->iter_size[a];
+  }
+  long long fs[OPS_MAX_DIM], ts[OPS_MAX_DIM], fb, tb;
+  halo_side(f, h->from_base, h->from_dir, h->iter_size, fs, &fb);
+  halo_side(t, h->to_base, h->to_dir, h->iter_size, ts, &tb);
+  int ncomp = f->dim < t->dim ? f->dim : t->dim;
+  // element strides in units of one component (AoS)
+  for (int a = 0; a < OPS_MAX_DIM; a++) {
+    c.sstride[a] = fs[a] * f->dim;
+    c.dstride[a] = ts[a] * t->dim;
+  }
+  c.sbase = fb * f->dim;
+  c.dbase = tb * t->dim;
+  c.ncomp = ncomp;
+  c.scomp = 1;
+  c.dcomp = 1;
+  c.total = pts * ncomp;
+  if (pts <= 0) return;
+  int fc = type_code(f->type), tc = type_code(t->type);
+  if (f->type_size != t->type_size && fc && tc) {
+    // mixed precision: through the narrower storage type, as the staged buffer does
+    if (fc == 8 && tc == 4) launch_affine<double, float, float>(f->data_d, t->data_d, c);
+    else launch_affine<float, float, double>(f->data_d, t->data_d, c);
+  } else {
+    if (f->type_size != t->type_size)
+      ops_throw(OPS_NOT_IMPLEMENTED, "halo transfer between dats of different types");
+    launch_by_size(f->type_size, f->data_d, t->data_d, c);
+  }
+}
+
+// Slab copy between a dat and a dense, packed buffer (no halo): range is given in
+// interior coordinates [begin, end) per dimension (ops_util.cpp:353-500).
+void ops_dev_slab_copy(ops_dat dat, char *buf, int buf_on_device, const int *range, int dir) {
+  OPS_instance *in = ops_instance_get();
+  int nd = dat->block->dims;
+  AffineCopy c;
+  c.ndim = nd;
+  long long pts = 1, cum = 1, bcum = 1, base = 0;
+  int soa = in->soa;
+  long long plane = 1;
+  for (int d = 0; d < nd; d++) plane *= dat->size[d];
+  for (int d = 0; d < nd; d++) {
+    int n = range[2 * d + 1] - range[2 * d];
+    if (n < 0) n = 0;
+    c.count[d] = n;
+    pts *= n;
+    long long datstride = soa ? cum : cum * dat->dim;
+    base += (long long)(range[2 * d] - dat->d_m[d]) * datstride;
+    long long bufstride = bcum * dat->dim;
+    if (dir == 0) {
+      c.sstride[d] = datstride;
+      c.dstride[d] = bufstride;
+    } else {
+      c.sstride[d] = bufstride;
+      c.dstride[d] = datstride;
+    }
+    cum *= dat->size[d];
+    bcum *= n;
+  }
+  c.ncomp = dat->dim;
+  long long dcompstride = soa ? plane : 1;
+  if (dir == 0) {
+    c.sbase = base;
+    c.dbase = 0;
+    c.scomp = dcompstride;
+    c.dcomp = 1;
+  } else {
+    c.sbase = 0;
+    c.dbase = base;
+    c.scomp = 1;
+    c.dcomp = dcompstride;
+  }
+  c.total = pts * dat->dim;
+  size_t bytes = (size_t)pts * (size_t)dat->elem_size;
+  if (pts == 0) return;
+  char *dbuf = buf;
+  if (!buf_on_device) dbuf = (char *)ops_dev_malloc(bytes);
+  if (dir == 0) {
+    launch_by_size(dat->type_size, dat->data_d, dbuf, c);
+    if (!buf_on_device) ops_dev_d2h(buf, dbuf, bytes);
+  } else {
+    if (!buf_on_device) ops_dev_h2d(dbuf, buf, bytes);
+    launch_by_size(dat->type_size, dbuf, dat->data_d, c);
+  }
+  if (!buf_on_device) {
+    ops_dev_sync();
+    ops_dev_free(dbuf);
+  }
+}

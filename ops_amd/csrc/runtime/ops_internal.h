@@ -1,0 +1,75 @@
+// Internal state of libops_hip.so. Not part of the public ABI.
+#ifndef OPS_AMD_INTERNAL_H
+#define OPS_AMD_INTERNAL_H
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ops_hip_rt.h"
+#include "ops_lib_core.h"
+
+struct ops_kernel_entry {
+  std::string name;
+  long calls = 0;
+  double total_ms = 0.0;  // sum of HIP-event durations (timing on)
+  double bytes = 0.0;     // ops_compute_transfer bytes summed over calls
+  double host_s = 0.0;    // host wall time spent in begin..end (stub overhead)
+  int bx = 0, by = 0, bz = 0;  // tuned workgroup shape, 0 = default
+};
+
+struct ops_const_entry {
+  const void *symbol = nullptr;  // host handle of the __constant__/__device__ symbol
+  void *dev = nullptr;           // its device address (resolved on first use)
+  long nbytes = 0;      // fixed size, or < 0 for runtime-sized (symbol holds a pointer)
+  void *owned = nullptr;  // runtime-sized: current device allocation
+  long owned_bytes = 0;
+};
+
+struct OPS_instance {
+  int diags = 1;
+  int soa = 0;
+  int block_size_x = 0, block_size_y = 0, block_size_z = 0;
+  int device_id = 0;
+  int timing = 0;  // per-kernel HIP events
+  std::vector<ops_block> blocks;
+  std::vector<ops_dat> dats;
+  std::vector<ops_stencil> stencils;
+  std::vector<ops_reduction> reductions;
+  std::vector<ops_halo> halos;
+  std::vector<ops_halo_group> groups;
+  ops_stencil internal0[OPS_MAX_DIM + 1] = {};  // one-point stencils per dimensionality
+  std::string dump_dir;                          // OPS_DUMP_DIR= (dats written at ops_exit)
+};
+
+OPS_instance *ops_instance_get(bool create = false);
+[[noreturn]] void ops_throw(int code, const std::string &msg);
+
+// kernel + const registries live for the whole process (generated code registers them
+// from static initialisers, before any ops_init)
+std::vector<ops_kernel_entry> &ops_kernel_table();
+std::map<std::string, ops_const_entry> &ops_const_table();
+
+// ---- device layer (ops_device.hip) ----
+void ops_dev_init(OPS_instance *inst);
+void ops_dev_exit(OPS_instance *inst);
+void *ops_dev_malloc(size_t bytes);
+void ops_dev_free(void *p);
+void ops_dev_memset(void *p, int v, size_t bytes);
+void ops_dev_h2d(void *dst, const void *src, size_t bytes);   // synchronous
+void ops_dev_d2h(void *dst, const void *src, size_t bytes);   // synchronous (drains stream)
+void ops_dev_d2d(void *dst, const void *src, size_t bytes);   // stream-ordered
+void ops_dev_stage_h2d(void *dst, const void *src, size_t bytes);  // stream-ordered, src reusable
+void ops_dev_sync();
+void ops_dev_halo_copy(ops_halo h);
+// packed slab copy between a dat (device) and a dense buffer; dir 0: dat->buf, 1: buf->dat
+void ops_dev_slab_copy(ops_dat dat, char *buf, int buf_on_device, const int *range, int dir);
+void *ops_dev_symbol_address(const void *symbol);
+void ops_dev_timing_flush();
+
+// ---- host-side helpers (ops_core.cpp) ----
+void ops_put_data(ops_dat dat);  // host-dirty -> device
+void ops_dat_host_alloc(ops_dat dat);
+void ops_dump_all_dats(OPS_instance *inst, const char *dir);
+
+#endif

@@ -1,0 +1,415 @@
+"""`hip_gfx950` code-generation target of the OPS translator.
+
+For an application it emits
+  * `<src>_ops.cpp` per source file: the program with every `ops_par_loop(kernel, ...)`
+    rewritten to `ops_par_loop_<kernel>(...)` and the prototypes inserted after the OPS
+    include (the reference's program rewrite: ops_translator/ops-translator/cpp/
+    translator/program.py:10-48);
+  * `<app>_kernels.hip`: one translation unit holding every loop of the application as a
+    gfx950 kernel plus its host stub, the `ops_decl_const` symbols, and the user kernel
+    headers compiled for the device.
+
+Kernel shape (one thread per iteration point): 2D/3D workgroups of 256 lanes with 64
+lanes along x so that each wavefront reads whole row segments, an XCD-aware tile map,
+ACC<T> views at the thread's point, wavefront butterfly + workgroup reductions finished
+on the device, small ops_arg_gbl values passed by value in the kernel arguments.
+"""
+from __future__ import annotations
+
+import os
+import re
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Dict, List, Optional, Tuple
+
+from . import cparse
+
+ACCESS = {"OPS_READ": 0, "OPS_WRITE": 1, "OPS_RW": 2, "OPS_INC": 3, "OPS_MIN": 4, "OPS_MAX": 5}
+RED_OP = {"OPS_INC": "ops_red_inc", "OPS_MIN": "ops_red_min", "OPS_MAX": "ops_red_max"}
+CTYPES = {"double": "double", "float": "float", "int": "int", "ll": "long long",
+          "long long": "long long"}
+
+
+class TranslateError(Exception):
+    pass
+
+
+@dataclass
+class Arg:
+    kind: str                 # dat | gbl | reduce | idx
+    expr: str = ""            # dat / pointer / handle expression
+    dim: str = "1"
+    stencil: str = ""
+    typ: str = ""
+    acc: str = "OPS_READ"
+    opt: bool = False
+
+    def key(self):
+        return (self.kind, self.typ, self.acc if self.kind != "dat" else self.acc, self.dim
+                if self.kind != "dat" else (self.dim if cparse.int_literal(self.dim) == 1 else "m"))
+
+
+@dataclass
+class Loop:
+    kernel: str
+    name_expr: str
+    args: List[Arg]
+    file: str
+    line: int
+    call: cparse.Call
+    stub: str = ""
+
+
+@dataclass
+class Const:
+    name: str
+    dim: str
+    typ: str
+
+
+@dataclass
+class Param:
+    text: str
+    is_acc: bool
+    ctype: str
+    const: bool
+
+
+def parse_param(p: str) -> Param:
+    s = " ".join(p.split())
+    m = re.search(r"ACC\s*<\s*([^>]+?)\s*>", s)
+    if m:
+        return Param(s, True, m.group(1), s.startswith("const"))
+    m = re.match(r"(const\s+)?([A-Za-z_][\w\s]*?)\s*\*", s)
+    if m:
+        return Param(s, False, m.group(2).strip(), bool(m.group(1)))
+    raise TranslateError(f"unsupported kernel parameter '{p}' (expected ACC<T>& or T*)")
+
+
+def parse_arg(text: str) -> Arg:
+    masked = cparse.mask(text)
+    calls = cparse.find_calls(text, masked, ["ops_arg_dat_opt", "ops_arg_dat", "ops_arg_gbl",
+                                             "ops_arg_reduce", "ops_arg_idx"])
+    if not calls or calls[0].start != len(text) - len(text.lstrip()):
+        raise TranslateError(f"unrecognised ops_par_loop argument: {text}")
+    c = calls[0]
+    a = c.args
+    if c.name == "ops_arg_idx":
+        return Arg("idx")
+    if c.name in ("ops_arg_dat", "ops_arg_dat_opt"):
+        return Arg("dat", a[0], a[1], a[2], cparse.unquote(a[3]), a[4].strip(),
+                   c.name == "ops_arg_dat_opt")
+    if c.name == "ops_arg_gbl":
+        acc = a[3].strip()
+        if acc != "OPS_READ":
+            raise TranslateError("ops_arg_gbl supports OPS_READ; use ops_arg_reduce for reductions")
+        return Arg("gbl", a[0], a[1], "", cparse.unquote(a[2]), acc)
+    return Arg("reduce", a[0], a[1], "", cparse.unquote(a[2]), a[3].strip())
+
+
+def scan_source(path: str):
+    text = Path(path).read_text()
+    masked = cparse.mask(text)
+    loops, consts = [], []
+    for c in cparse.find_calls(text, masked, "ops_par_loop"):
+        if len(c.args) < 5:
+            raise TranslateError(f"{path}:{c.line}: ops_par_loop needs kernel, name, block, dim, range")
+        loops.append(Loop(c.args[0], c.args[1], [parse_arg(x) for x in c.args[5:]], path,
+                          c.line, c))
+    for c in cparse.find_calls(text, masked, ["ops_decl_const", "ops_update_const"]):
+        if c.name == "ops_decl_const":
+            consts.append(Const(cparse.unquote(c.args[0]), c.args[1].strip(),
+                                cparse.unquote(c.args[2])))
+    m = re.search(r"(?m)^\s*#\s*define\s+OPS_([1-5])D\b", masked)
+    ndim = int(m.group(1)) if m else None
+    return text, loops, consts, ndim
+
+
+def kernel_headers(app_dir: str, sources: List[str]) -> Dict[str, Tuple[str, cparse.FuncDef]]:
+    """Map kernel function name -> (header path, definition) for every header in the
+    application directory."""
+    out = {}
+    for p in sorted(Path(app_dir).glob("*.h")):
+        text = p.read_text()
+        for f in cparse.find_functions(text):
+            out[f.name] = (str(p), f)
+    return out
+
+
+# ------------------------------------------------------------------ code generation
+
+def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
+    k = loop.stub
+    L = []
+    # kernel-argument struct
+    L.append(f"struct ops_kp_{k} {{")
+    L.append("  int cnt0, cnt1, cnt2, st0, st1, st2;")
+    for i, a in enumerate(loop.args):
+        if a.kind == "dat":
+            t = params[i].ctype
+            L.append(f"  {t} *p{i}; int sx{i}, sy{i}, sz{i}, xd{i}, yd{i}, zd{i}, md{i};")
+        elif a.kind == "gbl":
+            d = cparse.int_literal(a.dim)
+            if d is None:
+                raise TranslateError(f"{loop.file}:{loop.line}: ops_arg_gbl dim must be a literal")
+            L.append(f"  ops_gbl<{CTYPES.get(a.typ, a.typ)}, {d}> g{i};")
+        elif a.kind == "reduce":
+            t = CTYPES.get(a.typ, a.typ)
+            L.append(f"  {t} *rp{i}; unsigned *rc{i}; {t} *rv{i}; int ri{i};")
+    L.append("};")
+    L.append("")
+    red = [i for i, a in enumerate(loop.args) if a.kind == "reduce"]
+    L.append(f"__global__ __launch_bounds__(256) void ops_kern_{k}(const ops_kp_{k} p) {{")
+    L.append("  unsigned bx, by, bz;")
+    L.append("  ops_block_coords(bx, by, bz);")
+    L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
+    L.append("  const int y = (int)(by * blockDim.y + threadIdx.y);")
+    L.append("  const int z = (int)(bz * blockDim.z + threadIdx.z);")
+    for i in red:
+        a = loop.args[i]
+        t = CTYPES.get(a.typ, a.typ)
+        d = cparse.int_literal(a.dim)
+        if d is None:
+            raise TranslateError(f"{loop.file}:{loop.line}: ops_arg_reduce dim must be a literal")
+        L.append(f"  {t} r{i}[{d}];")
+        L.append(f"  for (int d = 0; d < {d}; d++) r{i}[d] = {RED_OP[a.acc]}<{t}>::identity();")
+    L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
+    call = []
+    for i, a in enumerate(loop.args):
+        pr = params[i]
+        if a.kind == "dat":
+            if not pr.is_acc:
+                raise TranslateError(f"{loop.file}:{loop.line}: kernel {loop.kernel} parameter {i} "
+                                     "must be ACC<T>& for an ops_arg_dat argument")
+            t = pr.ctype
+            off = f"x * p.sx{i} + y * p.sy{i} + z * p.sz{i}"
+            ptr = f"p.p{i} + ({off})"
+            mlit = cparse.int_literal(a.dim)
+            if mlit == 1:
+                if ndim == 1:
+                    ctor = f"{ptr}"
+                elif ndim == 2:
+                    ctor = f"p.xd{i}, {ptr}"
+                else:
+                    ctor = f"p.xd{i}, p.yd{i}, {ptr}"
+            else:
+                if ndim == 1:
+                    ctor = f"p.md{i}, p.xd{i}, {ptr}"
+                elif ndim == 2:
+                    ctor = f"p.md{i}, p.xd{i}, p.yd{i}, {ptr}"
+                else:
+                    ctor = f"p.md{i}, p.xd{i}, p.yd{i}, p.zd{i}, {ptr}"
+            cq = "const " if pr.const else ""
+            L.append(f"    {cq}ACC<{t}> a{i}({ctor});")
+            call.append(f"a{i}")
+        elif a.kind == "gbl":
+            L.append(f"    const ops_gbl<{CTYPES.get(a.typ, a.typ)}, {cparse.int_literal(a.dim)}> g{i} = p.g{i};")
+            call.append(f"g{i}.v")
+        elif a.kind == "reduce":
+            call.append(f"r{i}")
+        else:
+            L.append(f"    int idx{i}[3] = {{p.st0 + x, p.st1 + y, p.st2 + z}};")
+            call.append(f"idx{i}")
+    L.append(f"    {loop.kernel}({', '.join(call)});")
+    L.append("  }")
+    for i in red:
+        a = loop.args[i]
+        t = CTYPES.get(a.typ, a.typ)
+        d = cparse.int_literal(a.dim)
+        L.append(f"  ops_reduce_finish<{RED_OP[a.acc]}<{t}>, {t}, {d}>(r{i}, p.rp{i}, p.rc{i}, p.rv{i}, p.ri{i});")
+    L.append("}")
+    return "\n".join(L)
+
+
+def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
+    k = loop.stub
+    n = len(loop.args)
+    sig = ", ".join(f"ops_arg arg{i}" for i in range(n))
+    L = [f"void ops_par_loop_{k}(char const *name, ops_block block, int dim, int *range"
+         + (f", {sig}" if n else "") + ") {"]
+    L.append(f"  static int kid = ops_hip_register_kernel(\"{loop.kernel}\");")
+    L.append(f"  ops_arg args[{max(n, 1)}] = {{{', '.join(f'arg{i}' for i in range(n)) or 'ops_arg_idx()'}}};")
+    L.append("  ops_hip_loop lp;")
+    L.append(f"  if (ops_hip_loop_begin(&lp, kid, name, block, dim, range, args, {n})) {{")
+    L.append(f"    {ns}::ops_kp_{k} p;")
+    L.append("    p.cnt0 = lp.count[0]; p.cnt1 = lp.count[1]; p.cnt2 = lp.count[2];")
+    L.append("    p.st0 = lp.start[0]; p.st1 = lp.start[1]; p.st2 = lp.start[2];")
+    for i, a in enumerate(loop.args):
+        if a.kind == "dat":
+            t = params[i].ctype
+            L.append(f"    p.p{i} = ({t} *)lp.ptr[{i}]; p.sx{i} = lp.step[{i}][0]; p.sy{i} = lp.step[{i}][1]; "
+                     f"p.sz{i} = lp.step[{i}][2];")
+            L.append(f"    p.xd{i} = lp.xdim[{i}]; p.yd{i} = lp.ydim[{i}]; p.zd{i} = lp.zdim[{i}]; p.md{i} = lp.mdim[{i}];")
+        elif a.kind == "gbl":
+            L.append(f"    memcpy(p.g{i}.v, arg{i}.data, sizeof(p.g{i}.v));")
+        elif a.kind == "reduce":
+            t = CTYPES.get(a.typ, a.typ)
+            L.append(f"    p.rp{i} = ({t} *)lp.red_part[{i}]; p.rc{i} = lp.red_count[{i}]; "
+                     f"p.rv{i} = ({t} *)lp.red_value[{i}]; p.ri{i} = lp.red_init[{i}];")
+    L.append(f"    hipLaunchKernelGGL({ns}::ops_kern_{k}, dim3(lp.grid[0], lp.grid[1], lp.grid[2]),")
+    L.append("                       dim3(lp.block[0], lp.block[1], lp.block[2]), 0, (hipStream_t)lp.stream, p);")
+    L.append("    ops_hip_check_launch(name);")
+    L.append("  }")
+    L.append(f"  ops_hip_loop_end(&lp, args, {n});")
+    L.append("}")
+    return "\n".join(L)
+
+
+def const_decl(c: Const) -> Tuple[str, str]:
+    t = CTYPES.get(c.typ, c.typ)
+    d = cparse.int_literal(c.dim)
+    if d is None:
+        return f"__device__ {t} *{c.name};", "-1"
+    if d == 1:
+        return f"__constant__ {t} {c.name};", f"(long)sizeof({t})"
+    return f"__constant__ {t} {c.name}[{d}];", f"(long)sizeof({t}) * {d}"
+
+
+def translate_app(app_name: str, sources: List[str], out_dir: str,
+                  app_dir: Optional[str] = None) -> Dict[str, object]:
+    """Translate the application made of `sources`; returns the generated file list."""
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    app_dir = app_dir or str(Path(sources[0]).parent)
+    headers = kernel_headers(app_dir, sources)
+    scanned = {s: scan_source(s) for s in sources}
+    ndims = {nd for (_, _, _, nd) in scanned.values() if nd is not None}
+    if len(ndims) > 1:
+        raise TranslateError(f"{app_name}: sources mix OPS_nD dimensionalities {ndims}")
+    ndim = ndims.pop() if ndims else 2
+    consts: Dict[str, Const] = {}
+    for (_, _, cs, _) in scanned.values():
+        for c in cs:
+            consts.setdefault(c.name, c)
+    # stubs: one per (kernel, argument signature)
+    variants: Dict[str, List[tuple]] = {}
+    loops_all: List[Loop] = []
+    for s in sources:
+        for lp in scanned[s][1]:
+            key = tuple(a.key() for a in lp.args)
+            vs = variants.setdefault(lp.kernel, [])
+            if key not in vs:
+                vs.append(key)
+            idx = vs.index(key)
+            lp.stub = lp.kernel if idx == 0 else f"{lp.kernel}__v{idx}"
+            loops_all.append(lp)
+    ns = f"ops_gen_{re.sub(r'[^A-Za-z0-9_]', '_', app_name)}"
+    # ---- kernels translation unit
+    used_headers = []
+    for lp in loops_all:
+        if lp.kernel not in headers:
+            raise TranslateError(f"{lp.file}:{lp.line}: kernel {lp.kernel} not found in any header of {app_dir}")
+        h = headers[lp.kernel][0]
+        if h not in used_headers:
+            used_headers.append(h)
+    hoisted, bodies = [], []
+    for h in used_headers:
+        text = Path(h).read_text()
+        keep = []
+        for line in text.splitlines():
+            if re.match(r"\s*#\s*include\b", line):
+                if line.strip() not in hoisted:
+                    hoisted.append(line.strip())
+                keep.append("")
+            else:
+                keep.append(line)
+        bodies.append((h, "\n".join(keep)))
+    H = [f"// Generated by ops_amd.translator (target hip_gfx950) for application '{app_name}'.",
+         "// Do not edit: regenerate from the application sources.",
+         f"#define OPS_{ndim}D",
+         "#include <hip/hip_runtime.h>",
+         "#include <string.h>",
+         "#include <math.h>",
+         "#include \"ops_lib_core.h\"",
+         "#include \"ops_hip_rt.h\"",
+         "#include \"ops_hip_kernel.h\""]
+    if (Path(app_dir) / "user_types.h").exists():
+        H.append("#include \"user_types.h\"")
+    H += hoisted
+    H.append("")
+    H.append(f"namespace {ns} {{")
+    H.append("")
+    H.append("// ---- ops_decl_const symbols")
+    regs = []
+    for c in consts.values():
+        decl, size = const_decl(c)
+        H.append(decl)
+        regs.append(f"    ops_hip_register_const(\"{c.name}\", (const void *)&{ns}::{c.name}, {size});")
+    H.append("")
+    H.append("// ---- user kernels, compiled for host and gfx950")
+    H.append("#pragma clang force_cuda_host_device begin")
+    for h, body in bodies:
+        H.append(f"// from {os.path.basename(h)}")
+        H.append(body)
+    H.append("#pragma clang force_cuda_host_device end")
+    H.append("")
+    done = set()
+    stub_defs = []
+    for lp in loops_all:
+        if lp.stub in done:
+            continue
+        done.add(lp.stub)
+        fdef = headers[lp.kernel][1]
+        params = [parse_param(p) for p in fdef.params]
+        if len(params) != len(lp.args):
+            raise TranslateError(f"{lp.file}:{lp.line}: kernel {lp.kernel} takes {len(params)} parameters "
+                                 f"but the loop passes {len(lp.args)} arguments")
+        H.append(gen_kernel(lp, params, ndim))
+        H.append("")
+        stub_defs.append(gen_stub(lp, params, ns))
+    H.append(f"}}  // namespace {ns}")
+    H.append("")
+    H += stub_defs
+    H.append("")
+    H.append("namespace {")
+    H.append("struct ops_const_registration {")
+    H.append("  ops_const_registration() {")
+    H += regs
+    H.append("  }")
+    H.append("} ops_const_registration_instance;")
+    H.append("}  // namespace")
+    kpath = out / f"{app_name}_kernels.hip"
+    _write_if_changed(kpath, "\n".join(H) + "\n")
+    # ---- rewritten host sources
+    host_files = []
+    for s in sources:
+        text, loops, _, _ = scanned[s]
+        new = text
+        for lp in sorted(loops, key=lambda l: l.call.start, reverse=True):
+            c = lp.call
+            # drop the kernel argument and rename the callee
+            first_end = c.arg_spans[1][0] if len(c.arg_spans) > 1 else c.close
+            new = new[:c.start] + f"ops_par_loop_{lp.stub}(" + new[first_end:]
+        protos = []
+        seen = set()
+        for lp in loops:
+            if lp.stub in seen:
+                continue
+            seen.add(lp.stub)
+            protos.append(f"void ops_par_loop_{lp.stub}(char const *, ops_block, int, int *"
+                          + ", ops_arg" * len(lp.args) + ");")
+        if protos:
+            lines = new.split("\n")
+            at = None
+            for i, line in enumerate(lines):
+                if re.match(r'\s*#\s*include\s*[<"]ops_seq(_v2)?\.h[>"]', line):
+                    at = i + 1
+            if at is None:
+                for i, line in enumerate(lines):
+                    if re.match(r"\s*#\s*include\b", line):
+                        at = i + 1
+            at = at or 0
+            lines[at:at] = ["/* ops_par_loop declarations (hip_gfx950) */"] + protos
+            new = "\n".join(lines)
+        hp = out / (Path(s).stem + "_ops.cpp")
+        _write_if_changed(hp, new)
+        host_files.append(str(hp))
+    return {"kernels": str(kpath), "host": host_files, "ndim": ndim,
+            "loops": [(lp.kernel, lp.stub, lp.file, lp.line) for lp in loops_all],
+            "consts": list(consts)}
+
+
+def _write_if_changed(path: Path, text: str):
+    if path.exists() and path.read_text() == text:
+        return
+    path.write_text(text)
