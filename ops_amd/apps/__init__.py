@@ -11,6 +11,15 @@ SIGNATURES = {
         "laplace2d_fetch_full": (_i, [_p]),
         "laplace2d_finalize": (_i, []),
     },
+    "cloverleaf": {
+        "cloverleaf_init": (_i, [_i, _pp, ctypes.c_char_p]),
+        "cloverleaf_step": (_i, [_i]),
+        "cloverleaf_run": (_i, [_i]),
+        "cloverleaf_summary": (_i, [_p]),
+        "cloverleaf_state": (_i, [_p]),
+        "cloverleaf_fetch": (ctypes.c_long, [ctypes.c_char_p, _p]),
+        "cloverleaf_finalize": (_i, []),
+    },
 }
 
 

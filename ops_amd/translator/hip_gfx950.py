@@ -138,6 +138,16 @@ def kernel_headers(app_dir: str, sources: List[str]) -> Dict[str, Tuple[str, cpa
 
 # ------------------------------------------------------------------ code generation
 
+def const_dim(a: Arg, loop: Loop) -> str:
+    """Dimension of a gbl/reduce argument as a compile-time expression: an integer
+    literal or a macro / constant the kernels translation unit can see."""
+    d = a.dim.strip()
+    if cparse.int_literal(d) is not None or re.fullmatch(r"[A-Za-z_]\w*", d):
+        return d
+    raise TranslateError(f"{loop.file}:{loop.line}: argument dim '{d}' must be a literal or a "
+                         "compile-time constant name")
+
+
 def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
     k = loop.stub
     L = []
@@ -149,10 +159,7 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
             t = params[i].ctype
             L.append(f"  {t} *p{i}; int sx{i}, sy{i}, sz{i}, xd{i}, yd{i}, zd{i}, md{i};")
         elif a.kind == "gbl":
-            d = cparse.int_literal(a.dim)
-            if d is None:
-                raise TranslateError(f"{loop.file}:{loop.line}: ops_arg_gbl dim must be a literal")
-            L.append(f"  ops_gbl<{CTYPES.get(a.typ, a.typ)}, {d}> g{i};")
+            L.append(f"  ops_gbl<{CTYPES.get(a.typ, a.typ)}, {const_dim(a, loop)}> g{i};")
         elif a.kind == "reduce":
             t = CTYPES.get(a.typ, a.typ)
             L.append(f"  {t} *rp{i}; unsigned *rc{i}; {t} *rv{i}; int ri{i};")
@@ -168,9 +175,7 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
     for i in red:
         a = loop.args[i]
         t = CTYPES.get(a.typ, a.typ)
-        d = cparse.int_literal(a.dim)
-        if d is None:
-            raise TranslateError(f"{loop.file}:{loop.line}: ops_arg_reduce dim must be a literal")
+        d = const_dim(a, loop)
         L.append(f"  {t} r{i}[{d}];")
         L.append(f"  for (int d = 0; d < {d}; d++) r{i}[d] = {RED_OP[a.acc]}<{t}>::identity();")
     L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
@@ -203,7 +208,7 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
             L.append(f"    {cq}ACC<{t}> a{i}({ctor});")
             call.append(f"a{i}")
         elif a.kind == "gbl":
-            L.append(f"    const ops_gbl<{CTYPES.get(a.typ, a.typ)}, {cparse.int_literal(a.dim)}> g{i} = p.g{i};")
+            L.append(f"    const ops_gbl<{CTYPES.get(a.typ, a.typ)}, {const_dim(a, loop)}> g{i} = p.g{i};")
             call.append(f"g{i}.v")
         elif a.kind == "reduce":
             call.append(f"r{i}")
@@ -215,7 +220,7 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
     for i in red:
         a = loop.args[i]
         t = CTYPES.get(a.typ, a.typ)
-        d = cparse.int_literal(a.dim)
+        d = const_dim(a, loop)
         L.append(f"  ops_reduce_finish<{RED_OP[a.acc]}<{t}>, {t}, {d}>(r{i}, p.rp{i}, p.rc{i}, p.rv{i}, p.ri{i});")
     L.append("}")
     return "\n".join(L)

@@ -1,0 +1,188 @@
+/*
+ * ORACLE — test infrastructure only. CPU execution of ops_par_loop for the oracle build
+ * of the applications (oracle/ops_cpu.cpp supplies the rest of the API on host memory).
+ *
+ * Restates the reference's sequential loop semantics, ops/c/include/ops_seq_v2.h:55-335:
+ *   - ACC views built at the loop start from address() (:55-62): per dimension
+ *     elem * cumulative size * (start*stride - base - d_m);
+ *   - every index of the half-open range is visited, x fastest;
+ *   - ops_arg_gbl READ passes the user's pointer, ops_arg_reduce the handle's running
+ *     value (:117-126), ops_arg_idx the loop index (:127-134);
+ *   - writes mark the dat host-dirty (:299-301).
+ * The outer loop (y in 2D, y*z in 3D) is split over OpenMP threads, as the reference's
+ * mpi_openmp target does (templates/cpp/mpi_openmp/loop_host.cpp.j2:150-330); reductions
+ * go to per-thread copies combined in thread order, and with one thread the kernel
+ * accumulates straight into the handle exactly as the sequential path does.
+ */
+#ifndef ORACLE_OPS_SEQ_V2_H
+#define ORACLE_OPS_SEQ_V2_H
+
+#include <string.h>
+
+#include <new>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "ops_lib_core.h"
+#include "ops_tridiag.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" void ops_cpu_before_loop(ops_arg *args, int nargs);
+extern "C" void ops_cpu_after_loop(ops_arg *args, int nargs);
+extern "C" int ops_cpu_soa(void);
+
+namespace ops_cpu {
+
+constexpr int MAXRED = 64;  // doubles of private reduction storage per argument
+
+struct Ctx {
+  int idx[OPS_MAX_DIM];
+  double red[MAXRED];
+  alignas(16) unsigned char acc[64];  // the ACC<T> view handed to the kernel (an lvalue)
+};
+
+template <typename P> struct Param;
+
+template <typename T> struct Param<ACC<T>> {
+  struct St {
+    T *base;
+    long sx, sy, sz;
+    int xd, yd, zd, md;
+  };
+  static St prepare(const ops_arg &a, const int *start, int ndim) {
+    St s{};
+    if (a.argtype != OPS_ARG_DAT || !a.opt) return s;
+    ops_dat d = a.dat;
+    int soa = ops_cpu_soa();
+    long esz = soa ? d->type_size : d->elem_size;
+    long off = 0, cum = 1;
+    for (int i = 0; i < ndim; i++) {
+      off += esz * cum * ((long)start[i] * a.stencil->stride[i] - d->base[i] - d->d_m[i]);
+      cum *= d->size[i];
+    }
+    s.base = (T *)(d->data + off);
+    long comp = soa ? 1 : d->dim;
+    s.sx = a.stencil->stride[0] * comp;
+    s.sy = ndim > 1 ? (long)a.stencil->stride[1] * d->size[0] * comp : 0;
+    s.sz = ndim > 2 ? (long)a.stencil->stride[2] * d->size[0] * d->size[1] * comp : 0;
+    s.xd = d->size[0];
+    s.yd = d->size[1];
+    s.zd = d->size[2];
+    s.md = d->dim;
+    return s;
+  }
+  static ACC<T> &get(const St &s, const ops_arg &, Ctx &c, const int *lc, const int *, int) {
+    static_assert(sizeof(ACC<T>) <= sizeof(c.acc), "ACC view does not fit its slot");
+    T *p = s.base ? s.base + (lc[0] * s.sx + lc[1] * s.sy + lc[2] * s.sz) : nullptr;
+#if defined(OPS_1D)
+    return *(s.md == 1 ? new (c.acc) ACC<T>(p) : new (c.acc) ACC<T>(s.md, s.xd, p));
+#elif defined(OPS_2D)
+    return *(s.md == 1 ? new (c.acc) ACC<T>(s.xd, p) : new (c.acc) ACC<T>(s.md, s.xd, s.yd, p));
+#else
+    return *(s.md == 1 ? new (c.acc) ACC<T>(s.xd, s.yd, p)
+                       : new (c.acc) ACC<T>(s.md, s.xd, s.yd, s.zd, p));
+#endif
+  }
+};
+
+template <typename T> struct Param<T *> {
+  struct St {};
+  static St prepare(const ops_arg &, const int *, int) { return St{}; }
+  static T *get(const St &, const ops_arg &a, Ctx &c, const int *lc, const int *start,
+                int direct) {
+    if (a.argtype == OPS_ARG_IDX) {
+      for (int d = 0; d < 3; d++) c.idx[d] = start[d] + lc[d];
+      return (T *)c.idx;
+    }
+    if (a.acc == OPS_READ) return (T *)a.data;
+    if (direct) return (T *)((ops_reduction)a.data)->data;
+    return (T *)c.red;
+  }
+};
+
+template <typename P> using base_t = typename std::remove_cv<typename std::remove_reference<P>::type>::type;
+template <typename P> struct Strip { using type = base_t<P>; };
+template <typename T> struct Strip<const T *> { using type = T *; };
+template <typename T> struct Strip<T *> { using type = T *; };
+
+void red_init(const ops_arg &a, double *buf);
+void red_combine(const ops_arg &a, const double *buf);
+
+template <typename... P, size_t... I>
+void run(std::index_sequence<I...>, void (*kernel)(P...), ops_block block, int dim, int *range,
+         ops_arg *args) {
+  constexpr int N = sizeof...(P);
+  int start[3] = {0, 0, 0}, cnt[3] = {1, 1, 1};
+  for (int d = 0; d < dim && d < 3; d++) {
+    start[d] = range[2 * d];
+    cnt[d] = range[2 * d + 1] - range[2 * d];
+    if (cnt[d] <= 0) return;
+  }
+  (void)block;
+  ops_cpu_before_loop(args, N);
+  auto st = std::make_tuple(Param<typename Strip<P>::type>::prepare(args[I], start, dim)...);
+  bool has_red = false;
+  for (int i = 0; i < N; i++)
+    if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) has_red = true;
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+  if ((long)cnt[1] * cnt[2] < nth) nth = 1;
+#endif
+  const long outer = (long)cnt[1] * cnt[2];
+  if (nth == 1) {
+    Ctx c[N > 0 ? N : 1];
+    for (long o = 0; o < outer; o++) {
+      int lc[3] = {0, (int)(o % cnt[1]), (int)(o / cnt[1])};
+      for (lc[0] = 0; lc[0] < cnt[0]; lc[0]++)
+        kernel(Param<typename Strip<P>::type>::get(std::get<I>(st), args[I], c[I], lc, start,
+                                                   1)...);
+    }
+  } else {
+    std::vector<Ctx> ctx((size_t)nth * (N > 0 ? N : 1));
+    if (has_red)
+      for (int t = 0; t < nth; t++)
+        for (int i = 0; i < N; i++)
+          if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ)
+            red_init(args[i], ctx[(size_t)t * N + i].red);
+#pragma omp parallel num_threads(nth)
+    {
+      int t = 0;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+#endif
+      Ctx *c = &ctx[(size_t)t * (N > 0 ? N : 1)];
+#pragma omp for schedule(static)
+      for (long o = 0; o < outer; o++) {
+        int lc[3] = {0, (int)(o % cnt[1]), (int)(o / cnt[1])};
+        for (lc[0] = 0; lc[0] < cnt[0]; lc[0]++)
+          kernel(Param<typename Strip<P>::type>::get(std::get<I>(st), args[I], c[I], lc, start,
+                                                     0)...);
+      }
+    }
+    if (has_red)
+      for (int t = 0; t < nth; t++)
+        for (int i = 0; i < N; i++)
+          if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ)
+            red_combine(args[i], ctx[(size_t)t * N + i].red);
+  }
+  ops_cpu_after_loop(args, N);
+}
+
+}  // namespace ops_cpu
+
+template <typename... P, typename... A>
+void ops_par_loop(void (*kernel)(P...), char const *name, ops_block block, int dim, int *range,
+                  A... a) {
+  static_assert(sizeof...(P) == sizeof...(A), "kernel parameters and ops_args differ in number");
+  (void)name;
+  ops_arg args[sizeof...(A) > 0 ? sizeof...(A) : 1] = {a...};
+  ops_cpu::run(std::index_sequence_for<P...>{}, kernel, block, dim, range, args);
+}
+
+#endif

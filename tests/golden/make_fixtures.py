@@ -83,8 +83,74 @@ def laplace():
     (OUT / "laplace_step7.json").write_text(json.dumps(fx, indent=1))
 
 
+
+
+CLOVER_DECKS = {
+    # small rectangle problem stopped by end_time after 14 steps
+    "clover_small": """*clover
+ state 1 density=0.2 energy 1.0
+ state 2 density=1.0 energy=2.5 geometry=rectangle xmin=0.0 xmax=5.0 ymin=0.0 ymax=2.0
+ x_cells=48
+ y_cells=40
+ xmin=0.0
+ ymin=0.0
+ xmax=10.0
+ ymax=10.0
+ initial_timestep=0.04
+ timestep_rise=1.5
+ max_timestep=0.04
+ end_time=0.55
+ end_step=25
+ summary_frequency=0
+ test_problem 0
+*endclover
+""",
+    # apps/c/CloverLeaf/clover.in_default verbatim (known answer ke 1.82280367310258)
+    "clover_default": None,
+    # apps/c/CloverLeaf/clover_bm_short.in verbatim (known answer ke 1.19316898756307)
+    "clover_bm_short": None,
+    # apps/c/CloverLeaf/clover_bm16_short.in verbatim (3840^2, 87 steps, ke 0.307475452287895)
+    "clover_bm16_short": None,
+}
+CLOVER_FILES = {"clover_default": "clover.in_default", "clover_bm_short": "clover_bm_short.in",
+                "clover_bm16_short": "clover_bm16_short.in"}
+
+
+def _dump_fixture(dats, reds):
+    out = {}
+    for k, v in dats.items():
+        arr = v["data"]
+        out[k] = {"shape": list(arr.shape), "dtype": str(arr.dtype),
+                  "sha256": hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest(),
+                  "sum": float(np.asarray(arr, dtype=np.float64).sum())}
+    return {"dats": out, "reductions": {k: {"hex": v.tobytes().hex(), "values": v.tolist()}
+                                        for k, v in reds.items()}}
+
+
+def clover(which=None):
+    sys.path.insert(0, str(ROOT / "tests"))
+    from dumps import read_dump, read_reductions as rr
+    names = [which] if which else ["clover_small", "clover_default", "clover_bm_short"]
+    for name in names:
+        deck = CLOVER_DECKS[name]
+        if deck is None:
+            deck = Path("/root/reference/apps/c/CloverLeaf", CLOVER_FILES[name]).read_text()
+        with tempfile.TemporaryDirectory() as t:
+            out = run_ref("cloverleaf_dev_seq", t, deck=deck, deck_name="clover.in")
+            fx = _dump_fixture(read_dump(t), rr(t))
+            steps = (Path(t) / "clover.out").read_text().count(" Step ")
+        fx.update({"source": "apps/c/CloverLeaf (reference dev_seq)", "deck": deck,
+                   "steps": steps, "tail": out.splitlines()[-6:]})
+        (OUT / f"{name}.json").write_text(json.dumps(fx, indent=1))
+        print("wrote", name, steps, "steps")
+
+
+def clover_big():
+    clover("clover_bm16_short")
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["laplace"]
+    which = sys.argv[1:] or ["laplace", "clover"]
     for w in which:
         globals()[w]()
-        print("wrote", w)
+        print("done", w)
