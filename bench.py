@@ -1,0 +1,207 @@
+"""Benchmark: cell-updates/s and HBM roofline fraction of the OPS hot path on one MI355X.
+
+Default workload: CloverLeaf 2D, deck clover_bm16_short.in (3840 x 3840 cells, 87 hydro
+steps, FP64), through libops_hip.so and the translated application. A step is one hydro
+time step of the whole mesh; a cell-update is one cell advanced by one step.
+
+    python bench.py [--steps K] [--warmup W] [--workload cloverleaf|laplace]
+                    [--no-cpu] [--cpu-threads T]
+
+Prints one JSON line. `value` is measured with the inputs resident in HBM: the deck is
+generated on the device before the timed region; the K timed steps are bracketed by
+device synchronisation. Per-kernel durations come from HIP events recorded on the
+runtime's stream around every launch inside the timed region; `roofline` is computed for
+the kernel with the largest total time from its ops_compute_transfer bytes per launch.
+`cpu_baseline` runs the same application on the oracle's host backend (OpenMP) for a
+bounded sample on this machine's cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "cell-updates/s + % HBM roofline; TeaLeaf time-to-tol learned+mixed vs FP64 CG"
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+GOLD_KE_BM16 = 0.307475452287895  # apps/c/CloverLeaf/field_summary.cpp:94
+
+BM16_DECK = """*clover
+ state 1 density=0.2 energy 1.0
+ state 2 density=1.0 energy=2.5 geometry=rectangle xmin=0.0 xmax=5.0 ymin=0.0 ymax=2.0
+ x_cells={n}
+ y_cells={n}
+ xmin=0.0
+ ymin=0.0
+ xmax=10.0
+ ymax=10.0
+ initial_timestep=0.04
+ timestep_rise=1.5
+ max_timestep=0.04
+ end_time=0.55
+ end_step=87
+ test_problem 4
+*endclover
+"""
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    committed under profiles/ (null when not measured for this kernel)."""
+    p = ROOT / "profiles" / "pmc_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        e = d.get("kernels", {}).get(kernel)
+        return None if e is None else float(e["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def roofline(stats):
+    k = max(stats, key=lambda n: stats[n]["ms"])
+    e = stats[k]
+    per_launch_bytes = e["bytes"] / e["calls"]
+    avg_ms = e["ms"] / e["calls"]
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    return k, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+               "traffic": traffic_from_profile(k), "kernel": k,
+               "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_ms * 1e3, 2)}
+
+
+def whole_step_bw(stats, seconds):
+    tot_bytes = sum(e["bytes"] for e in stats.values())
+    tot_ms = sum(e["ms"] for e in stats.values())
+    return {"algorithmic_GB": round(tot_bytes / 1e9, 3),
+            "kernel_busy_s": round(tot_ms * 1e-3, 4),
+            "GBps_over_kernel_time": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1) if tot_ms else None,
+            "GBps_over_wall": round(tot_bytes / seconds / 1e9, 1)}
+
+
+def clover_gpu(args):
+    import ops_amd
+    L = ops_amd.app("cloverleaf")
+    R = ops_amd.runtime()
+    n = args.cells
+    with tempfile.TemporaryDirectory() as t:
+        deck = Path(t) / "clover.in"
+        deck.write_text(BM16_DECK.format(n=n))
+        argc, argv = ops_amd.argv()
+        if args.warmup > 0:
+            log(f"[bench] warmup: {args.warmup} steps")
+            assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+            assert L.cloverleaf_step(args.warmup) == 0
+            assert L.cloverleaf_finalize() == 0
+        log("[bench] generating the deck on the device")
+        assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+        R.ops_hip_device_sync()
+        R.ops_hip_reset_stats()
+        R.ops_hip_set_timing(1)
+        log(f"[bench] timed: {args.steps} steps")
+        t0 = time.perf_counter()
+        rc = L.cloverleaf_step(args.steps)
+        R.ops_hip_device_sync()
+        t1 = time.perf_counter()
+        R.ops_hip_set_timing(0)
+        assert rc == 0
+        stats = ops_amd.kernel_stats()
+        s = (ctypes.c_double * 5)()
+        assert L.cloverleaf_summary(s) == 0
+        assert L.cloverleaf_finalize() == 0
+    secs = t1 - t0
+    checks = {"ke": s[3]}
+    if args.steps == 87 and n == 3840:
+        checks["ke_golden"] = GOLD_KE_BM16
+        checks["ke_pct_diff"] = abs(100.0 * (s[3] / GOLD_KE_BM16) - 100.0)
+        checks["ke_pass_0.001pct"] = checks["ke_pct_diff"] < 0.001
+    return secs, stats, checks
+
+
+def clover_cpu(args, threads):
+    """Oracle host backend, same application and deck, bounded sample of steps."""
+    lib = ctypes.CDLL(str(ROOT / "oracle" / "_build" / "libcloverleaf_cpu.so"),
+                      mode=ctypes.RTLD_LOCAL)
+    lib.ops_cpu_set_threads.argtypes = [ctypes.c_int]
+    lib.ops_cpu_set_threads(threads)
+    import ops_amd
+    lib.cloverleaf_init.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p]
+    lib.cloverleaf_step.argtypes = [ctypes.c_int]
+    n = args.cells
+    steps = args.cpu_steps
+    with tempfile.TemporaryDirectory() as t:
+        deck = Path(t) / "clover.in"
+        deck.write_text(BM16_DECK.format(n=n))
+        argc, argv = ops_amd.argv()
+        assert lib.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+        assert lib.cloverleaf_step(1) == 0  # warm-up step (page faults, first touch)
+        t0 = time.perf_counter()
+        assert lib.cloverleaf_step(steps) == 0
+        t1 = time.perf_counter()
+        assert lib.cloverleaf_finalize() == 0
+    return {"value": n * n * steps / (t1 - t0), "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"CloverLeaf {n}x{n}, steps 2..{steps + 1} of clover_bm16_short.in on the "
+                      f"oracle host backend (OpenMP, {threads} threads), {t1 - t0:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=87)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cloverleaf", choices=["cloverleaf"])
+    ap.add_argument("--cells", type=int, default=3840)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the mesh would be decomposed)", file=sys.stderr)
+        sys.exit(2)
+    secs, stats, checks = clover_gpu(args)
+    n = args.cells
+    value = n * n * args.steps / secs
+    kname, roof = roofline(stats)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"cloverleaf_2d clover_bm16_short.in {n}x{n} cells, "
+                               f"{args.steps} hydro steps", "cells": n * n,
+                   "halo_depth": 2, "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_bandwidth": whole_step_bw(stats, secs),
+        "checks": checks,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"[bench] cpu baseline: {threads} threads, {args.cpu_steps} steps")
+        out["cpu_baseline"] = clover_cpu(args, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

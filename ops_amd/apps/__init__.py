@@ -20,6 +20,16 @@ SIGNATURES = {
         "cloverleaf_fetch": (ctypes.c_long, [ctypes.c_char_p, _p]),
         "cloverleaf_finalize": (_i, []),
     },
+    "tealeaf": {
+        "tealeaf_init": (_i, [_i, _pp, ctypes.c_char_p]),
+        "tealeaf_set_solver": (_i, [_i]),
+        "tealeaf_step": (_i, [_i]),
+        "tealeaf_run": (_i, [_i]),
+        "tealeaf_summary": (_i, [_p]),
+        "tealeaf_state": (_i, [_p]),
+        "tealeaf_fetch": (ctypes.c_long, [ctypes.c_char_p, _p]),
+        "tealeaf_finalize": (_i, []),
+    },
 }
 
 

@@ -1,0 +1,34 @@
+// Device-resident CG solvers for the TeaLeaf implicit step (no reference counterpart).
+//
+// Both solve A u = u0 with A = I + rx*Dx(Kx) + ry*Dy(Ky), the 5-point operator of
+// tea_leaf_cg_calc_w_reduce_kernel, on the interior of the mesh, starting from the
+// residual r = u0 - A u already computed by the caller. They stop when sqrt(r.r) falls
+// below eps * initial_residual, the reference's test (apps/c/TeaLeaf/tea_solve.cpp:349-355).
+//   solver 1: FP64 CG; alpha/beta and the stop test stay on the GPU (no per-iteration
+//             host round trip), p-update fused into the operator application.
+//   solver 2: FP32 storage and arithmetic for the inner CG iterations inside FP64
+//             iterative refinement: the residual recomputation, its norm and the
+//             solution update run in FP64; the stop test uses the FP64 residual.
+#ifndef TEALEAF_NATIVE_H
+#define TEALEAF_NATIVE_H
+
+#include "ops_lib_core.h"
+
+typedef struct {
+  int solver;
+  int nx, ny;
+  double rx, ry, eps;
+  int max_iters;
+  double initial_residual;
+} tea_native_solve_params;
+
+typedef struct {
+  int iterations;     // CG iterations (all inner iterations for solver 2)
+  int outer;          // refinement steps (solver 2), 1 otherwise
+  double error;       // sqrt(r.r) at exit
+} tea_native_result;
+
+void tea_native_cg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat z, ops_dat Kx,
+                   ops_dat Ky, const tea_native_solve_params *P, tea_native_result *R);
+
+#endif

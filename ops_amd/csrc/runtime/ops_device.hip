@@ -207,8 +207,10 @@ extern "C" int ops_hip_register_kernel(const char *name) {
 extern "C" void ops_hip_set_block_shape(int kernel_id, int bx, int by, int bz) {
   auto &kt = ops_kernel_table();
   if (kernel_id < 0 || kernel_id >= (int)kt.size()) return;
-  if (bx * by * bz > OPS_HIP_BLOCK_THREADS || bx <= 0 || by <= 0 || bz <= 0)
-    ops_throw(OPS_INVALID_ARGUMENT, "ops_hip_set_block_shape: workgroup must have <= 256 threads");
+  if (bx * by * bz > OPS_HIP_BLOCK_THREADS || bx <= 0 || by <= 0 || bz <= 0 ||
+      (bx * by * bz) % OPS_WAVE != 0)
+    ops_throw(OPS_INVALID_ARGUMENT,
+              "ops_hip_set_block_shape: workgroup must be whole 64-lane wavefronts, <= 256 threads");
   kt[kernel_id].bx = bx;
   kt[kernel_id].by = by;
   kt[kernel_id].bz = bz;
@@ -334,16 +336,28 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
     by = kt[kernel_id].by;
     bz = kt[kernel_id].bz;
   } else {
+    // always a full 256-lane workgroup (four whole wavefronts): the wavefront reductions
+    // exchange values across all 64 lanes, so a partial wavefront would read lanes that
+    // do not exist. Lanes outside the range are masked by the kernel's bounds test.
     bx = pow2_at_least(lp->count[0]);
     if (bx > 64) bx = 64;
     if (in->block_size_x > 0) bx = in->block_size_x;
-    by = pow2_at_least(lp->count[1]);
-    if (by > OPS_HIP_BLOCK_THREADS / bx) by = OPS_HIP_BLOCK_THREADS / bx;
-    if (in->block_size_y > 0) by = in->block_size_y;
-    bz = pow2_at_least(lp->count[2]);
-    if (bz > OPS_HIP_BLOCK_THREADS / (bx * by)) bz = OPS_HIP_BLOCK_THREADS / (bx * by);
-    if (bz < 1) bz = 1;
+    if (dim >= 3) {
+      by = pow2_at_least(lp->count[1]);
+      if (by > OPS_HIP_BLOCK_THREADS / bx) by = OPS_HIP_BLOCK_THREADS / bx;
+      if (in->block_size_y > 0) by = in->block_size_y;
+      bz = OPS_HIP_BLOCK_THREADS / (bx * by);
+      if (bz < 1) bz = 1;
+    } else {
+      by = OPS_HIP_BLOCK_THREADS / bx;
+      if (in->block_size_y > 0) by = in->block_size_y;
+      bz = 1;
+    }
   }
+  if ((bx * by * bz) % OPS_WAVE != 0 || bx * by * bz > OPS_HIP_BLOCK_THREADS)
+    ops_throw(OPS_INVALID_ARGUMENT,
+              "ops_par_loop: workgroup shape must be a whole number of 64-lane wavefronts "
+              "and at most 256 threads");
   lp->block[0] = bx;
   lp->block[1] = by;
   lp->block[2] = bz;

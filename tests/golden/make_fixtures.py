@@ -149,6 +149,29 @@ def clover_big():
     clover("clover_bm16_short")
 
 
+def tea():
+    sys.path.insert(0, str(ROOT / "tests"))
+    sys.path.insert(0, str(ROOT))
+    from dumps import read_dump, read_reductions as rr
+    from ops_amd.apps.tealeaf.decks import tea_deck
+    decks = {
+        "tea_cg_64": tea_deck(64, 3),
+        "tea_cg_200": tea_deck(200, 2),
+        # apps/c/TeaLeaf/tea.in verbatim: Jacobi, 10x10, 10 steps (temp 157.55084183279294)
+        "tea_jacobi_ref": Path("/root/reference/apps/c/TeaLeaf/tea.in").read_text(),
+    }
+    for name, deck in decks.items():
+        with tempfile.TemporaryDirectory() as t:
+            out = run_ref("tealeaf_dev_seq", t, deck=deck, deck_name="tea.in")
+            fx = _dump_fixture(read_dump(t), rr(t))
+            text = (Path(t) / "tea.out").read_text()
+        iters = [int(l.split()[-1]) for l in text.splitlines() if "Iteration count" in l]
+        fx.update({"source": "apps/c/TeaLeaf (reference dev_seq)", "deck": deck,
+                   "iteration_counts_printed": iters, "tail": out.splitlines()[-4:]})
+        (OUT / f"{name}.json").write_text(json.dumps(fx, indent=1))
+        print("wrote", name, iters)
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["laplace", "clover"]
     for w in which:

@@ -1,0 +1,150 @@
+"""TeaLeaf 2D: oracle pinned to the reference (bitwise CG and Jacobi runs, known answer,
+dense-solve check of the operator); GPU plain path vs the reference; device-resident and
+mixed-precision solvers vs the plain path."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import tea_util as tu
+from dumps import read_dump
+
+GOLD_TEMP = 157.55084183279294  # apps/c/TeaLeaf/field_summary.cpp:84 (tea.in)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def check_dump(d, fx):
+    got = read_dump(d)
+    bad = []
+    for name, meta in fx["dats"].items():
+        if name not in got:
+            continue  # dats of solvers this build does not provide (rstore, Mi, tri_*)
+        arr = got[name]["data"]
+        if list(arr.shape) != meta["shape"] or sha(arr) != meta["sha256"]:
+            bad.append(name)
+    return bad
+
+
+def ref_summary(fx):
+    r = fx["reductions"]
+    return np.array([r[k]["values"][0] for k in ("vol", "mass", "ie", "temp")])
+
+
+def rx_of(deck):
+    nx, _ = tu.deck_cells(deck)
+    dt = 0.1
+    for line in deck.splitlines():
+        t = line.replace("=", " ").split()
+        if t and t[0] == "initial_timestep":
+            dt = float(t[1])
+    dx = 10.0 / nx
+    return dt / (dx * dx)
+
+
+# ---------------------------------------------------------------- oracle pins (CPU)
+
+@pytest.mark.parametrize("name", ["tea_cg_64", "tea_cg_200", "tea_jacobi_ref"])
+def test_oracle_bitwise_equals_reference(golden, tmp_path, name):
+    fx = golden(f"{name}.json")
+    out = tu.run("cpu", fx["deck"], dump_dir=str(tmp_path), threads=1)
+    assert check_dump(tmp_path, fx) == []
+    # the reference prints n-1 for a solve that stopped at iteration n
+    assert [int(s[2]) - 1 for s in out["per_step"]] == fx["iteration_counts_printed"]
+    assert np.array_equal(out["summary"], ref_summary(fx))
+
+
+def test_oracle_known_answer_jacobi(golden):
+    fx = golden("tea_jacobi_ref.json")
+    out = tu.run("cpu", fx["deck"], threads=1)
+    assert abs(100.0 * (out["summary"][3] / GOLD_TEMP) - 100.0) < 0.001
+
+
+def test_oracle_cg_matches_dense_solve():
+    # one implicit step on a 16x16 mesh: CG converged to 1e-14 against a direct solve of
+    # the same 5-point system assembled independently from Kx, Ky
+    from ops_amd.apps.tealeaf.decks import tea_deck
+    n = 16
+    deck = tea_deck(n, 1, eps=1e-14, max_iters=5000)
+    out = tu.run("cpu", deck, threads=1)
+    f = out["fields"]
+    sx = n + 2
+    Kx = f["vector_Kx"].reshape(-1, sx)
+    Ky = f["vector_Ky"].reshape(-1, sx)
+    u0 = f["u0"].reshape(-1, sx)[1:n + 1, 1:n + 1].ravel()
+    rx = ry = rx_of(deck)
+    A = np.zeros((n * n, n * n))
+    for y in range(n):
+        for x in range(n):
+            k = x + y * n
+            X, Y = x + 1, y + 1
+            A[k, k] = 1.0 + ry * (Ky[Y + 1, X] + Ky[Y, X]) + rx * (Kx[Y, X + 1] + Kx[Y, X])
+            if x + 1 < n:
+                A[k, k + 1] = -rx * Kx[Y, X + 1]
+            if x > 0:
+                A[k, k - 1] = -rx * Kx[Y, X]
+            if y + 1 < n:
+                A[k, k + n] = -ry * Ky[Y + 1, X]
+            if y > 0:
+                A[k, k - n] = -ry * Ky[Y, X]
+    assert np.allclose(A, A.T)  # symmetric: the face conductivity is shared
+    u = np.linalg.solve(A, u0)
+    got = f["u"].reshape(-1, sx)[1:n + 1, 1:n + 1].ravel()
+    assert np.max(np.abs(got - u) / np.abs(u)) < 1e-10
+
+
+def test_oracle_cg_true_residual_meets_tolerance():
+    from ops_amd.apps.tealeaf.decks import tea_deck
+    deck = tea_deck(96, 2)
+    out = tu.run("cpu", deck, threads=4)
+    r = tu.operator_residual(out["fields"], 96, 96, rx_of(deck), rx_of(deck))
+    st = out["per_step"][-1]
+    assert np.sqrt(np.sum(r * r)) < 2e-10 * st[4]
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+def test_oracle_native_solvers_agree_with_plain_cg(solver):
+    from ops_amd.apps.tealeaf.decks import tea_deck
+    deck = tea_deck(96, 2)
+    ref = tu.run("cpu", deck, solver=0, threads=4)
+    got = tu.run("cpu", deck, solver=solver, threads=4)
+    rx = rx_of(deck)
+    r = tu.operator_residual(got["fields"], 96, 96, rx, rx)
+    assert np.sqrt(np.sum(r * r)) < 1.5e-10 * got["per_step"][-1][4]
+    u0, u1 = ref["fields"]["u"], got["fields"]["u"]
+    assert np.max(np.abs(u1 - u0)) / np.max(np.abs(u0)) < 1e-8
+
+
+# ---------------------------------------------------------------- GPU path
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["tea_cg_64", "tea_cg_200", "tea_jacobi_ref"])
+def test_gpu_plain_path_vs_reference(golden, tmp_path, name):
+    fx = golden(f"{name}.json")
+    out = tu.run("gpu", fx["deck"], dump_dir=str(tmp_path))
+    got = read_dump(tmp_path)
+    # set-up loops are exact; the solves differ only through the order of their sums
+    for k in ("density", "volume", "xarea", "yarea", "cellx", "celly"):
+        if k in fx["dats"] and k in got:
+            assert sha(got[k]["data"]) == fx["dats"][k]["sha256"], k
+    iters = [int(s[2]) - 1 for s in out["per_step"]]
+    ref_iters = fx["iteration_counts_printed"]
+    assert all(abs(a - b) <= 1 for a, b in zip(iters, ref_iters)), (iters, ref_iters)
+    ref = ref_summary(fx)
+    assert np.max(np.abs(out["summary"] - ref) / np.abs(ref)) < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", [1, 2])
+def test_gpu_native_solvers(solver):
+    from ops_amd.apps.tealeaf.decks import tea_deck
+    deck = tea_deck(256, 2)
+    ref = tu.run("gpu", deck, solver=0)
+    got = tu.run("gpu", deck, solver=solver)
+    rx = rx_of(deck)
+    r = tu.operator_residual(got["fields"], 256, 256, rx, rx)
+    assert np.sqrt(np.sum(r * r)) < 1.5e-10 * got["per_step"][-1][4]
+    u0, u1 = ref["fields"]["u"], got["fields"]["u"]
+    assert np.max(np.abs(u1 - u0)) / np.max(np.abs(u0)) < 1e-8
