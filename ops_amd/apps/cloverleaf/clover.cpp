@@ -60,7 +60,9 @@ struct Sim {
   ops_dat xarea, yarea, work1, work2, work3, work4, work5, work6, work7;
   ops_dat cellx, celly, vertexx, vertexy, celldx, celldy, vertexdx, vertexdy, xx, yy;
   ops_stencil S00, P10, P01, M10, M01, S5, P10_P01_P11, M10_M01_M11, P10_M01_P1M1,
-      P01_M10_M1P1, X_M_P, P10_M10_M20, P01_M01_M02, P10_P20_M10, P01_P02_M01, HX, HY;
+      P01_M10_M1P1, X_M_P, P10_M10_M20, P01_M01_M02, P10_P20_M10, P01_P02_M01;
+  // reflective halo sources, one stencil per side: bottom, top, left, right
+  ops_stencil H[4];
   ops_stencil SX, SY, SX_P10, SY_P01, SX_P10_M10, SY_P01_M01;
   ops_reduction r_dt, r_vol, r_mass, r_ie, r_ke, r_press;
 } S;
@@ -242,8 +244,10 @@ void build_field() {
   S.P01_M01_M02 = sten(4, {0, 0, 0, 1, 0, -1, 0, -2}, "0,0:0,1:0,-1:0,-2");
   S.P10_P20_M10 = sten(4, {0, 0, 1, 0, 2, 0, -1, 0}, "0,0:1,0:2,0:-1,0");
   S.P01_P02_M01 = sten(4, {0, 0, 0, 1, 0, 2, 0, -1}, "0,0:0,1:0,2:0,-1");
-  S.HX = sten(9, {0, 0, 1, 0, 2, 0, 3, 0, 4, 0, -1, 0, -2, 0, -3, 0, -4, 0}, "halo mirror x");
-  S.HY = sten(9, {0, 0, 0, 1, 0, 2, 0, 3, 0, 4, 0, -1, 0, -2, 0, -3, 0, -4}, "halo mirror y");
+  S.H[0] = sten(5, {0, 0, 0, 1, 0, 2, 0, 3, 0, 4}, "halo mirror +y");
+  S.H[1] = sten(5, {0, 0, 0, -1, 0, -2, 0, -3, 0, -4}, "halo mirror -y");
+  S.H[2] = sten(5, {0, 0, 1, 0, 2, 0, 3, 0, 4, 0}, "halo mirror +x");
+  S.H[3] = sten(5, {0, 0, -1, 0, -2, 0, -3, 0, -4, 0}, "halo mirror -x");
   S.SX = ssten(1, {0, 0}, 1, 0, "s2D_00_stride2D_x");
   S.SY = ssten(1, {0, 0}, 0, 1, "s2D_00_stride2D_y");
   S.SX_P10 = ssten(2, {0, 0, 1, 0}, 1, 0, "s2D_00_P10_stride2D_x");
@@ -342,7 +346,7 @@ void update_halo(int *fl, int depth) {
     int *ms[4] = {mb, mt, ml, mr};
     int *rs[4] = {rb, rt, rl, rr};
     for (int s = 0; s < 4; s++) {
-      ops_stencil st = s < 2 ? S.HY : S.HX;
+      ops_stencil st = S.H[s];
       ops_par_loop(cl_halo_cells, "update_halo_kernel1", S.blk, 2, rs[s],
                    ops_arg_dat_opt(S.density0, 1, st, "double", OPS_RW, fl[FIELD_DENSITY0]),
                    ops_arg_dat_opt(S.density1, 1, st, "double", OPS_RW, fl[FIELD_DENSITY1]),
@@ -366,7 +370,7 @@ void update_halo(int *fl, int depth) {
     int *ms[4] = {mb, mt, ml, mr};
     int *rs[4] = {rb, rt, rl, rr};
     for (int s = 0; s < 4; s++) {
-      ops_stencil st = s < 2 ? S.HY : S.HX;
+      ops_stencil st = S.H[s];
       ops_par_loop(cl_halo_vel, "update_halo_kernel2_vel", S.blk, 2, rs[s],
                    ops_arg_dat_opt(S.xvel0, 1, st, "double", OPS_RW, fl[FIELD_XVEL0]),
                    ops_arg_dat_opt(S.xvel1, 1, st, "double", OPS_RW, fl[FIELD_XVEL1]),
@@ -388,7 +392,7 @@ void update_halo(int *fl, int depth) {
     int *ms[4] = {mb, mt, ml, mr};
     int *rs[4] = {rb, rt, rl, rr};
     for (int s = 0; s < 4; s++) {
-      ops_stencil st = s < 2 ? S.HY : S.HX;
+      ops_stencil st = S.H[s];
       ops_par_loop(cl_halo_flux, "update_halo_kernel3_flux_x", S.blk, 2, rs[s],
                    ops_arg_dat_opt(S.vol_flux_x, 1, st, "double", OPS_RW, fl[FIELD_VOL_FLUX_X]),
                    ops_arg_dat_opt(S.mass_flux_x, 1, st, "double", OPS_RW, fl[FIELD_MASS_FLUX_X]),
@@ -408,7 +412,7 @@ void update_halo(int *fl, int depth) {
     int *ms[4] = {mb, mt, ml, mr};
     int *rs[4] = {rb, rt, rl, rr};
     for (int s = 0; s < 4; s++) {
-      ops_stencil st = s < 2 ? S.HY : S.HX;
+      ops_stencil st = S.H[s];
       ops_par_loop(cl_halo_flux, "update_halo_kernel4_flux_y", S.blk, 2, rs[s],
                    ops_arg_dat_opt(S.vol_flux_y, 1, st, "double", OPS_RW, fl[FIELD_VOL_FLUX_Y]),
                    ops_arg_dat_opt(S.mass_flux_y, 1, st, "double", OPS_RW, fl[FIELD_MASS_FLUX_Y]),

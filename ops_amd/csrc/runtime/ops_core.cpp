@@ -42,6 +42,46 @@ void ops_throw(int code, const std::string &msg) {
 
 extern "C" const char *ops_get_last_error(void) { return g_last_error.c_str(); }
 
+// The reference addresses dats without a range check (ops_seq_v2.h:55-62); on the GPU an
+// access outside the allocation is a silent corruption or a memory fault, so every loop
+// checks, on the host, the extent its stencils reach in each dimension.
+void ops_check_loop_bounds(const char *name, int dim, const int *start, const int *end,
+                           const ops_arg *args, int nargs) {
+  for (int i = 0; i < nargs; i++) {
+    const ops_arg &a = args[i];
+    if (a.argtype != OPS_ARG_DAT || !a.opt || a.dat == nullptr || a.stencil == nullptr) continue;
+    const ops_stencil s = a.stencil;
+    if (s->type != 0) continue;  // restrict/prolong stencils map coordinates between grids
+    const ops_dat dat = a.dat;
+    for (int d = 0; d < dim && d < dat->block->dims; d++) {
+      int stride = d < s->dims ? s->stride[d] : 1;
+      int pmin = 0, pmax = 0;
+      if (d < s->dims)
+        for (int p = 0; p < s->points; p++) {
+          int v = s->stencil[p * s->dims + d];
+          if (p == 0 || v < pmin) pmin = v;
+          if (p == 0 || v > pmax) pmax = v;
+        }
+      long off = -(long)dat->base[d] - dat->d_m[d];
+      long lo = (long)start[d] * stride + pmin + off;
+      long hi = (long)(end[d] - 1) * stride + pmax + off;
+      if (stride == 0) {
+        lo = pmin + off;
+        hi = pmax + off;
+      }
+      if (lo < 0 || hi >= dat->size[d]) {
+        char buf[512];
+        snprintf(buf, sizeof buf,
+                 "Error: ops_par_loop %s: dat %s is addressed at %ld..%ld in dimension %d, "
+                 "outside its allocation 0..%d (size %d, d_m %d, d_p %d)",
+                 name ? name : "", dat->name, lo, hi, d, dat->size[d] - 1, dat->size[d],
+                 dat->d_m[d], dat->d_p[d]);
+        ops_throw(OPS_INVALID_ARGUMENT, buf);
+      }
+    }
+  }
+}
+
 OPS_instance *ops_instance_get(bool create) {
   if (g_inst == nullptr) {
     if (!create) ops_throw(OPS_RUNTIME_ERROR, "Error: OPS used before ops_init");

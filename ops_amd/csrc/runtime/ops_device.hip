@@ -263,7 +263,6 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
                                   ops_block block, int dim, const int *range, ops_arg *args,
                                   int nargs) {
   OPS_instance *in = ops_instance_get();
-  (void)name;
   if (nargs > OPS_HIP_MAX_ARGS) ops_throw(OPS_NOT_IMPLEMENTED, "too many loop arguments");
   lp->kernel_id = kernel_id;
   lp->ndim = dim;
@@ -327,6 +326,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
     lp->nblocks = 0;
     return 0;
   }
+  ops_check_loop_bounds(name, dim, start, end, args, nargs);
   // workgroup shape: tuned, else 64 lanes along x (coalesced, one wavefront per row
   // segment) and the remaining threads stacked along y/z
   auto &kt = ops_kernel_table();

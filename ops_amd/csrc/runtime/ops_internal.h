@@ -44,6 +44,10 @@ struct OPS_instance {
 
 OPS_instance *ops_instance_get(bool create = false);
 [[noreturn]] void ops_throw(int code, const std::string &msg);
+// Throws OPS_INVALID_ARGUMENT when a dat argument of a loop over [start, end) would be
+// addressed outside its allocation (stencil extent included).
+void ops_check_loop_bounds(const char *name, int dim, const int *start, const int *end,
+                           const ops_arg *args, int nargs);
 
 // kernel + const registries live for the whole process (generated code registers them
 // from static initialisers, before any ops_init)

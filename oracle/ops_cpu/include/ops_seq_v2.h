@@ -33,6 +33,7 @@
 #endif
 
 extern "C" void ops_cpu_before_loop(ops_arg *args, int nargs);
+extern "C" void ops_cpu_check_bounds(int dim, const int *range, const ops_arg *args, int nargs);
 extern "C" void ops_cpu_after_loop(ops_arg *args, int nargs);
 extern "C" int ops_cpu_soa(void);
 
@@ -124,6 +125,7 @@ void run(std::index_sequence<I...>, void (*kernel)(P...), ops_block block, int d
     if (cnt[d] <= 0) return;
   }
   (void)block;
+  ops_cpu_check_bounds(dim, range, args, N);
   ops_cpu_before_loop(args, N);
   auto st = std::make_tuple(Param<typename Strip<P>::type>::prepare(args[I], start, dim)...);
   bool has_red = false;

@@ -63,6 +63,35 @@ extern "C" void ops_cpu_before_loop(ops_arg *args, int n) {
     if (args[i].argtype == OPS_ARG_DAT && args[i].opt && args[i].dat->locked_hd)
       fail(OPS_RUNTIME_ERROR, "ops_par_loop on a dat locked by ops_dat_get_raw_pointer");
 }
+// Extent check of every dat argument (loop range plus stencil reach, per dimension);
+// an out-of-allocation access raises OPS_INVALID_ARGUMENT, as the device runtime does.
+extern "C" void ops_cpu_check_bounds(int dim, const int *range, const ops_arg *args, int n) {
+  for (int i = 0; i < n; i++) {
+    const ops_arg &a = args[i];
+    if (a.argtype != OPS_ARG_DAT || !a.opt || !a.dat || !a.stencil || a.stencil->type != 0)
+      continue;
+    const ops_stencil s = a.stencil;
+    for (int d = 0; d < dim && d < a.dat->block->dims; d++) {
+      const int st = d < s->dims ? s->stride[d] : 1;
+      int lo_p = 0, hi_p = 0;
+      for (int p = 0; d < s->dims && p < s->points; p++) {
+        const int v = s->stencil[p * s->dims + d];
+        lo_p = (p == 0 || v < lo_p) ? v : lo_p;
+        hi_p = (p == 0 || v > hi_p) ? v : hi_p;
+      }
+      const long shift = -(long)a.dat->base[d] - a.dat->d_m[d];
+      const long first = (st ? (long)range[2 * d] * st : 0) + lo_p + shift;
+      const long last = (st ? (long)(range[2 * d + 1] - 1) * st : 0) + hi_p + shift;
+      if (first < 0 || last >= a.dat->size[d])
+        fail(OPS_INVALID_ARGUMENT, std::string("ops_par_loop: dat ") + a.dat->name +
+                                       " addressed outside its allocation in dimension " +
+                                       std::to_string(d) + " (" + std::to_string(first) + ".." +
+                                       std::to_string(last) + " of size " +
+                                       std::to_string(a.dat->size[d]) + ", stencil " +
+                                       s->name + ")");
+    }
+  }
+}
 extern "C" void ops_cpu_after_loop(ops_arg *args, int n) {
   for (int i = 0; i < n; i++)
     if (args[i].argtype == OPS_ARG_DAT && args[i].opt && args[i].acc != OPS_READ)

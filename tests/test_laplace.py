@@ -90,11 +90,30 @@ def test_gpu_config1_bitwise_vs_oracle():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(1, 1), (7, 3), (65, 5), (130, 257)])
+@pytest.mark.parametrize("shape", [(2, 2), (7, 3), (65, 5), (130, 257), (3, 67)])
 def test_gpu_ragged_shapes_bitwise(shape):
     nx, ny = shape
     A, e = run_gpu(nx, ny, 13, 1)
     Ao, _, eo = run_oracle(nx, ny, 13, 1)
+    assert first_diff(A, Ao) is None and e == eo
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1)])
+def test_gpu_edge_dat_extent_is_rejected(shape):
+    # a block extent of 1 makes an edge dat of size 1 in that dimension (ops_lib_core.cpp:
+    # 474-481), so the one-cell halo loops address outside it: the loop raises instead
+    import ctypes
+    import ops_amd
+    L = ops_amd.app("laplace2d")
+    R = ops_amd.runtime()
+    R.ops_get_last_error.restype = ctypes.c_char_p
+    argc, argv = ops_amd.argv()
+    assert L.laplace2d_init(argc, argv, shape[0], shape[1], 1) != 0
+    assert b"outside its allocation" in R.ops_get_last_error()
+    assert L.laplace2d_finalize() == 0
+    A, e = run_gpu(16, 16, 5, 1)  # the runtime is usable again afterwards
+    Ao, _, eo = run_oracle(16, 16, 5, 1)
     assert first_diff(A, Ao) is None and e == eo
 
 
