@@ -102,45 +102,41 @@ template <typename OP, typename T> __device__ __forceinline__ T ops_block_reduce
 }
 
 /*
- * Finishes one reduction argument of dimension DIM for this workgroup. `vals` are the
- * thread's partial values. The workgroup writes its partial to part[b*DIM + d]; the last
- * workgroup to finish folds all partials in a fixed order and combines the result into
- * *value (overwriting it when `init` is set), then re-arms the counter for the next
- * launch. Deterministic for a given launch shape.
+ * Tile walk of a loop with reductions. Such loops run on a capped grid (at most a few
+ * thousand workgroups, a multiple of the 8 XCDs when it is that large), and each
+ * workgroup visits tiles t0, t0+step, ... < t1. Workgroup b runs on XCD b % 8, so the
+ * tiles are split into 8 contiguous chunks, one per XCD: neighbouring tiles share an L2.
+ */
+__device__ __forceinline__ void ops_tile_span(unsigned ntiles, unsigned &t0, unsigned &t1,
+                                              unsigned &step) {
+  const unsigned g = gridDim.x, b = blockIdx.x;
+  if (g >= OPS_XCDS && g % OPS_XCDS == 0) {
+    const unsigned chunk = (ntiles + OPS_XCDS - 1) / OPS_XCDS, x = b % OPS_XCDS;
+    t0 = x * chunk + b / OPS_XCDS;
+    t1 = min(ntiles, x * chunk + chunk);
+    step = g / OPS_XCDS;
+  } else {
+    t0 = b;
+    t1 = ntiles;
+    step = g;
+  }
+}
+
+/*
+ * Writes this workgroup's partial of one reduction argument of dimension DIM to
+ * part[workgroup * DIM + d]. No atomics and no device-scope fences: a second kernel,
+ * launched by the runtime on the same stream, folds the partials in a fixed order.
  */
 template <typename OP, typename T, int DIM>
-__device__ __forceinline__ void ops_reduce_finish(const T (&vals)[DIM], T *part,
-                                                  unsigned *count, T *value, int init) {
+__device__ __forceinline__ void ops_reduce_partial(const T (&vals)[DIM], T *part) {
   __shared__ T lds[16];
-  __shared__ int last;
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-  const int nthreads = blockDim.x * blockDim.y * blockDim.z;
-  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
   const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
     T r = ops_block_reduce<OP>(vals[d], lds);
     if (tid == 0) part[(size_t)b * DIM + d] = r;
   }
-  if (tid == 0) {
-    __threadfence();
-    unsigned ticket = atomicAdd(count, 1u);
-    last = (ticket == nb - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-#pragma unroll
-  for (int d = 0; d < DIM; d++) {
-    T acc = OP::identity();
-    for (unsigned i = tid; i < nb; i += nthreads) {
-      T p = __builtin_nontemporal_load(&part[(size_t)i * DIM + d]);
-      acc = OP::op(acc, p);
-    }
-    T r = ops_block_reduce<OP>(acc, lds);
-    if (tid == 0) value[d] = init ? r : OP::op(value[d], r);
-  }
-  if (tid == 0) *count = 0u;
 }
 
 /* by-value carrier for small ops_arg_gbl READ arguments */

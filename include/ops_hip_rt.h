@@ -20,6 +20,8 @@
 
 #define OPS_HIP_MAX_ARGS 24
 #define OPS_HIP_BLOCK_THREADS 256
+/* grid cap of loops with reductions (a multiple of the 8 XCDs; 8 workgroups per CU) */
+#define OPS_HIP_RED_BLOCKS 2048
 
 #ifdef __cplusplus
 extern "C" {
@@ -43,8 +45,7 @@ typedef struct ops_hip_loop {
   int zdim[OPS_HIP_MAX_ARGS];  /* size[2] */
   int mdim[OPS_HIP_MAX_ARGS];  /* components per point */
   /* reduction args: per-block partials, completion counter, running device value */
-  void *red_part[OPS_HIP_MAX_ARGS];
-  unsigned *red_count[OPS_HIP_MAX_ARGS];
+  void *red_part[OPS_HIP_MAX_ARGS];  /* per-workgroup partials, [workgroup][dim] */
   void *red_value[OPS_HIP_MAX_ARGS];
   int red_init[OPS_HIP_MAX_ARGS];  /* 1: overwrite the running value (first use) */
   /* bookkeeping */

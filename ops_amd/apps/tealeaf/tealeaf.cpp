@@ -387,7 +387,12 @@ void solve() {
   double error = 0.0;
   int n = 1;
   if (T.use_cg && T.solver != 0) {
-    // device-resident solvers: start from r = u0 - A u already computed above
+    // device-resident solvers: start from r = u0 - A u already computed above. They keep
+    // the halos of the solve vectors at zero, which is exact only when the boundary faces
+    // carry zero conductivity (no reflective boundary).
+    if (T.reflective)
+      throw OPSException(OPS_NOT_IMPLEMENTED,
+                         "tealeaf: the device-resident solvers need reflective_boundary off");
     tea_native_solve_params P;
     P.solver = T.solver;
     P.nx = grid.x_cells;

@@ -34,7 +34,6 @@ struct DevState {
   // reduction scratch: per-block partials (one region per loop argument) + counters
   char *red_scratch = nullptr;
   size_t red_scratch_cap = 0;
-  unsigned *red_counters = nullptr;
   // event timing
   struct Pending {
     int kid;
@@ -94,8 +93,6 @@ void ops_dev_init(OPS_instance *inst) {
   HIPCHECK(hipHostMalloc((void **)&g_dev.stage, g_dev.stage_cap, hipHostMallocDefault));
   g_dev.stage_off = 0;
   HIPCHECK(hipHostMalloc((void **)&g_dev.pinned_small, 1u << 16, hipHostMallocDefault));
-  HIPCHECK(hipMalloc((void **)&g_dev.red_counters, sizeof(unsigned) * OPS_HIP_MAX_ARGS));
-  HIPCHECK(hipMemset(g_dev.red_counters, 0, sizeof(unsigned) * OPS_HIP_MAX_ARGS));
   g_dev.live = true;
 }
 
@@ -109,7 +106,6 @@ void ops_dev_exit(OPS_instance *inst) {
   if (g_dev.red_scratch) (void)hipFree(g_dev.red_scratch);
   g_dev.red_scratch = nullptr;
   g_dev.red_scratch_cap = 0;
-  (void)hipFree(g_dev.red_counters);
   (void)hipHostFree(g_dev.stage);
   (void)hipHostFree(g_dev.pinned_small);
   (void)hipStreamDestroy(g_dev.stream);
@@ -290,7 +286,6 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
     lp->ptr[i] = nullptr;
     lp->red_part[i] = nullptr;
     lp->red_value[i] = nullptr;
-    lp->red_count[i] = nullptr;
     lp->red_init[i] = 0;
     if (a.argtype == OPS_ARG_DAT && a.opt) {
       ops_dat dat = a.dat;
@@ -365,6 +360,17 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   lp->grid[1] = (lp->count[1] + by - 1) / by;
   lp->grid[2] = (lp->count[2] + bz - 1) / bz;
   lp->nblocks = lp->grid[0] * lp->grid[1] * lp->grid[2];
+  bool has_red = false;
+  for (int i = 0; i < nargs; i++)
+    if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) has_red = true;
+  if (has_red) {
+    // loops with reductions walk their tiles on a capped 1-D grid (see ops_tile_span)
+    int g = lp->nblocks < OPS_HIP_RED_BLOCKS ? lp->nblocks : OPS_HIP_RED_BLOCKS;
+    lp->grid[0] = g;
+    lp->grid[1] = 1;
+    lp->grid[2] = 1;
+    lp->nblocks = g;
+  }
   // reduction scratch
   size_t need = 0;
   for (int i = 0; i < nargs; i++)
@@ -387,7 +393,6 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       ops_reduction h = (ops_reduction)args[i].data;
       lp->red_part[i] = g_dev.red_scratch + off;
       off += ((size_t)lp->nblocks * (size_t)h->size + 255) & ~(size_t)255;
-      lp->red_count[i] = g_dev.red_counters + i;
       lp->red_value[i] = h->data_d;
       lp->red_init[i] = h->dirty_d ? 0 : 1;
     }
@@ -400,7 +405,69 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   return 1;
 }
 
+// ------------------------------------------------------------------ reduction finish
+// One 1024-lane workgroup folds the per-workgroup partials of one reduction argument in
+// a fixed order (lane-strided, then the workgroup tree) and combines the result into the
+// handle's device value: overwritten on the handle's first use, else combined with it.
+
+template <typename OP, typename T>
+__global__ __launch_bounds__(1024) void red_finish_kernel(const T *__restrict__ part,
+                                                          unsigned nb, int dim, T *value,
+                                                          int init) {
+  __shared__ T lds[16];
+  for (int d = 0; d < dim; d++) {
+    T a0 = OP::identity(), a1 = OP::identity(), a2 = OP::identity(), a3 = OP::identity();
+    unsigned i = threadIdx.x;
+    for (; i + 3 * 1024 < nb; i += 4 * 1024) {
+      a0 = OP::op(a0, part[(size_t)i * dim + d]);
+      a1 = OP::op(a1, part[(size_t)(i + 1024) * dim + d]);
+      a2 = OP::op(a2, part[(size_t)(i + 2048) * dim + d]);
+      a3 = OP::op(a3, part[(size_t)(i + 3072) * dim + d]);
+    }
+    for (; i < nb; i += 1024) a0 = OP::op(a0, part[(size_t)i * dim + d]);
+    T r = ops_block_reduce<OP>(OP::op(OP::op(a0, a1), OP::op(a2, a3)), lds);
+    if (threadIdx.x == 0) value[d] = init ? r : OP::op(value[d], r);
+  }
+}
+
+template <typename T>
+static void launch_finish(int acc, const ops_hip_loop *lp, int i, int dim) {
+  const T *part = (const T *)lp->red_part[i];
+  T *value = (T *)lp->red_value[i];
+  unsigned nb = (unsigned)lp->nblocks;
+  int init = lp->red_init[i];
+  hipStream_t st = g_dev.stream;
+  switch (acc) {
+    case OPS_INC:
+      hipLaunchKernelGGL((red_finish_kernel<ops_red_inc<T>, T>), dim3(1), dim3(1024), 0, st, part,
+                         nb, dim, value, init);
+      break;
+    case OPS_MIN:
+      hipLaunchKernelGGL((red_finish_kernel<ops_red_min<T>, T>), dim3(1), dim3(1024), 0, st, part,
+                         nb, dim, value, init);
+      break;
+    case OPS_MAX:
+      hipLaunchKernelGGL((red_finish_kernel<ops_red_max<T>, T>), dim3(1), dim3(1024), 0, st, part,
+                         nb, dim, value, init);
+      break;
+    default: ops_throw(OPS_INVALID_ARGUMENT, "reduction with an unsupported access mode");
+  }
+  ops_hip_check_launch("reduction finish");
+}
+
+static void finish_reduction(const ops_hip_loop *lp, const ops_arg &a, int i) {
+  ops_reduction h = (ops_reduction)a.data;
+  const char *t = h->type;
+  if (!strcmp(t, "double")) launch_finish<double>(a.acc, lp, i, a.dim);
+  else if (!strcmp(t, "float")) launch_finish<float>(a.acc, lp, i, a.dim);
+  else if (!strcmp(t, "int")) launch_finish<int>(a.acc, lp, i, a.dim);
+  else ops_throw(OPS_NOT_IMPLEMENTED, std::string("reduction of type ") + t);
+}
+
 extern "C" void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs) {
+  if (lp->nblocks > 0)
+    for (int i = 0; i < nargs; i++)
+      if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) finish_reduction(lp, args[i], i);
   if (lp->timed) {
     HIPCHECK(hipEventRecord(g_dev.pending.back().b, g_dev.stream));
     if (g_dev.pending.size() >= 8192) ops_dev_timing_flush();

@@ -162,22 +162,34 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
             L.append(f"  ops_gbl<{CTYPES.get(a.typ, a.typ)}, {const_dim(a, loop)}> g{i};")
         elif a.kind == "reduce":
             t = CTYPES.get(a.typ, a.typ)
-            L.append(f"  {t} *rp{i}; unsigned *rc{i}; {t} *rv{i}; int ri{i};")
+            L.append(f"  {t} *rp{i};")
     L.append("};")
     L.append("")
     red = [i for i, a in enumerate(loop.args) if a.kind == "reduce"]
     L.append(f"__global__ __launch_bounds__(256) void ops_kern_{k}(const ops_kp_{k} p) {{")
-    L.append("  unsigned bx, by, bz;")
-    L.append("  ops_block_coords(bx, by, bz);")
-    L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
-    L.append("  const int y = (int)(by * blockDim.y + threadIdx.y);")
-    L.append("  const int z = (int)(bz * blockDim.z + threadIdx.z);")
     for i in red:
         a = loop.args[i]
         t = CTYPES.get(a.typ, a.typ)
         d = const_dim(a, loop)
         L.append(f"  {t} r{i}[{d}];")
         L.append(f"  for (int d = 0; d < {d}; d++) r{i}[d] = {RED_OP[a.acc]}<{t}>::identity();")
+    if red:
+        # capped grid: each workgroup walks several tiles, accumulating in registers
+        L.append("  const unsigned tx = (p.cnt0 + blockDim.x - 1) / blockDim.x;")
+        L.append("  const unsigned ty = (p.cnt1 + blockDim.y - 1) / blockDim.y;")
+        L.append("  const unsigned tz = (p.cnt2 + blockDim.z - 1) / blockDim.z;")
+        L.append("  unsigned t, tend, tstep;")
+        L.append("  ops_tile_span(tx * ty * tz, t, tend, tstep);")
+        L.append("  for (; t < tend; t += tstep) {")
+        L.append("  const int x = (int)((t % tx) * blockDim.x + threadIdx.x);")
+        L.append("  const int y = (int)((t / tx % ty) * blockDim.y + threadIdx.y);")
+        L.append("  const int z = (int)((t / (tx * ty)) * blockDim.z + threadIdx.z);")
+    else:
+        L.append("  unsigned bx, by, bz;")
+        L.append("  ops_block_coords(bx, by, bz);")
+        L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
+        L.append("  const int y = (int)(by * blockDim.y + threadIdx.y);")
+        L.append("  const int z = (int)(bz * blockDim.z + threadIdx.z);")
     L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
     call = []
     for i, a in enumerate(loop.args):
@@ -217,11 +229,13 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
             call.append(f"idx{i}")
     L.append(f"    {loop.kernel}({', '.join(call)});")
     L.append("  }")
+    if red:
+        L.append("  }")
     for i in red:
         a = loop.args[i]
         t = CTYPES.get(a.typ, a.typ)
         d = const_dim(a, loop)
-        L.append(f"  ops_reduce_finish<{RED_OP[a.acc]}<{t}>, {t}, {d}>(r{i}, p.rp{i}, p.rc{i}, p.rv{i}, p.ri{i});")
+        L.append(f"  ops_reduce_partial<{RED_OP[a.acc]}<{t}>, {t}, {d}>(r{i}, p.rp{i});")
     L.append("}")
     return "\n".join(L)
 
@@ -249,8 +263,7 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
             L.append(f"    memcpy(p.g{i}.v, arg{i}.data, sizeof(p.g{i}.v));")
         elif a.kind == "reduce":
             t = CTYPES.get(a.typ, a.typ)
-            L.append(f"    p.rp{i} = ({t} *)lp.red_part[{i}]; p.rc{i} = lp.red_count[{i}]; "
-                     f"p.rv{i} = ({t} *)lp.red_value[{i}]; p.ri{i} = lp.red_init[{i}];")
+            L.append(f"    p.rp{i} = ({t} *)lp.red_part[{i}];")
     L.append(f"    hipLaunchKernelGGL({ns}::ops_kern_{k}, dim3(lp.grid[0], lp.grid[1], lp.grid[2]),")
     L.append("                       dim3(lp.block[0], lp.block[1], lp.block[2]), 0, (hipStream_t)lp.stream, p);")
     L.append("    ops_hip_check_launch(name);")
