@@ -102,27 +102,6 @@ template <typename OP, typename T> __device__ __forceinline__ T ops_block_reduce
 }
 
 /*
- * Tile walk of a loop with reductions. Such loops run on a capped grid (at most a few
- * thousand workgroups, a multiple of the 8 XCDs when it is that large), and each
- * workgroup visits tiles t0, t0+step, ... < t1. Workgroup b runs on XCD b % 8, so the
- * tiles are split into 8 contiguous chunks, one per XCD: neighbouring tiles share an L2.
- */
-__device__ __forceinline__ void ops_tile_span(unsigned ntiles, unsigned &t0, unsigned &t1,
-                                              unsigned &step) {
-  const unsigned g = gridDim.x, b = blockIdx.x;
-  if (g >= OPS_XCDS && g % OPS_XCDS == 0) {
-    const unsigned chunk = (ntiles + OPS_XCDS - 1) / OPS_XCDS, x = b % OPS_XCDS;
-    t0 = x * chunk + b / OPS_XCDS;
-    t1 = min(ntiles, x * chunk + chunk);
-    step = g / OPS_XCDS;
-  } else {
-    t0 = b;
-    t1 = ntiles;
-    step = g;
-  }
-}
-
-/*
  * Writes this workgroup's partial of one reduction argument of dimension DIM to
  * part[workgroup * DIM + d]. No atomics and no device-scope fences: a second kernel,
  * launched by the runtime on the same stream, folds the partials in a fixed order.

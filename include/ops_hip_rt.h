@@ -20,8 +20,6 @@
 
 #define OPS_HIP_MAX_ARGS 24
 #define OPS_HIP_BLOCK_THREADS 256
-/* grid cap of loops with reductions (a multiple of the 8 XCDs; 8 workgroups per CU) */
-#define OPS_HIP_RED_BLOCKS 2048
 
 #ifdef __cplusplus
 extern "C" {
@@ -90,6 +88,10 @@ int ops_hip_kernel_count(void);
 int ops_hip_kernel_stats(int kernel_id, char *name_out, int name_cap, long *calls,
                          double *total_ms, double *bytes);
 void ops_hip_reset_stats(void);
+/* Measured HBM bandwidth: a plain copy of `bytes` between two fresh device buffers,
+   `reps` times on the runtime stream; returns (read + write bytes) / time in GB/s, or a
+   negative value on error. Allocates and frees its buffers outside the timed region. */
+double ops_hip_copy_probe(size_t bytes, int reps);
 
 #ifdef __cplusplus
 }

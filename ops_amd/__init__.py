@@ -59,6 +59,8 @@ def _declare_runtime(L):
     L.ops_hip_set_timing.argtypes = [i]
     L.ops_hip_get_stream.restype = p
     L.ops_hip_set_block_shape.argtypes = [i, i, i, i]
+    L.ops_hip_copy_probe.restype = d
+    L.ops_hip_copy_probe.argtypes = [ctypes.c_size_t, i]
 
 
 def kernel_stats():

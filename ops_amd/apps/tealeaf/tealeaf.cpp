@@ -11,7 +11,7 @@
 // Solvers (tealeaf_set_solver):
 //   0  reference path: CG (or Jacobi) iterations as par_loops with host-side alpha/beta
 //   1  FP64 CG with the scalars kept on the device (native kernels, tealeaf_native.hip)
-//   2  FP32-storage CG inner iterations inside FP64 iterative refinement
+//   2  FP32-storage CG iterations with FP64 reliable updates (residual replacement)
 // Solver 1 and 2 stop on the same test as the reference: sqrt(r.r) below eps times the
 // residual norm of the step's initial guess.
 #include <math.h>

@@ -6,9 +6,16 @@
 // below eps * initial_residual, the reference's test (apps/c/TeaLeaf/tea_solve.cpp:349-355).
 //   solver 1: FP64 CG; alpha/beta and the stop test stay on the GPU (no per-iteration
 //             host round trip), p-update fused into the operator application.
-//   solver 2: FP32 storage and arithmetic for the inner CG iterations inside FP64
-//             iterative refinement: the residual recomputation, its norm and the
-//             solution update run in FP64; the stop test uses the FP64 residual.
+//   solver 2: FP32 storage and arithmetic for the CG iterations inside FP64 refinement
+//             by reliable updates (residual replacement): whenever the iterated
+//             residual has dropped by a factor delta (OPS_TEA_RELIABLE_DELTA, default
+//             0.1) since the last replacement, and when it meets the stop threshold,
+//             the FP32 residual is replaced by the FP64 residual u0 - A u, its norm
+//             taken in FP64. The search direction is kept, so the Krylov process
+//             continues rather than restarting. The FP32 solution increment is folded
+//             into the FP64 solution every 32 iterations. The stop follows the
+//             reference's rule on the iterated residual; the reported error is the
+//             FP64 residual after the final replacement.
 #ifndef TEALEAF_NATIVE_H
 #define TEALEAF_NATIVE_H
 
@@ -23,8 +30,8 @@ typedef struct {
 } tea_native_solve_params;
 
 typedef struct {
-  int iterations;     // CG iterations (all inner iterations for solver 2)
-  int outer;          // refinement steps (solver 2), 1 otherwise
+  int iterations;     // CG iterations
+  int outer;          // FP64 residual replacements (solver 2), 1 otherwise
   double error;       // sqrt(r.r) at exit
 } tea_native_result;
 

@@ -9,6 +9,7 @@
 //   stub pre/post work     templates/cpp/cuda/loop_host.cpp.j2:299-782
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -360,17 +361,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   lp->grid[1] = (lp->count[1] + by - 1) / by;
   lp->grid[2] = (lp->count[2] + bz - 1) / bz;
   lp->nblocks = lp->grid[0] * lp->grid[1] * lp->grid[2];
-  bool has_red = false;
-  for (int i = 0; i < nargs; i++)
-    if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) has_red = true;
-  if (has_red) {
-    // loops with reductions walk their tiles on a capped 1-D grid (see ops_tile_span)
-    int g = lp->nblocks < OPS_HIP_RED_BLOCKS ? lp->nblocks : OPS_HIP_RED_BLOCKS;
-    lp->grid[0] = g;
-    lp->grid[1] = 1;
-    lp->grid[2] = 1;
-    lp->nblocks = g;
-  }
+
   // reduction scratch
   size_t need = 0;
   for (int i = 0; i < nargs; i++)
@@ -406,27 +397,33 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
 }
 
 // ------------------------------------------------------------------ reduction finish
-// One 1024-lane workgroup folds the per-workgroup partials of one reduction argument in
-// a fixed order (lane-strided, then the workgroup tree) and combines the result into the
-// handle's device value: overwritten on the handle's first use, else combined with it.
+// Every workgroup of a loop with reductions writes its partial; one 1024-lane workgroup
+// then folds them in a fixed order (eight independent lane-strided accumulators, then the
+// workgroup tree) and combines the result into the handle's device value: overwritten on
+// the handle's first use, else combined with it. No atomics, no device-scope fences.
 
 template <typename OP, typename T>
 __global__ __launch_bounds__(1024) void red_finish_kernel(const T *__restrict__ part,
                                                           unsigned nb, int dim, T *value,
                                                           int init) {
   __shared__ T lds[16];
+  const unsigned tid = threadIdx.x;
   for (int d = 0; d < dim; d++) {
-    T a0 = OP::identity(), a1 = OP::identity(), a2 = OP::identity(), a3 = OP::identity();
-    unsigned i = threadIdx.x;
-    for (; i + 3 * 1024 < nb; i += 4 * 1024) {
-      a0 = OP::op(a0, part[(size_t)i * dim + d]);
-      a1 = OP::op(a1, part[(size_t)(i + 1024) * dim + d]);
-      a2 = OP::op(a2, part[(size_t)(i + 2048) * dim + d]);
-      a3 = OP::op(a3, part[(size_t)(i + 3072) * dim + d]);
+    T acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc[u] = OP::identity();
+    for (unsigned base = 0; base < nb; base += 8 * 1024) {
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const unsigned i = base + u * 1024 + tid;
+        if (i < nb) acc[u] = OP::op(acc[u], part[(size_t)i * dim + d]);
+      }
     }
-    for (; i < nb; i += 1024) a0 = OP::op(a0, part[(size_t)i * dim + d]);
-    T r = ops_block_reduce<OP>(OP::op(OP::op(a0, a1), OP::op(a2, a3)), lds);
-    if (threadIdx.x == 0) value[d] = init ? r : OP::op(value[d], r);
+    T a = acc[0];
+#pragma unroll
+    for (int u = 1; u < 8; u++) a = OP::op(a, acc[u]);
+    T r = ops_block_reduce<OP>(a, lds);
+    if (tid == 0) value[d] = init ? r : OP::op(value[d], r);
   }
 }
 
@@ -661,5 +658,46 @@ void ops_dev_slab_copy(ops_dat dat, char *buf, int buf_on_device, const int *ran
   if (!buf_on_device) {
     ops_dev_sync();
     ops_dev_free(dbuf);
+  }
+}
+
+// ------------------------------------------------------------------ bandwidth probe
+
+typedef double ops_v2d __attribute__((ext_vector_type(2)));
+
+// one 16-byte element per lane on a full grid, streamed past the caches: the variant of a
+// plain copy that reaches the highest HBM bandwidth on gfx950
+__global__ __launch_bounds__(256) void copy_probe_kernel(const ops_v2d *__restrict__ a,
+                                                         ops_v2d *__restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(&a[i]), &b[i]);
+}
+
+extern "C" double ops_hip_copy_probe(size_t bytes, int reps) {
+  try {
+    if (!g_dev.live || reps < 1 || bytes < 16) return -1.0;
+    const size_t n = bytes / sizeof(ops_v2d);
+    if ((n + 255) / 256 > 0x7fffffffu) return -1.0;
+    void *a = ops_dev_malloc(n * sizeof(ops_v2d)), *b = ops_dev_malloc(n * sizeof(ops_v2d));
+    ops_dev_memset(a, 0, n * sizeof(ops_v2d));
+    hipEvent_t e0 = get_event(), e1 = get_event();
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, g_dev.stream,
+                       (const ops_v2d *)a, (ops_v2d *)b, n);  // warm-up
+    HIPCHECK(hipEventRecord(e0, g_dev.stream));
+    for (int r = 0; r < reps; r++)
+      hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, g_dev.stream,
+                         (const ops_v2d *)a, (ops_v2d *)b, n);
+    HIPCHECK(hipEventRecord(e1, g_dev.stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    g_dev.pool.push_back(e0);
+    g_dev.pool.push_back(e1);
+    ops_dev_free(a);
+    ops_dev_free(b);
+    return 2.0 * (double)(n * sizeof(ops_v2d)) * reps / (ms * 1e-3) / 1e9;
+  } catch (std::exception &) {
+    return -1.0;
   }
 }

@@ -173,23 +173,11 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
         d = const_dim(a, loop)
         L.append(f"  {t} r{i}[{d}];")
         L.append(f"  for (int d = 0; d < {d}; d++) r{i}[d] = {RED_OP[a.acc]}<{t}>::identity();")
-    if red:
-        # capped grid: each workgroup walks several tiles, accumulating in registers
-        L.append("  const unsigned tx = (p.cnt0 + blockDim.x - 1) / blockDim.x;")
-        L.append("  const unsigned ty = (p.cnt1 + blockDim.y - 1) / blockDim.y;")
-        L.append("  const unsigned tz = (p.cnt2 + blockDim.z - 1) / blockDim.z;")
-        L.append("  unsigned t, tend, tstep;")
-        L.append("  ops_tile_span(tx * ty * tz, t, tend, tstep);")
-        L.append("  for (; t < tend; t += tstep) {")
-        L.append("  const int x = (int)((t % tx) * blockDim.x + threadIdx.x);")
-        L.append("  const int y = (int)((t / tx % ty) * blockDim.y + threadIdx.y);")
-        L.append("  const int z = (int)((t / (tx * ty)) * blockDim.z + threadIdx.z);")
-    else:
-        L.append("  unsigned bx, by, bz;")
-        L.append("  ops_block_coords(bx, by, bz);")
-        L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
-        L.append("  const int y = (int)(by * blockDim.y + threadIdx.y);")
-        L.append("  const int z = (int)(bz * blockDim.z + threadIdx.z);")
+    L.append("  unsigned bx, by, bz;")
+    L.append("  ops_block_coords(bx, by, bz);")
+    L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
+    L.append("  const int y = (int)(by * blockDim.y + threadIdx.y);")
+    L.append("  const int z = (int)(bz * blockDim.z + threadIdx.z);")
     L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
     call = []
     for i, a in enumerate(loop.args):
@@ -229,8 +217,6 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
             call.append(f"idx{i}")
     L.append(f"    {loop.kernel}({', '.join(call)});")
     L.append("  }")
-    if red:
-        L.append("  }")
     for i in red:
         a = loop.args[i]
         t = CTYPES.get(a.typ, a.typ)

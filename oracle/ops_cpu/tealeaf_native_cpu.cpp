@@ -1,8 +1,8 @@
 // ORACLE — test infrastructure only. Host restatement of the TeaLeaf device-resident
 // solvers (ops_amd/apps/tealeaf/tealeaf_native.h) for the oracle build of the
-// application: the same recurrences (FP64 CG; FP32 inner CG inside FP64 iterative
-// refinement with the correction equation scaled to a unit residual), evaluated in
-// plain loops. Dot products are summed in row order.
+// application: the same recurrences (FP64 CG; FP32 CG with reliable updates, i.e. FP64
+// residual replacement and solution accumulation), evaluated in plain loops. Dot
+// products are summed in row order; the replacement test runs every iteration.
 #include <math.h>
 #include <stdlib.h>
 
@@ -82,37 +82,94 @@ void tea_native_cg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_d
     R->error = err;
     return;
   }
-  const char *tau_env = getenv("OPS_TEA_MIXED_TAU");
-  const double tau = tau_env ? atof(tau_env) : 1e-4;
-  std::vector<float> kx32(n), ky32(n), r32(n, 0.f), p32(n, 0.f), w32(n, 0.f), d32(n, 0.f);
+  // FP32 CG with reliable updates: the residual is replaced by the FP64 one when the
+  // iterated r.r fell by delta^2 since the last replacement and when it meets the
+  // threshold (then the solve stops); the direction p is kept; the FP32 increment is
+  // folded into the FP64 solution every 32 iterations and at each replacement.
+  const char *delta_env = getenv("OPS_TEA_RELIABLE_DELTA");
+  const double delta = delta_env ? atof(delta_env) : 0.1;
+  std::vector<float> kx32(n), ky32(n), r32(n, 0.f), p32(n, 0.f), w32(n, 0.f), x32(n, 0.f);
   for (long i = 0; i < n; i++) {
     kx32[i] = (float)kx[i];
     ky32[i] = (float)ky[i];
   }
-  double rr = P->initial_residual * P->initial_residual, err = P->initial_residual;
-  int total = 0, outer = 0;
-  while (err >= thresh && total < P->max_iters) {
-    outer++;
-    const double scale = 1.0 / sqrt(rr);
-    double inner = tau, floor_rel = 0.5 * thresh * scale;
-    if (floor_rel > inner) inner = floor_rel;
+  float rro = 0.f;
+  for (int y = 1; y <= ny; y++)
+    for (int x = 1; x <= nx; x++) {
+      long i = x + y * sx;
+      r32[i] = (float)Rr[i];
+      p32[i] = r32[i];
+      rro += r32[i] * r32[i];
+    }
+  double maxrr = rro, err = P->initial_residual;
+  const float rx = (float)P->rx, ry = (float)P->ry;
+  int it = 0, outer = 0;
+  bool delta_on = true;
+  bool stalled = false;  // stopped by the rule (converged or at the iterated threshold)
+  auto fold_increment = [&]() {
     for (int y = 1; y <= ny; y++)
       for (int x = 1; x <= nx; x++) {
         long i = x + y * sx;
-        r32[i] = (float)(Rr[i] * scale);
-        d32[i] = 0.f;
+        U[i] = U[i] + (double)x32[i];
+        x32[i] = 0.f;
       }
-    double e32;
-    int it = cg<float, float, float>(kx32.data(), ky32.data(), d32.data(), r32.data(), p32.data(),
-                                     w32.data(), nx, ny, (float)P->rx, (float)P->ry, inner,
-                                     P->max_iters - total, &e32);
-    total += it;
-    rr = 0.0;
+  };
+  while (it < P->max_iters) {
+    float pw = 0.f;
     for (int y = 1; y <= ny; y++)
       for (int x = 1; x <= nx; x++) {
         long i = x + y * sx;
-        U[i] = U[i] + (double)d32[i] * (1.0 / scale);
+        w32[i] = apply_A<float, float>(kx32.data(), ky32.data(), p32.data(), i, sx, rx, ry);
+        pw += w32[i] * p32[i];
       }
+    const float alpha = rro / pw;
+    float rrn = 0.f;
+    for (int y = 1; y <= ny; y++)
+      for (int x = 1; x <= nx; x++) {
+        long i = x + y * sx;
+        x32[i] = x32[i] + alpha * p32[i];
+        r32[i] = r32[i] - alpha * w32[i];
+        rrn += r32[i] * r32[i];
+      }
+    it++;
+    float beta = rrn / rro;
+    rro = rrn;
+    const bool candidate = sqrt(fabs((double)rrn)) < thresh;
+    const bool replace = candidate || (delta_on && (double)rrn < delta * delta * maxrr);
+    if (it % 32 == 0 || replace) fold_increment();
+    if (replace) {
+      double rr = 0.0;
+      for (int y = 1; y <= ny; y++)
+        for (int x = 1; x <= nx; x++) {
+          long i = x + y * sx;
+          double ri = U0[i] - apply_A<double, double>(kx, ky, U, i, sx, P->rx, P->ry);
+          Rr[i] = ri;
+          r32[i] = (float)ri;
+          rr += ri * ri;
+        }
+      outer++;
+      beta = (float)((double)beta * (rr / (double)rro));
+      rro = (float)rr;
+      maxrr = rr;
+      const double prev = err;
+      err = sqrt(fabs(rr));
+      if (err < thresh || candidate) {
+        stalled = true;
+        break;
+      }
+      if (err >= 0.5 * prev) delta_on = false;  // attainable accuracy reached
+    }
+    for (int y = 1; y <= ny; y++)
+      for (int x = 1; x <= nx; x++) {
+        long i = x + y * sx;
+        p32[i] = beta * p32[i] + 1.0f * r32[i];
+      }
+  }
+  int total = it;
+  if (!stalled) {
+    // iteration limit: fold the increment in and report the FP64 residual
+    fold_increment();
+    double rr = 0.0;
     for (int y = 1; y <= ny; y++)
       for (int x = 1; x <= nx; x++) {
         long i = x + y * sx;
@@ -120,8 +177,8 @@ void tea_native_cg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_d
         Rr[i] = ri;
         rr += ri * ri;
       }
+    outer++;
     err = sqrt(fabs(rr));
-    if (it == 0) break;
   }
   R->iterations = total;
   R->outer = outer;
