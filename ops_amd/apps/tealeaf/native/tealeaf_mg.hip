@@ -1,0 +1,514 @@
+// TeaLeaf implicit step: FP64 flexible preconditioned CG whose preconditioner is one
+// FP32 multigrid V-cycle (no reference counterpart; interface in tealeaf_native.h).
+//
+// Operator on every level: (A x)_i = m x_i + sum over the four faces f of c_f (x_i - x_j),
+// with x = 0 outside the mesh (a boundary face only adds to the diagonal). Level 0 is the
+// TeaLeaf operator (m = 1, c = rx*Kx, ry*Ky). Level l+1 aggregates 2 x 2 cells: with the
+// piecewise-constant prolongation P, A_{l+1} = P^T A_l P keeps the same form, with mass
+// 4 m and each coarse face coefficient the sum of the two fine faces it covers.
+// Smoother: red-black Gauss-Seidel, red then black before the coarse correction, black
+// then red after it, so the V-cycle is a symmetric operator; the coarse correction is
+// scaled by 1.5 (aggregation under-corrects smooth modes). The coarsest level gets a
+// fixed number of symmetric sweeps.
+//
+// Arrays on level l have pitch s = n_l + 2 and one halo layer: cx[i] is the face to the
+// left of cell i (the face right of the last cell sits in the halo column), cy[i] the
+// face below it. x and b keep zero halos.
+//
+// The outer PCG runs in FP64 and stops on the reference's rule, sqrt(r.r) below
+// eps * initial_residual. beta is the flexible (Polak-Ribiere) form, so an FP32
+// preconditioner that is only nearly linear does not spoil the recurrences. Scalars stay
+// on the device; an iteration is captured once per parity as a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "ops_hip_kernel.h"
+#include "ops_hip_rt.h"
+#include "tealeaf_native.h"
+
+#define CK(x) ops_hip_check((int)(x), #x, __FILE__, __LINE__)
+
+namespace {
+
+constexpr int BX = 64, BY = 4, NT = BX * BY;
+constexpr int FT = 1024;
+constexpr int POLL = 8;          // replays of the two-iteration graph between polls
+constexpr int COARSEST = 16;     // stop coarsening at this size (or at an odd size)
+constexpr int COARSE_SWEEPS = 8; // symmetric sweep pairs on the coarsest level
+constexpr float ALPHA_C = 1.5f;  // coarse-correction scaling
+
+struct Level {
+  int n, s;      // cells per side, pitch
+  float m;       // mass term
+  float *cx, *cy, *x, *b;
+};
+
+struct Rec {
+  double rz, rzp, pw, alpha, beta, rr, err, thresh;
+  int stopped, iters, pad[14];
+};
+
+__device__ __forceinline__ unsigned lin_block() {
+  return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+
+// 2-D cell of this thread (1-based interior coordinates); false outside
+__device__ __forceinline__ bool cell2(int nxc, int nyc, int &x, int &y) {
+  unsigned bx, by, bz;
+  ops_block_coords(bx, by, bz);
+  x = 1 + (int)(bx * BX + threadIdx.x);
+  y = 1 + (int)(by * BY + threadIdx.y);
+  return x <= nxc && y <= nyc;
+}
+
+dim3 grid2(int nxc, int nyc) {
+  return dim3((unsigned)((nxc + BX - 1) / BX), (unsigned)((nyc + BY - 1) / BY));
+}
+
+template <typename T> __device__ __forceinline__ void write_partial(T v, T *part) {
+  __shared__ T lds[NT / OPS_WAVE];
+  T s = ops_block_reduce<ops_red_inc<T>>(v, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[lin_block()] = s;
+}
+
+// fixed-order sum of n partials by one FT-lane workgroup (valid in thread 0)
+__device__ __forceinline__ double fold(const double *__restrict__ part, unsigned n) {
+  __shared__ double lds[FT / OPS_WAVE];
+  double acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++) acc[u] = 0.0;
+  for (unsigned base = 0; base < n; base += 8 * FT) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const unsigned j = base + u * FT + threadIdx.x;
+      if (j < n) acc[u] += part[j];
+    }
+  }
+  const double a = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  return ops_block_reduce<ops_red_inc<double>>(a, lds);
+}
+
+// ---------------------------------------------------------------- level set-up
+
+// level 0 coefficients from the FP64 conduction coefficients
+__global__ __launch_bounds__(NT) void k_fine_coef(const double *__restrict__ Kx,
+                                                  const double *__restrict__ Ky, float *cx, float *cy,
+                                                  int n, double rx, double ry) {
+  int x, y;
+  // faces: x and y up to n + 1 (the far boundary faces sit in the halo column / row)
+  if (!cell2(n + 1, n + 1, x, y)) return;
+  const long i = x + (long)y * (n + 2);
+  cx[i] = (y <= n) ? (float)(rx * Kx[i]) : 0.f;
+  cy[i] = (x <= n) ? (float)(ry * Ky[i]) : 0.f;
+}
+
+// coarse faces: the sum of the two fine faces each covers
+__global__ __launch_bounds__(NT) void k_coarsen(const float *__restrict__ fcx, const float *__restrict__ fcy,
+                                                int fs, float *ccx, float *ccy, int nc) {
+  int X, Y;
+  if (!cell2(nc + 1, nc + 1, X, Y)) return;
+  const int cs = nc + 2;
+  const long ic = X + (long)Y * cs;
+  const int fx = 2 * X - 1, fy = 2 * Y - 1;  // first fine child (1-based)
+  ccx[ic] = (Y <= nc) ? fcx[fx + (long)fy * fs] + fcx[fx + (long)(fy + 1) * fs] : 0.f;
+  ccy[ic] = (X <= nc) ? fcy[fx + (long)fy * fs] + fcy[fx + 1 + (long)fy * fs] : 0.f;
+}
+
+// ---------------------------------------------------------------- V-cycle kernels
+
+// One red-black Gauss-Seidel half sweep over the cells of colour `color` ((x + y) % 2).
+// Thread t of a row owns the pair of cells 2t+1, 2t+2; with init set the level's x is
+// taken as zero (the sweep writes b / diag on its colour and zero on the other).
+__global__ __launch_bounds__(NT) void k_smooth(Level L, int color, int init, const Rec *S) {
+  if (S->stopped) return;
+  unsigned bx, by, bz;
+  ops_block_coords(bx, by, bz);
+  const int t = (int)(bx * BX + threadIdx.x);
+  const int y = 1 + (int)(by * BY + threadIdx.y);
+  if (y > L.n) return;
+  const int x0 = 2 * t + 1;
+  if (x0 > L.n) return;
+  const int x = ((x0 + y) % 2 == color) ? x0 : x0 + 1;  // the cell of this colour
+  const int xo = (x == x0) ? x0 + 1 : x0;               // the other cell of the pair
+  const int s = L.s;
+  if (x <= L.n) {
+    const long i = x + (long)y * s;
+    const float c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
+    const float diag = L.m + ((c0 + c1) + (c2 + c3));
+    float v = L.b[i];
+    if (!init) v += (c0 * L.x[i - 1] + c1 * L.x[i + 1]) + (c2 * L.x[i - s] + c3 * L.x[i + s]);
+    L.x[i] = v / diag;
+  }
+  if (init && xo <= L.n) L.x[xo + (long)y * s] = 0.f;
+}
+
+// b_coarse = sum over the 2 x 2 children of (b - A x)
+__global__ __launch_bounds__(NT) void k_restrict(Level F, Level C, const Rec *S) {
+  if (S->stopped) return;
+  int X, Y;
+  if (!cell2(C.n, C.n, X, Y)) return;
+  const int s = F.s;
+  float acc = 0.f;
+#pragma unroll
+  for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+    for (int dx = 0; dx < 2; dx++) {
+      const long i = (2 * X - 1 + dx) + (long)(2 * Y - 1 + dy) * s;
+      const float c0 = F.cx[i], c1 = F.cx[i + 1], c2 = F.cy[i], c3 = F.cy[i + s];
+      const float diag = F.m + ((c0 + c1) + (c2 + c3));
+      const float ax = diag * F.x[i] - ((c0 * F.x[i - 1] + c1 * F.x[i + 1]) + (c2 * F.x[i - s] + c3 * F.x[i + s]));
+      acc += F.b[i] - ax;
+    }
+  C.b[X + (long)Y * C.s] = acc;
+}
+
+// x_fine += ALPHA_C * x_coarse(parent)
+__global__ __launch_bounds__(NT) void k_prolong(Level F, Level C, const Rec *S) {
+  if (S->stopped) return;
+  int x, y;
+  if (!cell2(F.n, F.n, x, y)) return;
+  const long i = x + (long)y * F.s;
+  F.x[i] += ALPHA_C * C.x[(x + 1) / 2 + (long)((y + 1) / 2) * C.s];
+}
+
+// ---------------------------------------------------------------- outer FP64 PCG kernels
+
+struct Fine {
+  int n, s;
+  double rx, ry;
+};
+
+__device__ __forceinline__ bool fcell(const Fine &g, long &i) {
+  int x, y;
+  const bool ok = cell2(g.n, g.n, x, y);
+  i = x + (long)y * g.s;
+  return ok;
+}
+
+// start: b32 = (float) r, r.r partials
+__global__ __launch_bounds__(NT) void k_start(const double *__restrict__ r, float *__restrict__ b32, Fine g,
+                                              double *part) {
+  long i;
+  double v = 0.0;
+  if (fcell(g, i)) {
+    const double ri = r[i];
+    b32[i] = (float)ri;
+    v = ri * ri;
+  }
+  write_partial(v, part);
+}
+
+__global__ __launch_bounds__(FT) void k_fold_start(const double *part, unsigned n, Rec *S, double thresh) {
+  const double rr = fold(part, n);
+  if (threadIdx.x == 0) {
+    S->rr = rr;
+    S->err = sqrt(fabs(rr));
+    S->thresh = thresh;
+    S->rz = 0.0;
+    S->stopped = 0;
+    S->iters = 0;
+  }
+}
+
+// partials of r.z and r.z_prev (z = V-cycle output, z_prev the previous one)
+__global__ __launch_bounds__(NT) void k_dots(const double *__restrict__ r, const float *__restrict__ z,
+                                             const float *__restrict__ zp, Fine g, double *part,
+                                             const Rec *S) {
+  if (S->stopped) return;
+  long i;
+  double a = 0.0, b = 0.0;
+  if (fcell(g, i)) {
+    const double ri = r[i];
+    a = ri * (double)z[i];
+    b = ri * (double)zp[i];
+  }
+  __shared__ double lds[NT / OPS_WAVE];
+  const double sa = ops_block_reduce<ops_red_inc<double>>(a, lds);
+  const double sb = ops_block_reduce<ops_red_inc<double>>(b, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    part[2 * lin_block()] = sa;
+    part[2 * lin_block() + 1] = sb;
+  }
+}
+
+// beta = (r.z - r.z_prev) / rz_old (flexible form; zero on the first iteration)
+__global__ __launch_bounds__(FT) void k_fold_dots(const double *part, unsigned n, Rec *S) {
+  if (S->stopped) return;
+  __shared__ double lds[FT / OPS_WAVE];
+  double a0 = 0.0, b0 = 0.0, a1 = 0.0, b1 = 0.0;
+  unsigned j = threadIdx.x;
+  for (; j + FT < n; j += 2 * FT) {
+    a0 += part[2 * j];
+    b0 += part[2 * j + 1];
+    a1 += part[2 * (j + FT)];
+    b1 += part[2 * (j + FT) + 1];
+  }
+  if (j < n) {
+    a0 += part[2 * j];
+    b0 += part[2 * j + 1];
+  }
+  const double rz = ops_block_reduce<ops_red_inc<double>>(a0 + a1, lds);
+  const double rzp = ops_block_reduce<ops_red_inc<double>>(b0 + b1, lds);
+  if (threadIdx.x == 0) {
+    S->beta = (S->iters == 0) ? 0.0 : (rz - rzp) / S->rz;
+    S->rz = rz;
+    S->rzp = rzp;
+  }
+}
+
+// p_out = z + beta p_in at the five stencil points, w = A p_out (FP64), p.w partials
+__global__ __launch_bounds__(NT) void k_pw(const double *__restrict__ Kx, const double *__restrict__ Ky,
+                                           const float *__restrict__ z, const double *__restrict__ pin,
+                                           double *__restrict__ pout, double *__restrict__ w, Fine g,
+                                           double *part, const Rec *S) {
+  if (S->stopped) return;
+  const double beta = S->beta;
+  long i;
+  double v = 0.0;
+  if (fcell(g, i)) {
+    const int s = g.s;
+    const double pc = (double)z[i] + beta * pin[i];
+    const double pe = (double)z[i + 1] + beta * pin[i + 1];
+    const double pwst = (double)z[i - 1] + beta * pin[i - 1];
+    const double pn = (double)z[i + s] + beta * pin[i + s];
+    const double ps = (double)z[i - s] + beta * pin[i - s];
+    const double kx0 = Kx[i], kx1 = Kx[i + 1], ky0 = Ky[i], ky1 = Ky[i + s];
+    const double wi = (1.0 + g.ry * (ky1 + ky0) + g.rx * (kx1 + kx0)) * pc - g.ry * (ky1 * pn + ky0 * ps) -
+                      g.rx * (kx1 * pe + kx0 * pwst);
+    pout[i] = pc;
+    w[i] = wi;
+    v = wi * pc;
+  }
+  write_partial(v, part);
+}
+
+__global__ __launch_bounds__(FT) void k_fold_pw(const double *part, unsigned n, Rec *S) {
+  if (S->stopped) return;
+  const double pw = fold(part, n);
+  if (threadIdx.x == 0) {
+    S->pw = pw;
+    S->alpha = S->rz / pw;
+  }
+}
+
+// u += alpha p, r -= alpha w, b32 = (float) r for the next V-cycle, r.r partials
+__global__ __launch_bounds__(NT) void k_ur(double *__restrict__ u, double *__restrict__ r,
+                                           const double *__restrict__ p, const double *__restrict__ w,
+                                           float *__restrict__ b32, Fine g, double *part, const Rec *S) {
+  if (S->stopped) return;
+  const double alpha = S->alpha;
+  long i;
+  double v = 0.0;
+  if (fcell(g, i)) {
+    u[i] = u[i] + alpha * p[i];
+    const double ri = r[i] - alpha * w[i];
+    r[i] = ri;
+    b32[i] = (float)ri;
+    v = ri * ri;
+  }
+  write_partial(v, part);
+}
+
+__global__ __launch_bounds__(FT) void k_fold_rr(const double *part, unsigned n, Rec *S) {
+  if (S->stopped) return;
+  const double rr = fold(part, n);
+  if (threadIdx.x == 0) {
+    S->rr = rr;
+    S->iters = S->iters + 1;
+    S->err = sqrt(fabs(rr));
+    if (S->err < S->thresh) S->stopped = 1;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+
+struct MG {
+  int n = 0;
+  std::vector<Level> lv;
+  void *mem = nullptr;
+  size_t cap = 0;
+  Rec *rec = nullptr, *host = nullptr;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_n = -1;
+  const void *graph_key[8] = {};
+};
+MG G;
+
+void drop_graphs() {
+  for (auto &e : G.graph)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  G.graph_n = -1;
+}
+
+char *carve(char *&p, size_t bytes) {
+  char *r = p;
+  p += (bytes + 255) & ~(size_t)255;
+  return r;
+}
+
+// V-cycle on the levels; level 0's b must hold the right-hand side, x0 receives z
+void vcycle(hipStream_t st, float *x0) {
+  const dim3 blk(BX, BY);
+  const int L = (int)G.lv.size();
+  std::vector<Level> lv = G.lv;
+  lv[0].x = x0;
+  auto sgrid = [](const Level &l) { return grid2((l.n + 1) / 2, l.n); };
+  for (int l = 0; l + 1 < L; l++) {
+    hipLaunchKernelGGL(k_smooth, sgrid(lv[l]), blk, 0, st, lv[l], 0, 1, G.rec);
+    hipLaunchKernelGGL(k_smooth, sgrid(lv[l]), blk, 0, st, lv[l], 1, 0, G.rec);
+    hipLaunchKernelGGL(k_restrict, grid2(lv[l + 1].n, lv[l + 1].n), blk, 0, st, lv[l], lv[l + 1], G.rec);
+  }
+  const Level &c = lv[L - 1];
+  for (int k = 0; k < COARSE_SWEEPS; k++) {
+    hipLaunchKernelGGL(k_smooth, sgrid(c), blk, 0, st, c, 0, k == 0 ? 1 : 0, G.rec);
+    hipLaunchKernelGGL(k_smooth, sgrid(c), blk, 0, st, c, 1, 0, G.rec);
+  }
+  for (int k = 0; k < COARSE_SWEEPS; k++) {
+    hipLaunchKernelGGL(k_smooth, sgrid(c), blk, 0, st, c, 1, 0, G.rec);
+    hipLaunchKernelGGL(k_smooth, sgrid(c), blk, 0, st, c, 0, 0, G.rec);
+  }
+  for (int l = L - 2; l >= 0; l--) {
+    hipLaunchKernelGGL(k_prolong, grid2(lv[l].n, lv[l].n), blk, 0, st, lv[l], lv[l + 1], G.rec);
+    hipLaunchKernelGGL(k_smooth, sgrid(lv[l]), blk, 0, st, lv[l], 1, 0, G.rec);
+    hipLaunchKernelGGL(k_smooth, sgrid(lv[l]), blk, 0, st, lv[l], 0, 0, G.rec);
+  }
+}
+
+}  // namespace
+
+void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat Kx, ops_dat Ky,
+                     const tea_native_solve_params *P, tea_native_result *R) {
+  (void)u0;
+  if (P->nx != P->ny)
+    throw OPSException(OPS_NOT_IMPLEMENTED, "tealeaf multigrid solver: square meshes only");
+  ops_dat all[] = {u, u0, r, p, w, Kx, Ky};
+  for (ops_dat d : all) {
+    ops_memspace ms = OPS_DEVICE;
+    ops_dat_get_raw_pointer(d, 0, nullptr, &ms);  // uploads a host-newer copy
+    ops_dat_release_raw_data(d, 0, OPS_RW);
+  }
+  hipStream_t st = (hipStream_t)ops_hip_get_stream();
+  const int n = P->nx;
+  const Fine g{n, n + 2, P->rx, P->ry};
+  const dim3 blk(BX, BY), gf = grid2(n, n);
+  const unsigned nb = gf.x * gf.y;
+  // levels
+  std::vector<int> sizes{n};
+  while (sizes.back() > COARSEST && sizes.back() % 2 == 0) sizes.push_back(sizes.back() / 2);
+  size_t bytes = 4 * (((size_t)nb * 2 * sizeof(double) + 255) & ~(size_t)255);
+  const size_t fine64 = (((size_t)(n + 2) * (n + 2) * sizeof(double)) + 255) & ~(size_t)255;
+  bytes += fine64;  // second p buffer
+  for (int s : sizes) bytes += 4 * ((((size_t)(s + 2) * (s + 2) * sizeof(float)) + 255) & ~(size_t)255);
+  bytes += (((size_t)(n + 2) * (n + 2) * sizeof(float)) + 255) & ~(size_t)255;  // second z buffer
+  if (bytes > G.cap || G.n != n) {
+    ops_hip_device_sync();
+    drop_graphs();
+    if (G.mem) CK(hipFree(G.mem));
+    CK(hipMalloc(&G.mem, bytes));
+    G.cap = bytes;
+    G.n = n;
+  }
+  if (!G.rec) {
+    CK(hipMalloc((void **)&G.rec, sizeof(Rec)));
+    CK(hipHostMalloc((void **)&G.host, sizeof(Rec), hipHostMallocDefault));
+  }
+  char *q = (char *)G.mem;
+  double *part = (double *)carve(q, (size_t)nb * 2 * sizeof(double));
+  double *pb = (double *)carve(q, fine64);
+  G.lv.clear();
+  float mass = 1.f;
+  for (int s : sizes) {
+    const size_t lb = (size_t)(s + 2) * (s + 2) * sizeof(float);
+    Level L{s, s + 2, mass, (float *)carve(q, lb), (float *)carve(q, lb), (float *)carve(q, lb),
+            (float *)carve(q, lb)};
+    G.lv.push_back(L);
+    mass *= 4.f;
+  }
+  float *z2 = (float *)carve(q, (size_t)(n + 2) * (n + 2) * sizeof(float));
+  float *zbuf[2] = {G.lv[0].x, z2};
+  // zero every array once (halos stay zero), then build the coefficients
+  CK(hipMemsetAsync(G.mem, 0, bytes, st));
+  double *u64 = (double *)u->data_d, *r64 = (double *)r->data_d, *pa = (double *)p->data_d,
+         *w64 = (double *)w->data_d;
+  const double *kx = (const double *)Kx->data_d, *ky = (const double *)Ky->data_d;
+  hipLaunchKernelGGL(k_fine_coef, grid2(n + 1, n + 1), blk, 0, st, kx, ky, G.lv[0].cx, G.lv[0].cy, n,
+                     P->rx, P->ry);
+  for (size_t l = 1; l < G.lv.size(); l++)
+    hipLaunchKernelGGL(k_coarsen, grid2(G.lv[l].n + 1, G.lv[l].n + 1), blk, 0, st, G.lv[l - 1].cx,
+                       G.lv[l - 1].cy, G.lv[l - 1].s, G.lv[l].cx, G.lv[l].cy, G.lv[l].n);
+  // start: r holds u0 - A u; b32 = r
+  const double thresh = P->eps * P->initial_residual;
+  hipLaunchKernelGGL(k_start, gf, blk, 0, st, r64, G.lv[0].b, g, part);
+  hipLaunchKernelGGL(k_fold_start, dim3(1), dim3(FT), 0, st, part, nb, G.rec, thresh);
+  ops_hip_check_launch("tealeaf multigrid set-up");
+  // one PCG iteration with z in zbuf[par], the previous z in zbuf[1-par], p written to
+  // the buffer of parity par
+  auto iteration = [&](int par) {
+    float *z = zbuf[par], *zp = zbuf[1 - par];
+    const double *pin = par ? pa : pb;
+    double *pout = par ? pb : pa;
+    vcycle(st, z);
+    hipLaunchKernelGGL(k_dots, gf, blk, 0, st, r64, z, zp, g, part, G.rec);
+    hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
+    hipLaunchKernelGGL(k_pw, gf, blk, 0, st, kx, ky, z, pin, pout, w64, g, part, G.rec);
+    hipLaunchKernelGGL(k_fold_pw, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
+    hipLaunchKernelGGL(k_ur, gf, blk, 0, st, u64, r64, pout, w64, G.lv[0].b, g, part, G.rec);
+    hipLaunchKernelGGL(k_fold_rr, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
+  };
+  // capture the two parities as graphs (rebuilt when the arrays move)
+  const void *key[8] = {G.mem, u64, r64, pa, w64, kx, ky, (const void *)(long)n};
+  bool same = G.graph[0] != nullptr;
+  for (int k = 0; k < 8 && same; k++) same = key[k] == G.graph_key[k];
+  if (!same) {
+    drop_graphs();
+    for (int par = 0; par < 2; par++) {
+      hipGraph_t gr;
+      CK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      iteration(par);
+      CK(hipStreamEndCapture(st, &gr));
+      CK(hipGraphInstantiate(&G.graph[par], gr, nullptr, nullptr, 0));
+      CK(hipGraphDestroy(gr));
+    }
+    for (int k = 0; k < 8; k++) G.graph_key[k] = key[k];
+  }
+  // iterations 1, 2, ...: odd iterations use parity 1 (p from pa into pb)
+  int issued = 0;
+  bool pending = false;
+  hipEvent_t ev;
+  CK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  while (issued < P->max_iters) {
+    for (int k = 0; k < POLL && issued < P->max_iters; k++) {
+      CK(hipGraphLaunch(G.graph[(issued + 1) & 1], st));
+      issued++;
+    }
+    if (pending) {
+      CK(hipEventSynchronize(ev));
+      if (G.host->stopped) break;
+    }
+    CK(hipMemcpyAsync(G.host, G.rec, sizeof(Rec), hipMemcpyDeviceToHost, st));
+    CK(hipEventRecord(ev, st));
+    pending = true;
+  }
+  CK(hipMemcpyAsync(G.host, G.rec, sizeof(Rec), hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  CK(hipEventDestroy(ev));
+  const Rec h = *G.host;
+  // odd iteration counts leave the last p in the second buffer
+  if (h.iters & 1)
+    CK(hipMemcpyAsync(pa, pb, (size_t)(n + 2) * (n + 2) * sizeof(double), hipMemcpyDeviceToDevice, st));
+  R->iterations = h.iters;
+  R->outer = (int)G.lv.size();
+  R->error = h.err;
+  u->dirty_hd = 2;
+  r->dirty_hd = 2;
+  p->dirty_hd = 2;
+  w->dirty_hd = 2;
+}

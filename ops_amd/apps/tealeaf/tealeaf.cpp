@@ -12,6 +12,7 @@
 //   0  reference path: CG (or Jacobi) iterations as par_loops with host-side alpha/beta
 //   1  FP64 CG with the scalars kept on the device (native kernels, tealeaf_native.hip)
 //   2  FP32-storage CG iterations with FP64 reliable updates (residual replacement)
+//   3  FP64 flexible PCG with an FP32 multigrid V-cycle preconditioner
 // Solver 1 and 2 stop on the same test as the reference: sqrt(r.r) below eps times the
 // residual norm of the step's initial guess.
 #include <math.h>
@@ -403,7 +404,10 @@ void solve() {
     P.max_iters = T.max_iters;
     P.initial_residual = initial;
     tea_native_result R;
-    tea_native_cg(T.u, T.u0, T.r, T.p, T.w, T.z, T.Kx, T.Ky, &P, &R);
+    if (T.solver == 3)
+      tea_native_mgcg(T.u, T.u0, T.r, T.p, T.w, T.Kx, T.Ky, &P, &R);
+    else
+      tea_native_cg(T.u, T.u0, T.r, T.p, T.w, T.z, T.Kx, T.Ky, &P, &R);
     n = R.iterations + 1;
     error = R.error;
   } else {
@@ -552,7 +556,7 @@ extern "C" int tealeaf_init(int argc, const char **argv, const char *deck) {
 }
 
 extern "C" int tealeaf_set_solver(int solver) {
-  if (solver < 0 || solver > 2) return 1;
+  if (solver < 0 || solver > 3) return 1;
   T.solver = solver;
   return 0;
 }

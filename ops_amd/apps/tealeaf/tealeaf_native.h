@@ -38,4 +38,11 @@ typedef struct {
 void tea_native_cg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat z, ops_dat Kx,
                    ops_dat Ky, const tea_native_solve_params *P, tea_native_result *R);
 
+// solver 3: FP64 flexible preconditioned CG; the preconditioner is one FP32 multigrid
+// V-cycle (2 x 2 aggregation with Galerkin coarse operators, red-black Gauss-Seidel
+// smoothing, coarse correction scaled by 1.5). Same stop rule as above. Square meshes.
+// R->outer reports the number of multigrid levels.
+void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat Kx, ops_dat Ky,
+                     const tea_native_solve_params *P, tea_native_result *R);
+
 #endif

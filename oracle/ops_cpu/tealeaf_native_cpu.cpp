@@ -184,3 +184,167 @@ void tea_native_cg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_d
   R->outer = outer;
   R->error = err;
 }
+
+// ---------------------------------------------------------------- solver 3 (multigrid PCG)
+// Same algorithm as the device solver: FP64 flexible PCG; preconditioner = one FP32
+// V-cycle over 2 x 2 aggregation levels (Galerkin faces = sum of the two fine faces,
+// mass x4 per level), red-black Gauss-Seidel (red, black before; black, red after), the
+// coarse correction scaled by 1.5, 8 symmetric sweep pairs on the coarsest level.
+namespace {
+
+struct CpuLevel {
+  int n, s;
+  float m;
+  std::vector<float> cx, cy, x, b;
+};
+
+void cpu_smooth(CpuLevel &L, int color, bool init) {
+  const int s = L.s;
+  for (int y = 1; y <= L.n; y++)
+    for (int x = 1; x <= L.n; x++) {
+      const long i = x + (long)y * s;
+      if ((x + y) % 2 != color) {
+        if (init) L.x[i] = 0.f;
+        continue;
+      }
+      const float c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
+      const float diag = L.m + ((c0 + c1) + (c2 + c3));
+      float v = L.b[i];
+      if (!init) v += (c0 * L.x[i - 1] + c1 * L.x[i + 1]) + (c2 * L.x[i - s] + c3 * L.x[i + s]);
+      L.x[i] = v / diag;
+    }
+}
+
+void cpu_vcycle(std::vector<CpuLevel> &lv, size_t l) {
+  CpuLevel &L = lv[l];
+  if (l + 1 == lv.size()) {
+    for (int k = 0; k < 8; k++) {
+      cpu_smooth(L, 0, k == 0);
+      cpu_smooth(L, 1, false);
+    }
+    for (int k = 0; k < 8; k++) {
+      cpu_smooth(L, 1, false);
+      cpu_smooth(L, 0, false);
+    }
+    return;
+  }
+  cpu_smooth(L, 0, true);
+  cpu_smooth(L, 1, false);
+  CpuLevel &C = lv[l + 1];
+  const int s = L.s;
+  for (int Y = 1; Y <= C.n; Y++)
+    for (int X = 1; X <= C.n; X++) {
+      float acc = 0.f;
+      for (int dy = 0; dy < 2; dy++)
+        for (int dx = 0; dx < 2; dx++) {
+          const long i = (2 * X - 1 + dx) + (long)(2 * Y - 1 + dy) * s;
+          const float c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
+          const float diag = L.m + ((c0 + c1) + (c2 + c3));
+          const float ax =
+              diag * L.x[i] - ((c0 * L.x[i - 1] + c1 * L.x[i + 1]) + (c2 * L.x[i - s] + c3 * L.x[i + s]));
+          acc += L.b[i] - ax;
+        }
+      C.b[X + (long)Y * C.s] = acc;
+    }
+  cpu_vcycle(lv, l + 1);
+  for (int y = 1; y <= L.n; y++)
+    for (int x = 1; x <= L.n; x++)
+      L.x[x + (long)y * s] += 1.5f * C.x[(x + 1) / 2 + (long)((y + 1) / 2) * C.s];
+  cpu_smooth(L, 1, false);
+  cpu_smooth(L, 0, false);
+}
+
+}  // namespace
+
+void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat Kx, ops_dat Ky,
+                     const tea_native_solve_params *P, tea_native_result *R) {
+  (void)u0;
+  if (P->nx != P->ny) {
+    R->iterations = 0;
+    R->outer = 0;
+    R->error = -1.0;
+    return;
+  }
+  const int n = P->nx;
+  const long s = n + 2, tot = s * s;
+  double *U = (double *)u->data, *Rr = (double *)r->data, *Pp = (double *)p->data, *Ww = (double *)w->data;
+  const double *kx = (const double *)Kx->data, *ky = (const double *)Ky->data;
+  std::vector<CpuLevel> lv;
+  std::vector<int> sizes{n};
+  while (sizes.back() > 16 && sizes.back() % 2 == 0) sizes.push_back(sizes.back() / 2);
+  float mass = 1.f;
+  for (int sz : sizes) {
+    const long a = (long)(sz + 2) * (sz + 2);
+    lv.push_back(CpuLevel{sz, sz + 2, mass, std::vector<float>(a, 0.f), std::vector<float>(a, 0.f),
+                          std::vector<float>(a, 0.f), std::vector<float>(a, 0.f)});
+    mass *= 4.f;
+  }
+  for (int y = 1; y <= n + 1; y++)
+    for (int x = 1; x <= n + 1; x++) {
+      const long i = x + y * s;
+      lv[0].cx[i] = (y <= n) ? (float)(P->rx * kx[i]) : 0.f;
+      lv[0].cy[i] = (x <= n) ? (float)(P->ry * ky[i]) : 0.f;
+    }
+  for (size_t l = 1; l < lv.size(); l++) {
+    const CpuLevel &F = lv[l - 1];
+    CpuLevel &C = lv[l];
+    for (int Y = 1; Y <= C.n + 1; Y++)
+      for (int X = 1; X <= C.n + 1; X++) {
+        const long ic = X + (long)Y * C.s;
+        const int fx = 2 * X - 1, fy = 2 * Y - 1;
+        C.cx[ic] = (Y <= C.n) ? F.cx[fx + (long)fy * F.s] + F.cx[fx + (long)(fy + 1) * F.s] : 0.f;
+        C.cy[ic] = (X <= C.n) ? F.cy[fx + (long)fy * F.s] + F.cy[fx + 1 + (long)fy * F.s] : 0.f;
+      }
+  }
+  const double thresh = P->eps * P->initial_residual;
+  std::vector<float> zprev(tot, 0.f);
+  std::vector<double> pn(tot, 0.0);
+  double rz_old = 0.0, err = P->initial_residual;
+  for (long i = 0; i < tot; i++) lv[0].b[i] = 0.f;
+  int it = 0;
+  while (it < P->max_iters) {
+    for (int y = 1; y <= n; y++)
+      for (int x = 1; x <= n; x++) lv[0].b[x + y * s] = (float)Rr[x + y * s];
+    cpu_vcycle(lv, 0);
+    const std::vector<float> &z = lv[0].x;
+    double rz = 0.0, rzp = 0.0;
+    for (int y = 1; y <= n; y++)
+      for (int x = 1; x <= n; x++) {
+        const long i = x + y * s;
+        rz += Rr[i] * (double)z[i];
+        rzp += Rr[i] * (double)zprev[i];
+      }
+    const double beta = it == 0 ? 0.0 : (rz - rzp) / rz_old;
+    rz_old = rz;
+    for (long i = 0; i < tot; i++) pn[i] = 0.0;
+    for (int y = 1; y <= n; y++)
+      for (int x = 1; x <= n; x++) {
+        const long i = x + y * s;
+        pn[i] = (double)z[i] + beta * Pp[i];
+      }
+    double pw = 0.0;
+    for (int y = 1; y <= n; y++)
+      for (int x = 1; x <= n; x++) {
+        const long i = x + y * s;
+        Ww[i] = apply_A<double, double>(kx, ky, pn.data(), i, s, P->rx, P->ry);
+        pw += Ww[i] * pn[i];
+      }
+    const double alpha = rz / pw;
+    double rr = 0.0;
+    for (int y = 1; y <= n; y++)
+      for (int x = 1; x <= n; x++) {
+        const long i = x + y * s;
+        Pp[i] = pn[i];
+        U[i] = U[i] + alpha * pn[i];
+        Rr[i] = Rr[i] - alpha * Ww[i];
+        rr += Rr[i] * Rr[i];
+      }
+    zprev = z;
+    it++;
+    err = sqrt(fabs(rr));
+    if (err < thresh) break;
+  }
+  R->iterations = it;
+  R->outer = (int)lv.size();
+  R->error = err;
+}

@@ -104,7 +104,7 @@ def test_oracle_cg_true_residual_meets_tolerance():
     assert np.sqrt(np.sum(r * r)) < 2e-10 * st[4]
 
 
-@pytest.mark.parametrize("solver", [1, 2])
+@pytest.mark.parametrize("solver", [1, 2, 3])
 def test_oracle_native_solvers_agree_with_plain_cg(solver):
     from ops_amd.apps.tealeaf.decks import tea_deck
     deck = tea_deck(96, 2)
@@ -115,6 +115,9 @@ def test_oracle_native_solvers_agree_with_plain_cg(solver):
     assert np.sqrt(np.sum(r * r)) < 1.5e-10 * got["per_step"][-1][4]
     u0, u1 = ref["fields"]["u"], got["fields"]["u"]
     assert np.max(np.abs(u1 - u0)) / np.max(np.abs(u0)) < 1e-8
+    if solver == 3:
+        # the multigrid preconditioner cuts the iteration count several fold
+        assert got["per_step"][-1][2] * 4 < ref["per_step"][-1][2]
 
 
 # ---------------------------------------------------------------- GPU path
@@ -137,7 +140,7 @@ def test_gpu_plain_path_vs_reference(golden, tmp_path, name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("solver", [1, 2])
+@pytest.mark.parametrize("solver", [1, 2, 3])
 def test_gpu_native_solvers(solver):
     from ops_amd.apps.tealeaf.decks import tea_deck
     deck = tea_deck(256, 2)
