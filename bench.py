@@ -4,8 +4,8 @@ Default workload: CloverLeaf 2D, deck clover_bm16_short.in (3840 x 3840 cells, 8
 steps, FP64), through libops_hip.so and the translated application. A step is one hydro
 time step of the whole mesh; a cell-update is one cell advanced by one step.
 
-    python bench.py [--steps K] [--warmup W] [--workload cloverleaf|laplace]
-                    [--no-cpu] [--cpu-threads T]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--cpu-threads T]
+                    [--no-tealeaf] [--tea-cells N] [--tea-steps S]
 
 Prints one JSON line. `value` is measured with the inputs resident in HBM: the deck is
 generated on the device before the timed region; the K timed steps are bracketed by
@@ -13,7 +13,9 @@ device synchronisation. Per-kernel durations come from HIP events recorded on th
 runtime's stream around every launch inside the timed region; `roofline` is computed for
 the kernel with the largest total time from its ops_compute_transfer bytes per launch.
 `cpu_baseline` runs the same application on the oracle's host backend (OpenMP) for a
-bounded sample on this machine's cores.
+bounded sample on this machine's cores. `tealeaf` reports the second half of the metric:
+TeaLeaf 4096^2 time-to-solution of plain FP64 CG against the multigrid-preconditioned
+solver (FP64 outer CG, FP32 V-cycle), with residuals and field differences.
 """
 import argparse
 import ctypes
@@ -170,6 +172,67 @@ def clover_gpu(args, R):
     return secs, stats, checks
 
 
+def tealeaf_gpu(args):
+    """TeaLeaf 4096^2, 10 implicit steps to eps = 1e-10 (the reference's stop rule on the
+    CG residual): plain FP64 CG through the generated loops (the reference algorithm on
+    this backend) against the multigrid-preconditioned solver. A third run to eps = 1e-12
+    gives a tighter solution to measure both fields' errors against."""
+    import numpy as np
+    import ops_amd
+    from ops_amd.apps.tealeaf.decks import tea_deck
+    L = ops_amd.app("tealeaf")
+    R = ops_amd.runtime()
+    n, steps = args.tea_cells, args.tea_steps
+
+    def run(solver, eps):
+        with tempfile.TemporaryDirectory() as t:
+            deck = Path(t) / "tea.in"
+            deck.write_text(tea_deck(n, steps, eps=eps, max_iters=20000, extra=("tl_check_result",)))
+            argc, argv = ops_amd.argv()
+            assert L.tealeaf_init(argc, argv, str(deck).encode()) == 0
+            assert L.tealeaf_set_solver(solver) == 0
+            R.ops_hip_device_sync()
+            st = np.zeros(7)
+            iters, exact, rel = [], [], []
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                assert L.tealeaf_step(1) == 0
+                L.tealeaf_state(st.ctypes.data_as(ctypes.c_void_p))
+                iters.append(int(st[2]))
+                exact.append(float(st[6]))
+                rel.append(float(st[3] / st[4]))
+            R.ops_hip_device_sync()
+            t1 = time.perf_counter()
+            cnt = L.tealeaf_fetch(b"u", None)
+            u = np.zeros(cnt)
+            L.tealeaf_fetch(b"u", u.ctypes.data_as(ctypes.c_void_p))
+            s4 = np.zeros(4)
+            assert L.tealeaf_summary(s4.ctypes.data_as(ctypes.c_void_p)) == 0
+            assert L.tealeaf_finalize() == 0
+        log(f"[bench] tealeaf solver {solver} eps {eps}: {t1 - t0:.3f} s, {sum(iters)} iterations")
+        return {"s": t1 - t0, "iters": iters, "exact": exact, "rel": rel, "temp": s4[3], "u": u}
+
+    base = run(0, 1e-10)
+    mg = run(3, 1e-10)
+    tight = run(3, 1e-12)
+
+    def maxrel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    return {
+        "workload": f"tealeaf_2d {n}x{n}, {steps} steps, 8 seeded rectangles, tl_eps 1e-10",
+        "time_to_solution_s": {"fp64_cg_plain": round(base["s"], 4), "fp64_pcg_fp32_multigrid": round(mg["s"], 4)},
+        "speedup_vs_fp64_cg": round(base["s"] / mg["s"], 2),
+        "iterations": {"fp64_cg_plain": sum(base["iters"]), "fp64_pcg_fp32_multigrid": sum(mg["iters"])},
+        "max_relative_cg_residual": {"fp64_cg_plain": max(base["rel"]), "fp64_pcg_fp32_multigrid": max(mg["rel"])},
+        "final_exact_residual": {"fp64_cg_plain": base["exact"][-1], "fp64_pcg_fp32_multigrid": mg["exact"][-1]},
+        "field_max_rel_diff_vs_fp64_cg": maxrel(mg["u"], base["u"]),
+        "field_max_rel_err_vs_eps1e-12_solution": {"fp64_cg_plain": maxrel(base["u"], tight["u"]),
+                                                   "fp64_pcg_fp32_multigrid": maxrel(mg["u"], tight["u"])},
+        "temperature": {"fp64_cg_plain": base["temp"], "fp64_pcg_fp32_multigrid": mg["temp"]},
+    }
+
+
 def clover_cpu(args, threads):
     """Oracle host backend, same application and deck, bounded sample of steps."""
     lib = ctypes.CDLL(str(ROOT / "oracle" / "_build" / "libcloverleaf_cpu.so"),
@@ -207,6 +270,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-tealeaf", action="store_true")
+    ap.add_argument("--tea-cells", type=int, default=4096)
+    ap.add_argument("--tea-steps", type=int, default=10)
     ap.add_argument("--ingest-pmc", metavar="DIR", default=None,
                     help="read rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE CSV output under DIR "
                          "and write profiles/pmc_traffic.json, then exit")
@@ -247,6 +313,8 @@ def main():
         "step_bandwidth": whole_step_bw(stats, secs),
         "checks": checks,
     }
+    if not args.no_tealeaf:
+        out["tealeaf"] = tealeaf_gpu(args)
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] cpu baseline: {threads} threads, {args.cpu_steps} steps")
