@@ -30,6 +30,20 @@ SIGNATURES = {
         "tealeaf_fetch": (ctypes.c_long, [ctypes.c_char_p, _p]),
         "tealeaf_finalize": (_i, []),
     },
+    "apitest": {
+        "api_mblock": (_i, [_i, _pp, _i, _p, _p]),
+        "api_halo3d": (_i, [_i, _pp, _p, _p, _p, _i, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p]),
+        "api_tridiag": (_i, [_i, _pp, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
+        "api_reduce": (_i, [_i, _pp, _i, _i, _p, _p, _p]),
+        "api_random": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
+    },
+    "adi": {
+        "adi_init": (_i, [_i, _pp, _i, _p, _i, ctypes.c_ulonglong, _d]),
+        "adi_step": (_i, [_i]),
+        "adi_fetch": (ctypes.c_long, [_i, _p]),
+        "adi_set": (_i, [_i, _p]),
+        "adi_finalize": (_i, []),
+    },
 }
 
 

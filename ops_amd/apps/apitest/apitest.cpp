@@ -1,0 +1,298 @@
+// API test application: exercises the parts of the OPS API the benchmark apps do not
+// stress, each through an extern "C" entry point that runs a whole ops_init .. ops_exit
+// session and returns plain arrays.
+//   api_mblock     the reference's mblock test (apps/c/mblock/mblock.cpp:80-158): two
+//                  20 x 20 blocks with 2-deep halos, five halo groups including a flipped
+//                  axis and swapped axes
+//   api_halo3d     one 3-D halo between two dats, any orientation, any (float/double)
+//                  types, values written and read back through slab set/fetch
+//   api_tridiag    ops_tridMultiDimBatch[_Inc] on caller-provided coefficients
+//   api_reduce     INC/MIN/MAX reductions on double and int, a 2-component reduction,
+//                  accumulation across two loops before ops_reduction_result
+//   api_random     ops_randomgen_init + ops_fill_random_uniform on a dat, read back with
+//                  its halos
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#define OPS_2D
+#include <ops_seq_v2.h>
+
+#include "apitest_kernels.h"
+
+namespace {
+int fail(const char *where, const std::exception &e) {
+  fprintf(stderr, "%s: %s\n", where, e.what());
+  return 1;
+}
+
+// whole allocation (halos included) of a 2-D dat, x fastest
+void fetch_all(ops_dat d, double *out) {
+  int r[4] = {d->d_m[0], d->size[0] + d->d_m[0], d->d_m[1], d->size[1] + d->d_m[1]};
+  ops_dat_fetch_data_slab_memspace(d, 0, (char *)out, r, OPS_HOST);
+}
+}  // namespace
+
+// group_mask bit g runs halo group g (0..4) in that order; out0/out1 receive the 24 x 24
+// allocations of data0/data1
+extern "C" int api_mblock(int argc, const char **argv, int group_mask, double *out0, double *out1) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block grid0 = ops_decl_block(2, "grid0");
+    ops_block grid1 = ops_decl_block(2, "grid1");
+    int s00[] = {0, 0};
+    ops_stencil S00 = ops_decl_stencil(2, 1, s00, "00");
+    int d_p[2] = {2, 2}, d_m[2] = {-2, -2}, size[2] = {20, 20}, base[2] = {0, 0};
+    double *none = nullptr;
+    ops_dat data0 = ops_decl_dat(grid0, 1, size, base, d_m, d_p, none, "double", "data0");
+    ops_dat data1 = ops_decl_dat(grid1, 1, size, base, d_m, d_p, none, "double", "data1");
+    ops_halo_group groups[5];
+    {  // 0: x faces, same orientation
+      int it[] = {2, 20}, bf[] = {18, 0}, bt[] = {-2, 0}, dir[] = {1, 2};
+      ops_halo h0 = ops_decl_halo(data0, data1, it, bf, bt, dir, dir);
+      bf[0] = 0;
+      bt[0] = 20;
+      ops_halo h1 = ops_decl_halo(data1, data0, it, bf, bt, dir, dir);
+      ops_halo g[] = {h0, h1};
+      groups[0] = ops_decl_halo_group(2, g);
+    }
+    {  // 1: y faces, same orientation
+      int it[] = {20, 2}, bf[] = {0, 18}, bt[] = {0, -2}, dir[] = {1, 2};
+      ops_halo h0 = ops_decl_halo(data0, data1, it, bf, bt, dir, dir);
+      bf[1] = 0;
+      bt[1] = 20;
+      ops_halo h1 = ops_decl_halo(data1, data0, it, bf, bt, dir, dir);
+      ops_halo g[] = {h0, h1};
+      groups[1] = ops_decl_halo_group(2, g);
+    }
+    {  // 2: x faces, y flipped
+      int it[] = {2, 20}, bf[] = {0, 0}, bt[] = {20, 0}, dir[] = {1, 2}, dto[] = {1, -2};
+      ops_halo h0 = ops_decl_halo(data0, data1, it, bf, bt, dir, dto);
+      bf[0] = 18;
+      bt[0] = -2;
+      ops_halo h1 = ops_decl_halo(data1, data0, it, bf, bt, dto, dir);
+      ops_halo g[] = {h0, h1};
+      groups[2] = ops_decl_halo_group(2, g);
+    }
+    {  // 3: y faces, x flipped
+      int it[] = {20, 2}, bf[] = {0, 0}, bt[] = {0, 20}, dir[] = {1, 2}, dto[] = {-1, 2};
+      ops_halo h0 = ops_decl_halo(data0, data1, it, bf, bt, dir, dto);
+      bf[1] = 18;
+      bt[1] = -2;
+      ops_halo h1 = ops_decl_halo(data1, data0, it, bf, bt, dto, dir);
+      ops_halo g[] = {h0, h1};
+      groups[3] = ops_decl_halo_group(2, g);
+    }
+    {  // 4: x face of block 0 onto the y face of block 1 (axes swapped)
+      int it[] = {2, 20}, bf[] = {18, 0}, bt[] = {0, -2}, dir[] = {1, 2}, dto[] = {2, 1};
+      ops_halo h0 = ops_decl_halo(data0, data1, it, bf, bt, dir, dto);
+      bf[0] = 0;
+      bt[0] = 20;
+      bt[1] = 0;
+      ops_halo h1 = ops_decl_halo(data1, data0, it, bf, bt, dto, dir);
+      ops_halo g[] = {h0, h1};
+      groups[4] = ops_decl_halo_group(2, g);
+    }
+    int range[] = {0, 20, 0, 20};
+    ops_par_loop(api_populate2d, "mblock_populate_kernel", grid0, 2, range,
+                 ops_arg_dat(data0, 1, S00, "double", OPS_WRITE), ops_arg_idx());
+    ops_par_loop(api_populate2d, "mblock_populate_kernel", grid1, 2, range,
+                 ops_arg_dat(data1, 1, S00, "double", OPS_WRITE), ops_arg_idx());
+    for (int g = 0; g < 5; g++)
+      if (group_mask & (1 << g)) ops_halo_transfer(groups[g]);
+    fetch_all(data0, out0);
+    fetch_all(data1, out1);
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_mblock", e);
+  }
+}
+
+// One 3-D halo from dat A to dat B. sizes/d_m/d_p give each dat (3 ints each, A then B);
+// tA/tB are 0 for double, 1 for float. inA is A's whole allocation (x fastest, as doubles);
+// outB receives B's whole allocation after the transfer (B starts at zero).
+extern "C" int api_halo3d(int argc, const char **argv, const int *sizeA, const int *dmA,
+                          const int *dpA, int tA, const int *sizeB, const int *dmB, const int *dpB,
+                          int tB, const int *iter, const int *from_base, const int *to_base,
+                          const int *from_dir, const int *to_dir, const double *inA, double *outB) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block blk = ops_decl_block(3, "b3");
+    int base[3] = {0, 0, 0};
+    int s[3], m[3], p[3];
+    for (int i = 0; i < 3; i++) {
+      s[i] = sizeA[i];
+      m[i] = dmA[i];
+      p[i] = dpA[i];
+    }
+    ops_dat A = tA ? ops_decl_dat_char(blk, 1, s, base, m, p, nullptr, nullptr, 4, "float", "A")
+                   : ops_decl_dat_char(blk, 1, s, base, m, p, nullptr, nullptr, 8, "double", "A");
+    for (int i = 0; i < 3; i++) {
+      s[i] = sizeB[i];
+      m[i] = dmB[i];
+      p[i] = dpB[i];
+    }
+    ops_dat B = tB ? ops_decl_dat_char(blk, 1, s, base, m, p, nullptr, nullptr, 4, "float", "B")
+                   : ops_decl_dat_char(blk, 1, s, base, m, p, nullptr, nullptr, 8, "double", "B");
+    long nA = (long)A->size[0] * A->size[1] * A->size[2];
+    long nB = (long)B->size[0] * B->size[1] * B->size[2];
+    int rA[6], rB[6];
+    for (int i = 0; i < 3; i++) {
+      rA[2 * i] = A->d_m[i];
+      rA[2 * i + 1] = A->size[i] + A->d_m[i];
+      rB[2 * i] = B->d_m[i];
+      rB[2 * i + 1] = B->size[i] + B->d_m[i];
+    }
+    if (tA) {
+      std::vector<float> f(nA);
+      for (long i = 0; i < nA; i++) f[i] = (float)inA[i];
+      ops_dat_set_data_slab_memspace(A, 0, (char *)f.data(), rA, OPS_HOST);
+    } else {
+      ops_dat_set_data_slab_memspace(A, 0, (char *)inA, rA, OPS_HOST);
+    }
+    int it[3], bf[3], bt[3], fd[3], td[3];
+    for (int i = 0; i < 3; i++) {
+      it[i] = iter[i];
+      bf[i] = from_base[i];
+      bt[i] = to_base[i];
+      fd[i] = from_dir[i];
+      td[i] = to_dir[i];
+    }
+    ops_halo h = ops_decl_halo(A, B, it, bf, bt, fd, td);
+    ops_halo_group g = ops_decl_halo_group(1, &h);
+    ops_halo_transfer(g);
+    if (tB) {
+      std::vector<float> f(nB);
+      ops_dat_fetch_data_slab_memspace(B, 0, (char *)f.data(), rB, OPS_HOST);
+      for (long i = 0; i < nB; i++) outB[i] = f[i];
+    } else {
+      ops_dat_fetch_data_slab_memspace(B, 0, (char *)outB, rB, OPS_HOST);
+    }
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_halo3d", e);
+  }
+}
+
+// Tridiagonal batch on ndim-D dats of interior size `size` (one halo layer all round);
+// a, b, c, d, u are packed interiors (x fastest); the solve box is `range` (interior
+// coordinates). out receives d (inc = 0) or u (inc = 1) after the solve, packed.
+extern "C" int api_tridiag(int argc, const char **argv, int ndim, int solvedim, const int *size,
+                           const int *range, const double *a, const double *b, const double *c,
+                           const double *d, const double *u, int inc, double *out) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block blk = ops_decl_block(ndim, "trid");
+    int sz[3], base[3] = {0, 0, 0}, dm[3] = {-1, -1, -1}, dp[3] = {1, 1, 1}, full[6];
+    for (int i = 0; i < ndim; i++) {
+      sz[i] = size[i];
+      full[2 * i] = 0;
+      full[2 * i + 1] = size[i];
+    }
+    double *none = nullptr;
+    ops_dat A = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "a");
+    ops_dat B = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "b");
+    ops_dat C = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "c");
+    ops_dat D = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "d");
+    ops_dat U = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "u");
+    ops_dat_set_data_slab_memspace(A, 0, (char *)a, full, OPS_HOST);
+    ops_dat_set_data_slab_memspace(B, 0, (char *)b, full, OPS_HOST);
+    ops_dat_set_data_slab_memspace(C, 0, (char *)c, full, OPS_HOST);
+    ops_dat_set_data_slab_memspace(D, 0, (char *)d, full, OPS_HOST);
+    ops_dat_set_data_slab_memspace(U, 0, (char *)u, full, OPS_HOST);
+    int r[6];
+    for (int i = 0; i < 2 * ndim; i++) r[i] = range[i];
+    ops_tridsolver_params ctx(blk);
+    if (inc)
+      ops_tridMultiDimBatch_Inc(ndim, solvedim, r, A, B, C, D, U, &ctx);
+    else
+      ops_tridMultiDimBatch(ndim, solvedim, r, A, B, C, D, &ctx);
+    ops_dat_fetch_data_slab_memspace(inc ? U : D, 0, (char *)out, full, OPS_HOST);
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_tridiag", e);
+  }
+}
+
+// Reductions over the nx x ny interior: vals (double) and ivals (int), packed. The loop
+// runs twice over the two halves (rows [0, ny/2) and [ny/2, ny)) before one
+// ops_reduction_result per handle. out: sum, min, max, isum, imin, imax (as doubles),
+// pair[0], pair[1].
+extern "C" int api_reduce(int argc, const char **argv, int nx, int ny, const double *vals,
+                          const int *ivals, double *out) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block blk = ops_decl_block(2, "red");
+    int size[2] = {nx, ny}, base[2] = {0, 0}, dm[2] = {-1, -1}, dp[2] = {1, 1};
+    double *none = nullptr;
+    int *inone = nullptr;
+    ops_dat V = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "v");
+    ops_dat W = ops_decl_dat(blk, 1, size, base, dm, dp, inone, "int", "w");
+    int full[4] = {0, nx, 0, ny};
+    ops_dat_set_data_slab_memspace(V, 0, (char *)vals, full, OPS_HOST);
+    ops_dat_set_data_slab_memspace(W, 0, (char *)ivals, full, OPS_HOST);
+    int s00[] = {0, 0};
+    ops_stencil S00 = ops_decl_stencil(2, 1, s00, "00");
+    ops_reduction hs = ops_decl_reduction_handle(sizeof(double), "double", "sum");
+    ops_reduction hmn = ops_decl_reduction_handle(sizeof(double), "double", "min");
+    ops_reduction hmx = ops_decl_reduction_handle(sizeof(double), "double", "max");
+    ops_reduction his = ops_decl_reduction_handle(sizeof(int), "int", "isum");
+    ops_reduction himn = ops_decl_reduction_handle(sizeof(int), "int", "imin");
+    ops_reduction himx = ops_decl_reduction_handle(sizeof(int), "int", "imax");
+    ops_reduction hp = ops_decl_reduction_handle(2 * sizeof(double), "double", "pair");
+    for (int half = 0; half < 2; half++) {
+      int r[4] = {0, nx, half ? ny / 2 : 0, half ? ny : ny / 2};
+      ops_par_loop(api_reduce, "api_reduce", blk, 2, r, ops_arg_dat(V, 1, S00, "double", OPS_READ),
+                   ops_arg_dat(W, 1, S00, "int", OPS_READ), ops_arg_reduce(hs, 1, "double", OPS_INC),
+                   ops_arg_reduce(hmn, 1, "double", OPS_MIN), ops_arg_reduce(hmx, 1, "double", OPS_MAX),
+                   ops_arg_reduce(his, 1, "int", OPS_INC), ops_arg_reduce(himn, 1, "int", OPS_MIN),
+                   ops_arg_reduce(himx, 1, "int", OPS_MAX), ops_arg_reduce(hp, 2, "double", OPS_INC));
+    }
+    double s, mn, mx, pair[2];
+    int is, imn, imx;
+    ops_reduction_result(hs, &s);
+    ops_reduction_result(hmn, &mn);
+    ops_reduction_result(hmx, &mx);
+    ops_reduction_result(his, &is);
+    ops_reduction_result(himn, &imn);
+    ops_reduction_result(himx, &imx);
+    ops_reduction_result(hp, pair);
+    out[0] = s;
+    out[1] = mn;
+    out[2] = mx;
+    out[3] = is;
+    out[4] = imn;
+    out[5] = imx;
+    out[6] = pair[0];
+    out[7] = pair[1];
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_reduce", e);
+  }
+}
+
+// ops_fill_random_uniform on an nx x ny double dat with one halo layer; out receives the
+// whole allocation ((nx+2)*(ny+2), x fastest)
+extern "C" int api_random(int argc, const char **argv, unsigned seed, int nx, int ny, double *out) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block blk = ops_decl_block(2, "rng");
+    int size[2] = {nx, ny}, base[2] = {0, 0}, dm[2] = {-1, -1}, dp[2] = {1, 1};
+    double *none = nullptr;
+    ops_dat V = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "v");
+    ops_randomgen_init(seed, 0);
+    ops_fill_random_uniform(V);
+    fetch_all(V, out);
+    ops_randomgen_exit();
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_random", e);
+  }
+}

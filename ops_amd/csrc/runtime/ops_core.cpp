@@ -13,6 +13,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -533,6 +534,31 @@ extern "C" void ops_reduction_result_char(ops_reduction handle, int type_size, c
 
 // ---------------------------------------------------------------- halos
 
+// Every index a halo reads or writes must lie inside the dat's allocation: the window of
+// dat dimension i starts at base[i] - d_m[i] - dat->base[i] and spans
+// iter_size[|dir[i]| - 1] cells, whichever way it is walked.
+static void check_halo_side(ops_dat dat, const int *base, const int *dir, const int *iter,
+                            const char *which) {
+  const int nd = dat->block->dims;
+  for (int i = 0; i < nd; i++) {
+    const int ax = abs(dir[i]) - 1;
+    if (ax < 0 || ax >= nd)
+      ops_throw(OPS_INVALID_ARGUMENT,
+                std::string("Error: ops_decl_halo -- invalid direction for ") + which + " dat " +
+                    dat->name);
+    const long first = (long)base[i] - dat->d_m[i] - dat->base[i];
+    const long last = first + iter[ax] - 1;
+    if (iter[ax] < 0 || first < 0 || last >= dat->size[i]) {
+      char buf[400];
+      snprintf(buf, sizeof buf,
+               "Error: ops_decl_halo -- the %s window of dat %s spans %ld..%ld in dimension %d, "
+               "outside its allocation 0..%d",
+               which, dat->name, first, last, i, dat->size[i] - 1);
+      ops_throw(OPS_INVALID_ARGUMENT, buf);
+    }
+  }
+}
+
 extern "C" ops_halo ops_decl_halo(ops_dat from, ops_dat to, int *iter_size, int *from_base,
                                   int *to_base, int *from_dir, int *to_dir) {
   OPS_instance *in = ops_instance_get();
@@ -555,6 +581,8 @@ extern "C" ops_halo ops_decl_halo(ops_dat from, ops_dat to, int *iter_size, int 
     h->from_dir[i] = i + 1;
     h->to_dir[i] = i + 1;
   }
+  check_halo_side(from, h->from_base, h->from_dir, h->iter_size, "from");
+  check_halo_side(to, h->to_base, h->to_dir, h->iter_size, "to");
   in->halos.push_back(h);
   return h;
 }
@@ -892,8 +920,13 @@ static void fill_random(ops_dat dat, int normal) {
       std::uniform_real_distribution<float> dist(0.0f, 1.0f);
       for (size_t i = 0; i < n; i++) ((float *)dat->data)[i] = dist(g_rng);
     }
+  } else if (!strcmp(dat->type, "int") && !normal) {
+    std::uniform_int_distribution<int> dist(0, INT_MAX);
+    for (size_t i = 0; i < n; i++) ((int *)dat->data)[i] = dist(g_rng);
   } else {
-    ops_throw(OPS_NOT_IMPLEMENTED, "Error: random fill supports double and float dats");
+    ops_throw(OPS_RUNTIME_ERROR,
+              std::string("Error: random number generation not implemented for data type: ") +
+                  dat->type);
   }
   dat->dirty_hd = 1;
   ops_put_data(dat);

@@ -412,6 +412,17 @@ extern "C" ops_halo ops_decl_halo(ops_dat from, ops_dat to, int *iter, int *fb, 
     h->from_dir[i] = in ? fd[i] : i + 1;
     h->to_dir[i] = in ? td[i] : i + 1;
   }
+  // window inside each dat's allocation (same check as the device runtime)
+  ops_dat dd[2] = {from, to};
+  const int *bb[2] = {h->from_base, h->to_base}, *rr[2] = {h->from_dir, h->to_dir};
+  for (int s = 0; s < 2; s++)
+    for (int i = 0; i < dd[s]->block->dims; i++) {
+      const int ax = abs(rr[s][i]) - 1;
+      if (ax < 0 || ax >= dd[s]->block->dims) fail(OPS_INVALID_ARGUMENT, "ops_decl_halo: direction");
+      const long first = (long)bb[s][i] - dd[s]->d_m[i] - dd[s]->base[i];
+      if (first < 0 || first + h->iter_size[ax] > dd[s]->size[i])
+        fail(OPS_INVALID_ARGUMENT, "ops_decl_halo: window outside the allocation");
+    }
   g->halos.push_back(h);
   return h;
 }
@@ -654,9 +665,11 @@ static void thomas(int ndim, int sd, const int *range, ops_dat A, ops_dat B, ops
   int o1 = sd == 0 ? 1 : 0, o2 = sd == 2 ? 1 : 2;
   long st = s[sd];
   int len = n[sd];
-  std::vector<double> cp(len), dp(len);
-  for (int q = 0; q < n[o2]; q++)
-    for (int p = 0; p < n[o1]; p++) {
+  // lines are independent: threads split them, each line is swept in the textbook order
+#pragma omp parallel for schedule(static)
+  for (long line = 0; line < (long)n[o2] * n[o1]; line++) {
+      std::vector<double> cp(len), dp(len);
+      const int q = (int)(line / n[o1]), p = (int)(line % n[o1]);
       long o = p * s[o1] + q * s[o2];
       cp[0] = c[o] / b[o];
       dp[0] = d[o] / b[o];

@@ -1,0 +1,286 @@
+"""API-level parity: halo transfers (the reference's mblock groups and 3-D orientations,
+mixed precision), batched tridiagonal solves, reductions and the host random generator.
+The oracle host backend is checked against closed forms and independent computations;
+the GPU runtime is checked against the oracle bitwise."""
+import math
+
+import numpy as np
+import pytest
+
+from app_util import argv, lib, p
+
+KINDS = ["cpu", pytest.param("gpu", marks=pytest.mark.gpu)]
+
+
+# ---------------------------------------------------------------- mblock halo groups
+
+def mblock(kind, mask):
+    L = lib("apitest", kind)
+    a0 = np.zeros((24, 24))
+    a1 = np.zeros((24, 24))
+    argc, av = argv()
+    assert L.api_mblock(argc, av, mask, p(a0), p(a1)) == 0
+    return a0, a1
+
+
+def mblock_expected(mask):
+    """Halo values from the geometry of each group (apps/c/mblock/mblock.cpp:80-158),
+    written out face by face. Arrays are [y + 2, x + 2] of the 24 x 24 allocation."""
+    d = [np.zeros((24, 24)), np.zeros((24, 24))]
+    for b in (0, 1):
+        for y in range(20):
+            for x in range(20):
+                d[b][y + 2, x + 2] = x + 20 * y
+
+    def at(b, x, y):
+        return d[b][y + 2, x + 2]
+
+    def put(b, x, y, v):
+        d[b][y + 2, x + 2] = v
+
+    for g in range(5):
+        if not mask & (1 << g):
+            continue
+        if g == 0:    # block 1 sits right of block 0
+            for i in range(2):
+                for y in range(20):
+                    put(1, -2 + i, y, at(0, 18 + i, y))
+            for i in range(2):
+                for y in range(20):
+                    put(0, 20 + i, y, at(1, i, y))
+        elif g == 1:  # block 1 sits above block 0
+            for j in range(2):
+                for x in range(20):
+                    put(1, x, -2 + j, at(0, x, 18 + j))
+            for j in range(2):
+                for x in range(20):
+                    put(0, x, 20 + j, at(1, x, j))
+        elif g == 2:  # block 1 left of block 0, upside down
+            for i in range(2):
+                for y in range(20):
+                    put(1, 20 + i, 19 - y, at(0, i, y))
+            for i in range(2):
+                for y in range(20):
+                    put(0, -2 + i, y, at(1, 18 + i, 19 - y))
+        elif g == 3:  # block 1 below block 0, mirrored in x
+            for j in range(2):
+                for x in range(20):
+                    put(1, 19 - x, 20 + j, at(0, x, j))
+            for j in range(2):
+                for x in range(20):
+                    put(0, x, -2 + j, at(1, 19 - x, 18 + j))
+        elif g == 4:  # block 0's right face is block 1's bottom face, axes swapped
+            for i in range(2):
+                for y in range(20):
+                    put(1, y, -2 + i, at(0, 18 + i, y))
+            for i in range(2):
+                for y in range(20):
+                    put(0, 20 + i, y, at(1, y, i))
+    return d
+
+
+@pytest.mark.parametrize("mask", [1, 2, 4, 8, 16, 31])
+def test_oracle_mblock_halo_groups(mask):
+    a0, a1 = mblock("cpu", mask)
+    e0, e1 = mblock_expected(mask)
+    assert np.array_equal(a0, e0) and np.array_equal(a1, e1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mask", [1, 2, 4, 8, 16, 31])
+def test_gpu_mblock_halo_groups(mask):
+    a0, a1 = mblock("gpu", mask)
+    e0, e1 = mblock_expected(mask)
+    assert np.array_equal(a0, e0) and np.array_equal(a1, e1)
+
+
+# ---------------------------------------------------------------- 3-D orientations
+
+CASES3D = [
+    # sizes and halos of A and B, types, iter, from_base, to_base, from_dir, to_dir
+    dict(sa=(6, 5, 4), sb=(6, 5, 4), ta=0, tb=0, it=(2, 5, 4), fb=(4, 0, 0), tbse=(-2, 0, 0),
+         fd=(1, 2, 3), td=(1, 2, 3)),
+    dict(sa=(6, 5, 4), sb=(5, 4, 6), ta=0, tb=0, it=(5, 4, 2), fb=(0, 0, 2), tbse=(0, 0, -2),
+         fd=(1, 2, 3), td=(2, 3, 1)),
+    dict(sa=(7, 3, 5), sb=(5, 7, 3), ta=0, tb=1, it=(7, 3, 2), fb=(0, 0, 3), tbse=(0, 0, 0),
+         fd=(1, -2, 3), td=(3, 1, -2)),
+    dict(sa=(4, 4, 4), sb=(4, 4, 4), ta=1, tb=0, it=(4, 1, 4), fb=(0, 3, 0), tbse=(0, 4, 0),
+         fd=(-1, 2, -3), td=(1, 2, 3)),
+]
+
+
+def halo3d(kind, c):
+    L = lib("apitest", kind)
+    dm, dp = (-2, -2, -2), (2, 2, 2)
+    na = [s + 4 for s in c["sa"]]
+    nb = [s + 4 for s in c["sb"]]
+    i = np.arange(na[0] * na[1] * na[2])
+    inA = (1.0 + (i % na[0]) + 100.0 * ((i // na[0]) % na[1]) + 10000.0 * (i // (na[0] * na[1])))
+    inA = inA.astype(np.float64)
+    out = np.zeros(nb[0] * nb[1] * nb[2])
+    ia = lambda v: np.array(v, dtype=np.int32)
+    argc, av = argv()
+    rc = L.api_halo3d(argc, av, p(ia(c["sa"])), p(ia(dm)), p(ia(dp)), c["ta"], p(ia(c["sb"])),
+                      p(ia(dm)), p(ia(dp)), c["tb"], p(ia(c["it"])), p(ia(c["fb"])),
+                      p(ia(c["tbse"])), p(ia(c["fd"])), p(ia(c["td"])), p(inA), p(out))
+    assert rc == 0
+    return inA.reshape(na[::-1]), out.reshape(nb[::-1])
+
+
+@pytest.mark.parametrize("case", range(len(CASES3D)))
+def test_oracle_halo3d_moves_every_value_once(case):
+    # each value of the source window lands exactly once; the rest of B stays zero
+    c = CASES3D[case]
+    A, B = halo3d("cpu", c)
+    n = int(np.prod(c["it"]))
+    assert np.count_nonzero(B) == n
+    got = np.sort(B[B != 0])
+    # the source window in A, in the from dataset's own coordinates
+    ranges = []
+    for d in range(3):
+        ax = abs(c["fd"][d]) - 1
+        lo = c["fb"][d]
+        ranges.append(range(lo + 2, lo + 2 + c["it"][ax]))
+    src = A[np.ix_(ranges[2], ranges[1], ranges[0])]
+    assert np.array_equal(got, np.sort(src.ravel()))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(CASES3D)))
+def test_gpu_halo3d_equals_oracle(case):
+    c = CASES3D[case]
+    _, Bc = halo3d("cpu", c)
+    _, Bg = halo3d("gpu", c)
+    assert np.array_equal(Bc, Bg)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_halo_outside_allocation_is_rejected(kind):
+    # the z window of B would be [3, 6) in a dat whose allocation ends at z = 4
+    c = dict(CASES3D[2])
+    c["tbse"] = (0, 0, 3)
+    with pytest.raises(AssertionError):
+        halo3d(kind, c)
+
+
+def test_oracle_halo3d_identity_orientation_is_a_box_copy():
+    c = CASES3D[0]
+    A, B = halo3d("cpu", c)
+    # A x in [4, 6) -> B x in [-2, 0), all y, z (+2 for the halo offset in the arrays)
+    assert np.array_equal(B[2:6, 2:7, 0:2], A[2:6, 2:7, 6:8])
+
+
+# ---------------------------------------------------------------- tridiagonal batches
+
+def tri_inputs(size, seed):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(size))
+    a = rng.uniform(-1, 0, n)
+    c = rng.uniform(-1, 0, n)
+    b = 2.5 + rng.uniform(0, 1, n)  # diagonally dominant
+    d = rng.uniform(-1, 1, n)
+    u = rng.uniform(-1, 1, n)
+    return a, b, c, d, u
+
+
+def tri_run(kind, ndim, sd, size, rng_box, inc, seed=1):
+    L = lib("apitest", kind)
+    a, b, c, d, u = tri_inputs(size, seed)
+    out = np.zeros(int(np.prod(size)))
+    argc, av = argv()
+    sz = np.array(list(size) + [1] * (3 - ndim), dtype=np.int32)
+    rg = np.array(rng_box, dtype=np.int32)
+    assert L.api_tridiag(argc, av, ndim, sd, p(sz), p(rg), p(a), p(b), p(c), p(d), p(u), inc, p(out)) == 0
+    return (a, b, c, d, u), out
+
+
+def tri_dense(inputs, ndim, sd, size, box, inc):
+    a, b, c, d, u = [x.reshape(tuple(size[::-1])) for x in inputs]
+    res = (u if inc else d).copy()
+    sl = tuple(slice(box[2 * k], box[2 * k + 1]) for k in range(ndim))[::-1]
+    ax = ndim - 1 - sd  # numpy axis of the solve dimension
+    A, B, C, D = (np.moveaxis(x[sl], ax, -1) for x in (a, b, c, d))
+    R = np.moveaxis(res[sl], ax, -1)
+    m = A.shape[-1]
+    for idx in np.ndindex(A.shape[:-1]):
+        M = np.diag(B[idx]) + np.diag(A[idx][1:], -1) + np.diag(C[idx][:-1], 1)
+        x = np.linalg.solve(M, D[idx])
+        R[idx] = R[idx] + x if inc else x
+    return res.ravel()
+
+
+TRI = [
+    (1, 0, (37,), (3, 30), 0),
+    (2, 0, (17, 9), (0, 17, 2, 9), 0),
+    (2, 1, (17, 9), (1, 16, 0, 9), 1),
+    (3, 0, (9, 7, 5), (0, 9, 1, 6, 0, 5), 0),
+    (3, 1, (9, 7, 5), (2, 9, 0, 7, 1, 5), 0),
+    (3, 2, (9, 7, 5), (0, 9, 0, 7, 0, 5), 1),
+    (3, 0, (70, 3, 2), (0, 70, 0, 3, 0, 2), 0),   # more than one 64-line tile row of x-lines
+]
+
+
+@pytest.mark.parametrize("case", TRI)
+def test_oracle_tridiag_matches_dense_solve(case):
+    ndim, sd, size, box, inc = case
+    inputs, out = tri_run("cpu", ndim, sd, size, box, inc)
+    ref = tri_dense(inputs, ndim, sd, size, box, inc)
+    assert np.max(np.abs(out - ref)) < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", TRI)
+def test_gpu_tridiag_equals_oracle(case):
+    ndim, sd, size, box, inc = case
+    _, oc = tri_run("cpu", ndim, sd, size, box, inc)
+    _, og = tri_run("gpu", ndim, sd, size, box, inc)
+    assert np.array_equal(oc, og)
+
+
+# ---------------------------------------------------------------- reductions
+
+def reduce_run(kind, nx, ny, seed=3):
+    L = lib("apitest", kind)
+    rng = np.random.default_rng(seed)
+    v = rng.uniform(-5, 5, nx * ny)
+    w = rng.integers(-1000, 1000, nx * ny).astype(np.int32)
+    out = np.zeros(8)
+    argc, av = argv()
+    assert L.api_reduce(argc, av, nx, ny, p(v), p(w), p(out)) == 0
+    return v, w, out
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("shape", [(2, 3), (7, 5), (300, 201)])
+def test_reductions(kind, shape):
+    nx, ny = shape
+    v, w, out = reduce_run(kind, nx, ny)
+    assert abs(out[0] - math.fsum(v)) <= 1e-12 * max(1.0, math.fsum(abs(v)))
+    assert out[1] == v.min() and out[2] == v.max()
+    assert out[3] == int(w.astype(np.int64).sum()) and out[4] == w.min() and out[5] == w.max()
+    assert abs(out[6] - math.fsum(v * v)) <= 1e-12 * math.fsum(v * v)
+    assert out[7] == nx * ny
+
+
+# ---------------------------------------------------------------- random numbers
+
+def mt_uniform(seed, n):
+    """std::mt19937(seed) + std::uniform_real_distribution<double>(0, 1) as libstdc++'s
+    generate_canonical forms it: two 32-bit draws, (g1 + g2 * 2^32) / 2^64, rounded in
+    double at the addition."""
+    bg = np.random.RandomState(seed)._bit_generator
+    raw = bg.random_raw(2 * n).astype(np.float64)
+    s = raw[0::2] + raw[1::2] * 4294967296.0
+    r = s / 18446744073709551616.0
+    return np.where(r >= 1.0, np.nextafter(1.0, 0.0), r)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_random_uniform_is_mt19937_canonical(kind):
+    L = lib("apitest", kind)
+    nx, ny = 13, 7
+    out = np.zeros((nx + 2) * (ny + 2))
+    argc, av = argv()
+    assert L.api_random(argc, av, 1234, nx, ny, p(out)) == 0
+    # the whole allocation, halos included, in memory order (ops_lib_core.cpp:2404-2416)
+    assert np.array_equal(out, mt_uniform(1234, out.size))
