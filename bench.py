@@ -233,6 +233,36 @@ def tealeaf_gpu(args):
     }
 
 
+def adi_gpu(args):
+    """Batched ADI: 48 blocks with extents in {64..256} (splitmix64 seed 5), u ~ U[0,1),
+    lambda 1, 100 steps of preproc + x, y, z tridiagonal solves, one step replayed as a
+    graph. Algorithmic bytes per cell-step: preproc 8 + 80, x and y solves 40 each, z solve
+    with increment 48 (216 B)."""
+    import numpy as np
+    import ops_amd
+    from ops_amd.apps.adi.config import adi_sizes
+    L = ops_amd.app("adi")
+    R = ops_amd.runtime()
+    sizes = adi_sizes()
+    cells = sum(a * b * c for a, b, c in sizes)
+    sz = np.array([v for s in sizes for v in s], dtype=np.int32)
+    argc, argv = ops_amd.argv()
+    assert L.adi_init(argc, argv, len(sizes), sz.ctypes.data_as(ctypes.c_void_p), 1, 5, 1.0) == 0
+    assert L.adi_step(2) == 0  # eager step + graph capture + one replay
+    R.ops_hip_device_sync()
+    t0 = time.perf_counter()
+    assert L.adi_step(args.adi_steps) == 0
+    R.ops_hip_device_sync()
+    t1 = time.perf_counter()
+    assert L.adi_finalize() == 0
+    secs = t1 - t0
+    gbs = cells * 216.0 * args.adi_steps / secs / 1e9
+    return {"workload": f"adi_3d 48 blocks, {cells} cells, {args.adi_steps} steps",
+            "s": round(secs, 4), "ms_per_step": round(secs * 1e3 / args.adi_steps, 3),
+            "cell_steps_per_s": cells * args.adi_steps / secs,
+            "algorithmic_GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def clover_cpu(args, threads):
     """Oracle host backend, same application and deck, bounded sample of steps."""
     lib = ctypes.CDLL(str(ROOT / "oracle" / "_build" / "libcloverleaf_cpu.so"),
@@ -271,6 +301,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-tealeaf", action="store_true")
+    ap.add_argument("--no-adi", action="store_true")
+    ap.add_argument("--adi-steps", type=int, default=100)
     ap.add_argument("--tea-cells", type=int, default=4096)
     ap.add_argument("--tea-steps", type=int, default=10)
     ap.add_argument("--ingest-pmc", metavar="DIR", default=None,
@@ -315,6 +347,8 @@ def main():
     }
     if not args.no_tealeaf:
         out["tealeaf"] = tealeaf_gpu(args)
+    if not args.no_adi:
+        out["adi"] = adi_gpu(args)
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] cpu baseline: {threads} threads, {args.cpu_steps} steps")

@@ -31,7 +31,8 @@ typedef struct ops_hip_loop {
   int start[3];       /* first loop index per dimension */
   int count[3];       /* iterations per dimension (1 for unused dimensions) */
   unsigned grid[3];   /* workgroups per dimension */
-  unsigned block[3];  /* threads per workgroup (block[0]*block[1]*block[2] == 256) */
+  unsigned block[3];  /* threads per workgroup (whole 64-lane wavefronts, at most 256) */
+  int vx;             /* loop points per lane along x (set by the stub before begin) */
   unsigned nblocks;   /* grid[0]*grid[1]*grid[2] */
   void *stream;       /* hipStream_t every launch goes to */
   /* dat args: device address of the loop's first point, and element steps per loop
@@ -88,6 +89,14 @@ int ops_hip_kernel_count(void);
 int ops_hip_kernel_stats(int kernel_id, char *name_out, int name_cap, long *calls,
                          double *total_ms, double *bytes);
 void ops_hip_reset_stats(void);
+/* Graph capture of a fixed launch sequence (replayed without host launch cost). Between
+   begin and end only launches may be issued on the runtime stream: no host
+   synchronisation, no allocation, no ops_reduction_result, no host<->device copies.
+   begin returns 0 on success; end returns an executable graph (NULL on failure). */
+int ops_hip_graph_begin(void);
+void *ops_hip_graph_end(void);
+int ops_hip_graph_replay(void *graph, int times);
+void ops_hip_graph_free(void *graph);
 /* Measured HBM bandwidth: a plain copy of `bytes` between two fresh device buffers,
    `reps` times on the runtime stream; returns (read + write bytes) / time in GB/s, or a
    negative value on error. Allocates and frees its buffers outside the timed region. */

@@ -2,8 +2,8 @@
 // ops_tridMultiDimBatch). One block reproduces the reference app (apps/c/adi/adi.cpp:
 // 331-421): per step a 7-point preproc loop writes the right-hand side du and the
 // coefficients of the x, y and z systems, then x and y solves overwrite du and the z solve
-// adds its result into u (ops_tridMultiDimBatch_Inc). Several blocks of different sizes
-// are stepped one after the other, each on its own ops_block.
+// adds its result into u (ops_tridMultiDimBatch_Inc). Each block has its own ops_block;
+// a step issues each phase for all blocks before the next phase.
 //
 // Initial u: mode 0 is the reference's init_kernel (1 on the boundary layer, 0 inside);
 // mode 1 fills every block, x fastest and blocks in order, with U[0,1) numbers from
@@ -18,6 +18,7 @@
 
 #define OPS_3D
 #include <ops_seq_v2.h>
+#include <ops_hip_rt.h>
 
 #include "adi_kernels.h"
 
@@ -34,6 +35,7 @@ struct State {
   ops_stencil S000 = nullptr, S7 = nullptr;
   double lambda = 1.0;
   ops_tridsolver_params *ctx = nullptr;
+  void *graph = nullptr;  // one whole step over all blocks, captured once and replayed
 } A;
 
 uint64_t splitmix_next(uint64_t &s) {
@@ -44,7 +46,7 @@ uint64_t splitmix_next(uint64_t &s) {
   return z ^ (z >> 31);
 }
 
-void step_block(Blk &b) {
+void preproc(Blk &b) {
   int range[] = {0, b.n[0], 0, b.n[1], 0, b.n[2]};
   ops_par_loop(adi_preproc_kernel, "preproc_kernel", b.blk, 3, range,
                ops_arg_dat(b.u, 1, A.S7, "double", OPS_READ),
@@ -59,9 +61,25 @@ void step_block(Blk &b) {
                ops_arg_dat(b.bz, 1, A.S000, "double", OPS_WRITE),
                ops_arg_dat(b.cz, 1, A.S000, "double", OPS_WRITE), ops_arg_idx(),
                ops_arg_gbl(b.n, 3, "int", OPS_READ), ops_arg_gbl(&A.lambda, 1, "double", OPS_READ));
-  ops_tridMultiDimBatch(3, 0, range, b.ax, b.bx, b.cx, b.du, A.ctx);
-  ops_tridMultiDimBatch(3, 1, range, b.ay, b.by, b.cy, b.du, A.ctx);
-  ops_tridMultiDimBatch_Inc(3, 2, range, b.az, b.bz, b.cz, b.du, b.u, A.ctx);
+}
+
+// one step of every block: the blocks are independent, so each phase is issued for all
+// blocks before the next phase, which lets the runtime solve the lines of all blocks of a
+// sweep together
+void step_all() {
+  for (Blk &b : A.blocks) preproc(b);
+  for (Blk &b : A.blocks) {
+    int range[] = {0, b.n[0], 0, b.n[1], 0, b.n[2]};
+    ops_tridMultiDimBatch(3, 0, range, b.ax, b.bx, b.cx, b.du, A.ctx);
+  }
+  for (Blk &b : A.blocks) {
+    int range[] = {0, b.n[0], 0, b.n[1], 0, b.n[2]};
+    ops_tridMultiDimBatch(3, 1, range, b.ay, b.by, b.cy, b.du, A.ctx);
+  }
+  for (Blk &b : A.blocks) {
+    int range[] = {0, b.n[0], 0, b.n[1], 0, b.n[2]};
+    ops_tridMultiDimBatch_Inc(3, 2, range, b.az, b.bz, b.cz, b.du, b.u, A.ctx);
+  }
 }
 
 }  // namespace
@@ -106,6 +124,7 @@ extern "C" int adi_init(int argc, const char **argv, int nblocks, const int *siz
       A.blocks.push_back(b);
     }
     A.ctx = new ops_tridsolver_params(A.blocks.empty() ? nullptr : A.blocks[0].blk);
+    A.graph = nullptr;
     return 0;
   } catch (std::exception &e) {
     fprintf(stderr, "adi_init: %s\n", e.what());
@@ -113,10 +132,34 @@ extern "C" int adi_init(int argc, const char **argv, int nblocks, const int *siz
   }
 }
 
+// Steps every block nsteps times. With OPS_ADI_GRAPH unset or non-zero, the first call
+// runs one step eagerly (sizing every scratch buffer), then records one step of all blocks
+// as a graph that later steps replay; a backend without graph
+// capture falls back to eager launches.
 extern "C" int adi_step(int nsteps) {
   try {
-    for (int s = 0; s < nsteps; s++)
-      for (Blk &b : A.blocks) step_block(b);
+    const char *env = getenv("OPS_ADI_GRAPH");
+    const bool use_graph = env == nullptr || atoi(env) != 0;
+    int s = 0;
+    if (use_graph && A.graph == nullptr && nsteps > 1) {
+      step_all();
+      s = 1;
+      if (ops_hip_graph_begin() == 0) {
+        try {
+          step_all();
+        } catch (...) {
+          ops_hip_graph_free(ops_hip_graph_end());
+          throw;
+        }
+        A.graph = ops_hip_graph_end();
+      }
+    }
+    if (use_graph && A.graph != nullptr) {
+      if (ops_hip_graph_replay(A.graph, nsteps - s) != 0)
+        throw OPSException(OPS_RUNTIME_ERROR, "adi_step: graph replay failed");
+      return 0;
+    }
+    for (; s < nsteps; s++) step_all();
     return 0;
   } catch (std::exception &e) {
     fprintf(stderr, "adi_step: %s\n", e.what());
@@ -158,6 +201,8 @@ extern "C" int adi_finalize(void) {
   try {
     delete A.ctx;
     A.ctx = nullptr;
+    ops_hip_graph_free(A.graph);
+    A.graph = nullptr;
     A.blocks.clear();
     ops_exit();
     return 0;

@@ -100,6 +100,7 @@ void ops_dev_init(OPS_instance *inst) {
 void ops_dev_exit(OPS_instance *inst) {
   (void)inst;
   if (!g_dev.live) return;
+  ops_trid_flush();
   ops_dev_timing_flush();
   HIPCHECK(hipStreamSynchronize(g_dev.stream));
   for (hipEvent_t ev : g_dev.pool) (void)hipEventDestroy(ev);
@@ -120,22 +121,27 @@ void *ops_dev_malloc(size_t bytes) {
   return p;
 }
 void ops_dev_free(void *p) {
+  ops_trid_flush();
   if (p) {
     (void)hipStreamSynchronize(g_dev.stream);
     (void)hipFree(p);
   }
 }
 void ops_dev_memset(void *p, int v, size_t bytes) {
+  ops_trid_flush();
   HIPCHECK(hipMemsetAsync(p, v, bytes, g_dev.stream));
 }
 void ops_dev_sync() {
+  ops_trid_flush();
   if (g_dev.live) HIPCHECK(hipStreamSynchronize(g_dev.stream));
 }
 void ops_dev_h2d(void *dst, const void *src, size_t bytes) {
+  ops_trid_flush();
   HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_dev.stream));
   HIPCHECK(hipStreamSynchronize(g_dev.stream));
 }
 void ops_dev_d2h(void *dst, const void *src, size_t bytes) {
+  ops_trid_flush();
   if (bytes <= (1u << 16)) {
     HIPCHECK(hipMemcpyAsync(g_dev.pinned_small, src, bytes, hipMemcpyDeviceToHost, g_dev.stream));
     HIPCHECK(hipStreamSynchronize(g_dev.stream));
@@ -146,9 +152,11 @@ void ops_dev_d2h(void *dst, const void *src, size_t bytes) {
   }
 }
 void ops_dev_d2d(void *dst, const void *src, size_t bytes) {
+  ops_trid_flush();
   HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_dev.stream));
 }
 void ops_dev_stage_h2d(void *dst, const void *src, size_t bytes) {
+  ops_trid_flush();
   if (bytes > g_dev.stage_cap) {
     ops_dev_h2d(dst, src, bytes);
     return;
@@ -170,7 +178,11 @@ void *ops_dev_symbol_address(const void *symbol) {
   return p;
 }
 
-extern "C" void *ops_hip_get_stream(void) { return (void *)g_dev.stream; }
+extern "C" void *ops_hip_get_stream(void) {
+  ops_trid_flush();
+  return (void *)g_dev.stream;
+}
+void *ops_hip_get_stream_raw() { return (void *)g_dev.stream; }
 extern "C" void ops_hip_device_sync(void) { ops_dev_sync(); }
 
 // ------------------------------------------------------------------ timing
@@ -260,6 +272,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
                                   ops_block block, int dim, const int *range, ops_arg *args,
                                   int nargs) {
   OPS_instance *in = ops_instance_get();
+  ops_trid_flush();
   if (nargs > OPS_HIP_MAX_ARGS) ops_throw(OPS_NOT_IMPLEMENTED, "too many loop arguments");
   lp->kernel_id = kernel_id;
   lp->ndim = dim;
@@ -326,6 +339,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   // workgroup shape: tuned, else 64 lanes along x (coalesced, one wavefront per row
   // segment) and the remaining threads stacked along y/z
   auto &kt = ops_kernel_table();
+  const int vx = lp->vx >= 1 ? lp->vx : 1;
   unsigned bx, by, bz;
   if (kernel_id >= 0 && kernel_id < (int)kt.size() && kt[kernel_id].bx > 0) {
     bx = kt[kernel_id].bx;
@@ -335,7 +349,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
     // always a full 256-lane workgroup (four whole wavefronts): the wavefront reductions
     // exchange values across all 64 lanes, so a partial wavefront would read lanes that
     // do not exist. Lanes outside the range are masked by the kernel's bounds test.
-    bx = pow2_at_least(lp->count[0]);
+    bx = pow2_at_least((lp->count[0] + vx - 1) / vx);
     if (bx > 64) bx = 64;
     if (in->block_size_x > 0) bx = in->block_size_x;
     if (dim >= 3) {
@@ -357,7 +371,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   lp->block[0] = bx;
   lp->block[1] = by;
   lp->block[2] = bz;
-  lp->grid[0] = (lp->count[0] + bx - 1) / bx;
+  lp->grid[0] = (lp->count[0] + bx * vx - 1) / (bx * vx);
   lp->grid[1] = (lp->count[1] + by - 1) / by;
   lp->grid[2] = (lp->count[2] + bz - 1) / bz;
   lp->nblocks = lp->grid[0] * lp->grid[1] * lp->grid[2];
@@ -565,6 +579,7 @@ static void halo_side(ops_dat dat, const int *hbase, const int *dir, const int *
 }
 
 void ops_dev_halo_copy(ops_halo h) {
+  ops_trid_flush();
   ops_dat f = h->from, t = h->to;
   AffineCopy c;
   c.ndim = OPS_MAX_DIM;
@@ -604,6 +619,7 @@ void ops_dev_halo_copy(ops_halo h) {
 // Slab copy between a dat and a dense, packed buffer (no halo): range is given in
 // interior coordinates [begin, end) per dimension (ops_util.cpp:353-500).
 void ops_dev_slab_copy(ops_dat dat, char *buf, int buf_on_device, const int *range, int dir) {
+  ops_trid_flush();
   OPS_instance *in = ops_instance_get();
   int nd = dat->block->dims;
   AffineCopy c;
@@ -675,6 +691,7 @@ __global__ __launch_bounds__(256) void copy_probe_kernel(const ops_v2d *__restri
 
 extern "C" double ops_hip_copy_probe(size_t bytes, int reps) {
   try {
+    ops_trid_flush();
     if (!g_dev.live || reps < 1 || bytes < 16) return -1.0;
     const size_t n = bytes / sizeof(ops_v2d);
     if ((n + 255) / 256 > 0x7fffffffu) return -1.0;
@@ -700,4 +717,33 @@ extern "C" double ops_hip_copy_probe(size_t bytes, int reps) {
   } catch (std::exception &) {
     return -1.0;
   }
+}
+
+// ------------------------------------------------------------------ graph capture
+
+extern "C" int ops_hip_graph_begin(void) {
+  if (!g_dev.live) return 1;
+  ops_trid_flush();  // work deferred before the capture runs now, not inside the graph
+  if (g_timing) ops_throw(OPS_RUNTIME_ERROR, "ops_hip_graph_begin: per-kernel timing is on");
+  return hipStreamBeginCapture(g_dev.stream, hipStreamCaptureModeThreadLocal) == hipSuccess ? 0 : 1;
+}
+
+extern "C" void *ops_hip_graph_end(void) {
+  ops_trid_flush();
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture(g_dev.stream, &g) != hipSuccess || g == nullptr) return nullptr;
+  hipGraphExec_t e = nullptr;
+  const hipError_t st = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  return st == hipSuccess ? (void *)e : nullptr;
+}
+
+extern "C" int ops_hip_graph_replay(void *graph, int times) {
+  for (int i = 0; i < times; i++)
+    if (hipGraphLaunch((hipGraphExec_t)graph, g_dev.stream) != hipSuccess) return 1;
+  return 0;
+}
+
+extern "C" void ops_hip_graph_free(void *graph) {
+  if (graph) (void)hipGraphExecDestroy((hipGraphExec_t)graph);
 }

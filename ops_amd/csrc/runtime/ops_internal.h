@@ -65,6 +65,11 @@ void ops_dev_d2h(void *dst, const void *src, size_t bytes);   // synchronous (dr
 void ops_dev_d2d(void *dst, const void *src, size_t bytes);   // stream-ordered
 void ops_dev_stage_h2d(void *dst, const void *src, size_t bytes);  // stream-ordered, src reusable
 void ops_dev_sync();
+// the runtime stream itself (no flush of deferred work)
+void *ops_hip_get_stream_raw();
+// launches the deferred tridiagonal batch, if any (every launch, copy and synchronisation
+// of the device layer calls it first)
+void ops_trid_flush();
 void ops_dev_halo_copy(ops_halo h);
 // packed slab copy between a dat (device) and a dense buffer; dir 0: dat->buf, 1: buf->dat
 void ops_dev_slab_copy(ops_dat dat, char *buf, int buf_on_device, const int *range, int dir);

@@ -8,15 +8,24 @@
 //   m = 1/(b_i - a_i c'_{i-1}), c'_i = c_i m, d'_i = (d_i - a_i d'_{i-1}) m,
 //   x_{n-1} = d'_{n-1}, x_i = d'_i - c'_i x_{i+1}.
 //
+// Calls are deferred: consecutive calls with the same solve dimension, the same kind
+// (overwrite or increment) and type, and no dat written by one and touched by another,
+// are solved together by one launch per 16 systems. A batch of uneven blocks (the ADI
+// sweeps of many blocks) then fills the GPU, where one block's lines alone would occupy a
+// fraction of it. Every other runtime operation that launches work, copies or
+// synchronises flushes the pending batch first (ops_trid_flush).
+//
 // Layout on MI355X: one lane per line. For y- and z-lines consecutive lanes own
-// consecutive x positions, so every row access of a wavefront is one contiguous 512-byte
-// segment. x-lines are contiguous per line, so a one-wavefront workgroup stages a 64-line x 16-row
-// tile of a/b/c/d through LDS with coalesced loads before each lane sweeps its line.
-// The forward-sweep temporaries c', d' live in a scratch array laid out row-major over
-// lines (row i of all lines contiguous), so they are written and re-read coalesced.
+// consecutive x positions, so every row access of a wavefront is one contiguous segment.
+// x-lines are contiguous per line, so a one-wavefront workgroup stages a 64-line x 16-row
+// tile of a/b/c/d through LDS with coalesced loads before each lane sweeps its line
+// (systems are padded to whole 64-line workgroups). The forward-sweep temporaries c', d'
+// live in a scratch array laid out row-major over lines (row i of all lines contiguous),
+// so they are written and re-read coalesced.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <vector>
 
 #include "ops_internal.h"
 #include "ops_tridiag.h"
@@ -25,102 +34,123 @@
 
 namespace {
 
-struct TridGeom {
-  long long s[3];   // element strides of the padded arrays per dimension
-  int n[3];         // extents of the solve box
-  int solvedim;
+constexpr int MAXSYS = 16;       // systems per launch (kernel-argument size)
+constexpr int XL = 64, XR = 16;  // x-lines: lines per workgroup, rows per LDS tile
+
+template <typename T> struct Sys {
+  T *a, *b, *c, *d, *u;  // box origins
+  long long s1, s2;      // element strides of dimensions 1 and 2 (dimension 0: 1)
+  int n[3];              // box extents
+  long long line0;       // first global line (x-lines: padded to whole workgroups)
   long long nlines;
+  long long scr;         // scratch offset (elements) of this system's c' (d' follows)
 };
 
-// origin of line `l` (elements, relative to the box origin) and the stride along it
-__device__ __forceinline__ long long line_origin(const TridGeom &g, long long l) {
-  if (g.solvedim == 0) {
-    long long p = l % g.n[1], q = l / g.n[1];
-    return p * g.s[1] + q * g.s[2];
-  } else if (g.solvedim == 1) {
-    long long p = l % g.n[0], q = l / g.n[0];
-    return p + q * g.s[2];
-  } else {
-    long long p = l % g.n[0], q = l / g.n[0];
-    return p + q * g.s[1];
+template <typename T> struct Batch {
+  Sys<T> sys[MAXSYS];
+  int nsys;
+  int solvedim;
+};
+
+// line l of system S: its origin (elements from the box origin) and stride
+template <typename T>
+__device__ __forceinline__ long long line_origin(const Sys<T> &S, int sd, long long l) {
+  if (sd == 0) {
+    const long long p = l % S.n[1], q = l / S.n[1];
+    return p * S.s1 + q * S.s2;
+  } else if (sd == 1) {
+    const long long p = l % S.n[0], q = l / S.n[0];
+    return p + q * S.s2;
   }
+  const long long p = l % S.n[0], q = l / S.n[0];
+  return p + q * S.s1;
 }
 
+template <typename T> __device__ __forceinline__ int find_sys(const Batch<T> &B, long long g) {
+  int k = 0;
+  while (k + 1 < B.nsys && g >= B.sys[k + 1].line0) k++;
+  return k;
+}
+
+// y- and z-lines (and x-lines of tiny systems are not special-cased): one lane per line
 template <typename T, bool INC>
-__global__ __launch_bounds__(256) void thomas_strided(const T *__restrict__ a,
-                                                      const T *__restrict__ b,
-                                                      const T *__restrict__ c, T *__restrict__ d,
-                                                      T *__restrict__ u, T *__restrict__ cp,
-                                                      T *__restrict__ dp, TridGeom g) {
-  long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= g.nlines) return;
-  const long long o = line_origin(g, l);
-  const long long st = g.s[g.solvedim];
-  const int n = g.n[g.solvedim];
-  const long long L = g.nlines;
+__global__ __launch_bounds__(256) void thomas_strided(const Batch<T> B, T *__restrict__ scratch) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = find_sys(B, g);
+  const Sys<T> &S = B.sys[k];
+  const long long l = g - S.line0;
+  if (l < 0 || l >= S.nlines) return;
+  const int sd = B.solvedim;
+  const long long o = line_origin(S, sd, l);
+  const long long st = sd == 0 ? 1 : (sd == 1 ? S.s1 : S.s2);
+  const int n = S.n[sd];
+  const long long L = S.nlines;
+  T *cp = scratch + S.scr, *dp = cp + (long long)n * L;
+  const T *a = S.a, *b = S.b, *c = S.c;
+  T *d = S.d, *u = S.u;
   T cprev = c[o] / b[o];
   T dprev = d[o] / b[o];
   cp[l] = cprev;
   dp[l] = dprev;
   for (int i = 1; i < n; i++) {
-    long long k = o + i * st;
-    T m = T(1) / (b[k] - a[k] * cprev);
-    cprev = c[k] * m;
-    dprev = (d[k] - a[k] * dprev) * m;
+    const long long kk = o + i * st;
+    const T m = T(1) / (b[kk] - a[kk] * cprev);
+    cprev = c[kk] * m;
+    dprev = (d[kk] - a[kk] * dprev) * m;
     cp[i * L + l] = cprev;
     dp[i * L + l] = dprev;
   }
   T x = dprev;
-  long long k = o + (long long)(n - 1) * st;
-  if (INC) u[k] += x; else d[k] = x;
+  long long kk = o + (long long)(n - 1) * st;
+  if (INC) u[kk] += x; else d[kk] = x;
   for (int i = n - 2; i >= 0; i--) {
     x = dp[i * L + l] - cp[i * L + l] * x;
-    k = o + i * st;
-    if (INC) u[k] += x; else d[k] = x;
+    kk = o + i * st;
+    if (INC) u[kk] += x; else d[kk] = x;
   }
 }
 
-// x-lines: 64 lines per workgroup, rows staged through LDS in chunks of XR.
-constexpr int XL = 64, XR = 16;  // one wavefront per workgroup, 34.8 KB LDS (double)
-
+// x-lines: 64 lines of one system per workgroup, rows staged through LDS in chunks of XR
 template <typename T, bool INC>
-__global__ __launch_bounds__(64) void thomas_x(const T *__restrict__ a,
-                                                const T *__restrict__ b,
-                                                const T *__restrict__ c, T *__restrict__ d,
-                                                T *__restrict__ u, T *__restrict__ cp,
-                                                T *__restrict__ dp, TridGeom g) {
+__global__ __launch_bounds__(64) void thomas_x(const Batch<T> B, T *__restrict__ scratch) {
   __shared__ T sa[XL][XR + 1], sb[XL][XR + 1], sc[XL][XR + 1], sd[XL][XR + 1];
-  const int tid = threadIdx.x;  // 64 threads, lane tid sweeps line l0 + tid
-  const long long l0 = (long long)blockIdx.x * XL;
-  const int n = g.n[0];
-  const long long L = g.nlines;
-  const int lane = tid;
+  const int lane = threadIdx.x;
+  const long long g0 = (long long)blockIdx.x * XL;  // first global line of the workgroup
+  const int k = find_sys(B, g0);
+  const Sys<T> &S = B.sys[k];
+  const long long l0 = g0 - S.line0;
+  if (l0 >= S.nlines) return;  // padding workgroup past the last system (uniform)
+  const int n = S.n[0];
+  const long long L = S.nlines;
   const long long myl = l0 + lane;
   const bool own = myl < L;
+  T *cp = scratch + S.scr, *dp = cp + (long long)n * L;
+  const T *a = S.a, *b = S.b, *c = S.c;
+  T *d = S.d, *u = S.u;
   T cprev = T(0), dprev = T(0);
   for (int r0 = 0; r0 < n; r0 += XR) {
     const int rows = (n - r0) < XR ? (n - r0) : XR;
     // cooperative coalesced load: thread -> (line, row) with row fastest
-    for (int e = tid; e < XL * XR; e += XL) {
-      int li = e / XR, ri = e % XR;
-      long long ll = l0 + li;
+    for (int e = lane; e < XL * XR; e += XL) {
+      const int li = e / XR, ri = e % XR;
+      const long long ll = l0 + li;
       if (ll < L && ri < rows) {
-        long long k = line_origin(g, ll) + r0 + ri;
-        sa[li][ri] = a[k];
-        sb[li][ri] = b[k];
-        sc[li][ri] = c[k];
-        sd[li][ri] = d[k];
+        const long long kk = line_origin(S, 0, ll) + r0 + ri;
+        sa[li][ri] = a[kk];
+        sb[li][ri] = b[kk];
+        sc[li][ri] = c[kk];
+        sd[li][ri] = d[kk];
       }
     }
     __syncthreads();
     if (own) {
       for (int ri = 0; ri < rows; ri++) {
-        int i = r0 + ri;
+        const int i = r0 + ri;
         if (i == 0) {
           cprev = sc[lane][ri] / sb[lane][ri];
           dprev = sd[lane][ri] / sb[lane][ri];
         } else {
-          T m = T(1) / (sb[lane][ri] - sa[lane][ri] * cprev);
+          const T m = T(1) / (sb[lane][ri] - sa[lane][ri] * cprev);
           cprev = sc[lane][ri] * m;
           dprev = (sd[lane][ri] - sa[lane][ri] * dprev) * m;
         }
@@ -134,24 +164,24 @@ __global__ __launch_bounds__(64) void thomas_x(const T *__restrict__ a,
   // global stores are coalesced as well
   T x = T(0);
   const int nchunks = (n + XR - 1) / XR;
-  for (int cidx = nchunks - 1; cidx >= 0; cidx--) {
-    const int r0 = cidx * XR;
+  for (int ci = nchunks - 1; ci >= 0; ci--) {
+    const int r0 = ci * XR;
     const int rows = (n - r0) < XR ? (n - r0) : XR;
     if (own) {
       for (int ri = rows - 1; ri >= 0; ri--) {
-        int i = r0 + ri;
+        const int i = r0 + ri;
         if (i == n - 1) x = dp[i * L + myl];
         else x = dp[i * L + myl] - cp[i * L + myl] * x;
         sd[lane][ri] = x;
       }
     }
     __syncthreads();
-    for (int e = tid; e < XL * XR; e += XL) {
-      int li = e / XR, ri = e % XR;
-      long long ll = l0 + li;
+    for (int e = lane; e < XL * XR; e += XL) {
+      const int li = e / XR, ri = e % XR;
+      const long long ll = l0 + li;
       if (ll < L && ri < rows) {
-        long long k = line_origin(g, ll) + r0 + ri;
-        if (INC) u[k] += sd[li][ri]; else d[k] = sd[li][ri];
+        const long long kk = line_origin(S, 0, ll) + r0 + ri;
+        if (INC) u[kk] += sd[li][ri]; else d[kk] = sd[li][ri];
       }
     }
     __syncthreads();
@@ -174,41 +204,87 @@ void *scratch(size_t bytes) {
   return g_scratch.p;
 }
 
+// one deferred call
+struct Call {
+  int ndim, solvedim, inc, is_double;
+  int range[6];
+  ops_dat a, b, c, d, u;
+};
+
+struct Pending {
+  std::vector<Call> calls;
+  bool flushing = false;
+};
+Pending g_pending;
+
+template <typename T> T *box_origin(ops_dat x, long long off) {
+  return x == nullptr ? nullptr : (T *)(x->data_d + x->base_offset) + off;
+}
+
+// launches the systems calls[first, first + count) (same solvedim, kind and type)
 template <typename T, bool INC>
-void solve(int ndim, int solvedim, const int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
-           ops_dat u) {
-  TridGeom g;
-  long long off = 0;
-  g.s[0] = 1;
-  g.s[1] = a->size[0];
-  g.s[2] = (long long)a->size[0] * a->size[1];
-  for (int i = 0; i < 3; i++) g.n[i] = 1;
-  for (int i = 0; i < ndim; i++) {
-    g.n[i] = range[2 * i + 1] - range[2 * i];
-    if (g.n[i] <= 0) return;
-    off += (long long)range[2 * i] * g.s[i];
+void launch_chunk(const std::vector<Call> &calls, size_t first, size_t count) {
+  Batch<T> B;
+  B.nsys = 0;
+  B.solvedim = calls[first].solvedim;
+  const int sd = B.solvedim;
+  long long line = 0, scr = 0;
+  for (size_t k = first; k < first + count; k++) {
+    const Call &C = calls[k];
+    Sys<T> S;
+    long long s[3] = {1, C.a->size[0], (long long)C.a->size[0] * C.a->size[1]};
+    long long off = 0;
+    bool empty = false;
+    for (int i = 0; i < 3; i++) S.n[i] = 1;
+    for (int i = 0; i < C.ndim; i++) {
+      S.n[i] = C.range[2 * i + 1] - C.range[2 * i];
+      if (S.n[i] <= 0) empty = true;
+      off += (long long)C.range[2 * i] * s[i];
+    }
+    if (empty) continue;
+    S.a = box_origin<T>(C.a, off);
+    S.b = box_origin<T>(C.b, off);
+    S.c = box_origin<T>(C.c, off);
+    S.d = box_origin<T>(C.d, off);
+    S.u = box_origin<T>(C.u, off);
+    S.s1 = s[1];
+    S.s2 = s[2];
+    const long long total = (long long)S.n[0] * S.n[1] * S.n[2];
+    S.nlines = total / S.n[sd];
+    S.line0 = line;
+    S.scr = scr;
+    line += (sd == 0) ? ((S.nlines + XL - 1) / XL) * XL : S.nlines;
+    scr += 2 * total;
+    B.sys[B.nsys++] = S;
   }
-  g.solvedim = solvedim;
-  long long total = (long long)g.n[0] * g.n[1] * g.n[2];
-  g.nlines = total / g.n[solvedim];
-  // index-0 pointers (raw-pointer semantics: base_offset) plus the box origin
-  auto at = [&](ops_dat x) -> T * {
-    if (x == nullptr) return nullptr;
-    return (T *)(x->data_d + x->base_offset) + off;
-  };
-  T *cp = (T *)scratch(sizeof(T) * 2 * (size_t)total);
-  T *dp = cp + total;
-  hipStream_t s = (hipStream_t)ops_hip_get_stream();
-  if (solvedim == 0) {
-    unsigned blocks = (unsigned)((g.nlines + XL - 1) / XL);
-    hipLaunchKernelGGL((thomas_x<T, INC>), dim3(blocks), dim3(XL), 0, s, at(a), at(b), at(c),
-                       at(d), at(u), cp, dp, g);
-  } else {
-    unsigned blocks = (unsigned)((g.nlines + 255) / 256);
-    hipLaunchKernelGGL((thomas_strided<T, INC>), dim3(blocks), dim3(256), 0, s, at(a), at(b),
-                       at(c), at(d), at(u), cp, dp, g);
-  }
+  if (B.nsys == 0) return;
+  T *scr_p = (T *)scratch(sizeof(T) * (size_t)scr);
+  hipStream_t st = (hipStream_t)ops_hip_get_stream_raw();
+  if (sd == 0)
+    hipLaunchKernelGGL((thomas_x<T, INC>), dim3((unsigned)(line / XL)), dim3(XL), 0, st, B, scr_p);
+  else
+    hipLaunchKernelGGL((thomas_strided<T, INC>), dim3((unsigned)((line + 255) / 256)), dim3(256), 0,
+                       st, B, scr_p);
   ops_hip_check_launch("ops_tridMultiDimBatch");
+}
+
+bool writes(const Call &c, ops_dat x) { return x != nullptr && (c.inc ? c.u == x : c.d == x); }
+bool touches(const Call &c, ops_dat x) {
+  return x != nullptr && (c.a == x || c.b == x || c.c == x || c.d == x || c.u == x);
+}
+
+// a new call joins the pending batch when nothing one writes is touched by the other
+bool compatible(const Call &n) {
+  for (const Call &p : g_pending.calls) {
+    if (p.solvedim != n.solvedim || p.inc != n.inc || p.is_double != n.is_double) return false;
+    ops_dat nd[5] = {n.a, n.b, n.c, n.d, n.u};
+    for (ops_dat x : nd)
+      if (writes(p, x)) return false;
+    ops_dat pd[5] = {p.a, p.b, p.c, p.d, p.u};
+    for (ops_dat x : pd)
+      if (writes(n, x)) return false;
+  }
+  return true;
 }
 
 void check_args(int ndim, int solvedim, ops_dat a, ops_dat b, ops_dat c, ops_dat d, ops_dat u) {
@@ -224,40 +300,72 @@ void check_args(int ndim, int solvedim, ops_dat a, ops_dat b, ops_dat c, ops_dat
       (u && strcmp(u->type, d->type)))
     ops_throw(OPS_RUNTIME_ERROR,
               "Tridsolver error: the a,b,c,d datasets must all be of the same type");
+  if (strcmp(a->type, "double") && strcmp(a->type, "float"))
+    ops_throw(OPS_RUNTIME_ERROR,
+              "Tridsolver error: unsupported type. Dataset type must either be \"double\" or "
+              "\"float\"");
   ops_dat all[5] = {a, b, c, d, u};
   for (ops_dat x : all)
     if (x) ops_put_data(x);
 }
 
+void enqueue(int ndim, int solvedim, const int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
+             ops_dat u, int inc) {
+  check_args(ndim, solvedim, a, b, c, d, u);
+  Call C;
+  C.ndim = ndim;
+  C.solvedim = solvedim;
+  C.inc = inc;
+  C.is_double = !strcmp(a->type, "double");
+  for (int i = 0; i < 6; i++) C.range[i] = i < 2 * ndim ? range[i] : 0;
+  C.a = a;
+  C.b = b;
+  C.c = c;
+  C.d = d;
+  C.u = u;
+  if (!compatible(C)) ops_trid_flush();
+  g_pending.calls.push_back(C);
+  if (inc) u->dirty_hd = 2;
+  else d->dirty_hd = 2;
+}
+
 }  // namespace
+
+void ops_trid_flush() {
+  if (g_pending.calls.empty() || g_pending.flushing) return;
+  g_pending.flushing = true;
+  std::vector<Call> calls;
+  calls.swap(g_pending.calls);
+  try {
+    const Call &c0 = calls[0];
+    for (size_t first = 0; first < calls.size(); first += MAXSYS) {
+      const size_t count = calls.size() - first < (size_t)MAXSYS ? calls.size() - first : MAXSYS;
+      if (c0.is_double) {
+        if (c0.inc) launch_chunk<double, true>(calls, first, count);
+        else launch_chunk<double, false>(calls, first, count);
+      } else {
+        if (c0.inc) launch_chunk<float, true>(calls, first, count);
+        else launch_chunk<float, false>(calls, first, count);
+      }
+    }
+  } catch (...) {
+    g_pending.flushing = false;
+    throw;
+  }
+  g_pending.flushing = false;
+}
+
+void ops_trid_discard() { g_pending.calls.clear(); }
 
 extern "C" void ops_tridMultiDimBatch(int ndim, int solvedim, int *range, ops_dat a, ops_dat b,
                                       ops_dat c, ops_dat d, ops_tridsolver_params *ctx) {
   (void)ctx;
-  check_args(ndim, solvedim, a, b, c, d, nullptr);
-  if (!strcmp(a->type, "double"))
-    solve<double, false>(ndim, solvedim, range, a, b, c, d, nullptr);
-  else if (!strcmp(a->type, "float"))
-    solve<float, false>(ndim, solvedim, range, a, b, c, d, nullptr);
-  else
-    ops_throw(OPS_RUNTIME_ERROR,
-              "Tridsolver error: unsupported type. Dataset type must either be \"double\" or "
-              "\"float\"");
-  d->dirty_hd = 2;
+  enqueue(ndim, solvedim, range, a, b, c, d, nullptr, 0);
 }
 
 extern "C" void ops_tridMultiDimBatch_Inc(int ndim, int solvedim, int *range, ops_dat a,
                                           ops_dat b, ops_dat c, ops_dat d, ops_dat u,
                                           ops_tridsolver_params *ctx) {
   (void)ctx;
-  check_args(ndim, solvedim, a, b, c, d, u);
-  if (!strcmp(a->type, "double"))
-    solve<double, true>(ndim, solvedim, range, a, b, c, d, u);
-  else if (!strcmp(a->type, "float"))
-    solve<float, true>(ndim, solvedim, range, a, b, c, d, u);
-  else
-    ops_throw(OPS_RUNTIME_ERROR,
-              "Tridsolver error: unsupported type. Dataset type must either be \"double\" or "
-              "\"float\"");
-  u->dirty_hd = 2;
+  enqueue(ndim, solvedim, range, a, b, c, d, u, 1);
 }

@@ -148,6 +148,10 @@ def const_dim(a: Arg, loop: Loop) -> str:
                          "compile-time constant name")
 
 
+# loop points per lane along x (blockDim.x apart)
+VX = int(os.environ.get("OPS_GEN_VX", "1"))
+
+
 def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
     k = loop.stub
     L = []
@@ -175,9 +179,15 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
         L.append(f"  for (int d = 0; d < {d}; d++) r{i}[d] = {RED_OP[a.acc]}<{t}>::identity();")
     L.append("  unsigned bx, by, bz;")
     L.append("  ops_block_coords(bx, by, bz);")
-    L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
     L.append("  const int y = (int)(by * blockDim.y + threadIdx.y);")
     L.append("  const int z = (int)(bz * blockDim.z + threadIdx.z);")
+    if VX > 1:
+        # VX points per lane, blockDim.x apart: every pass of the wavefront stays contiguous
+        L.append("#pragma unroll")
+        L.append(f"  for (int v = 0; v < {VX}; v++) {{")
+        L.append(f"  const int x = (int)((bx * {VX} + v) * blockDim.x + threadIdx.x);")
+    else:
+        L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
     L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
     call = []
     for i, a in enumerate(loop.args):
@@ -217,6 +227,8 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
             call.append(f"idx{i}")
     L.append(f"    {loop.kernel}({', '.join(call)});")
     L.append("  }")
+    if VX > 1:
+        L.append("  }")
     for i in red:
         a = loop.args[i]
         t = CTYPES.get(a.typ, a.typ)
@@ -235,6 +247,7 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     L.append(f"  static int kid = ops_hip_register_kernel(\"{loop.kernel}\");")
     L.append(f"  ops_arg args[{max(n, 1)}] = {{{', '.join(f'arg{i}' for i in range(n)) or 'ops_arg_idx()'}}};")
     L.append("  ops_hip_loop lp;")
+    L.append(f"  lp.vx = {VX};")
     L.append(f"  if (ops_hip_loop_begin(&lp, kid, name, block, dim, range, args, {n})) {{")
     L.append(f"    {ns}::ops_kp_{k} p;")
     L.append("    p.cnt0 = lp.count[0]; p.cnt1 = lp.count[1]; p.cnt2 = lp.count[2];")

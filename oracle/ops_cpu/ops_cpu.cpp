@@ -729,3 +729,10 @@ extern "C" void ops_cpu_set_threads(int n) {
   (void)n;
 #endif
 }
+
+// graph capture is a device feature: the host backend reports it unavailable, so callers
+// run their launches eagerly
+extern "C" int ops_hip_graph_begin(void) { return 1; }
+extern "C" void *ops_hip_graph_end(void) { return nullptr; }
+extern "C" int ops_hip_graph_replay(void *, int) { return 1; }
+extern "C" void ops_hip_graph_free(void *) {}

@@ -131,3 +131,29 @@ def test_gpu_48_block_configuration_first_block_equals_oracle():
     cpu = run("cpu", sizes[:1], 100)
     assert np.array_equal(gpu[0], cpu[0])
     assert np.isfinite(gpu[1]).all()
+
+
+@pytest.mark.gpu
+def test_gpu_graph_replay_equals_eager(monkeypatch):
+    # the step is recorded once and replayed; replays separated by fetches (host
+    # synchronisation) must match eager launches after every replay
+    sizes = [(40, 33, 17), (9, 65, 30)]
+
+    def staged(graph):
+        monkeypatch.setenv("OPS_ADI_GRAPH", "1" if graph else "0")
+        L = lib("adi", "gpu")
+        sz = np.array([v for s in sizes for v in s], dtype=np.int32)
+        argc, av = argv()
+        assert L.adi_init(argc, av, len(sizes), p(sz), 1, 9, 1.0) == 0
+        snaps = []
+        for n in (3, 2, 4):
+            assert L.adi_step(n) == 0
+            for k in range(len(sizes)):
+                a = np.zeros(L.adi_fetch(k, None))
+                L.adi_fetch(k, p(a))
+                snaps.append(a)
+        assert L.adi_finalize() == 0
+        return snaps
+
+    for a, b in zip(staged(True), staged(False)):
+        assert np.array_equal(a, b)
