@@ -247,6 +247,12 @@ def adi_gpu(args):
     cells = sum(a * b * c for a, b, c in sizes)
     sz = np.array([v for s in sizes for v in s], dtype=np.int32)
     argc, argv = ops_amd.argv()
+    # warm-up session with eager launches (autotuning happens here), then the timed one
+    os.environ["OPS_ADI_GRAPH"] = "0"
+    assert L.adi_init(argc, argv, len(sizes), sz.ctypes.data_as(ctypes.c_void_p), 1, 5, 1.0) == 0
+    assert L.adi_step(3) == 0
+    assert L.adi_finalize() == 0
+    os.environ["OPS_ADI_GRAPH"] = "1"
     assert L.adi_init(argc, argv, len(sizes), sz.ctypes.data_as(ctypes.c_void_p), 1, 5, 1.0) == 0
     assert L.adi_step(2) == 0  # eager step + graph capture + one replay
     R.ops_hip_device_sync()
@@ -294,7 +300,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=87)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=12)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the runtime's default workgroup shapes (no OPS_AUTOTUNE)")
     ap.add_argument("--workload", default="cloverleaf", choices=["cloverleaf"])
     ap.add_argument("--cells", type=int, default=3840)
     ap.add_argument("--no-cpu", action="store_true")
@@ -312,6 +320,10 @@ def main():
     if args.ingest_pmc:
         ingest_pmc(Path(args.ingest_pmc))
         return
+    if not args.no_autotune:
+        # the runtime measures candidate workgroup shapes (and tridiagonal tile heights) on
+        # the warm-up launches and keeps the fastest for the timed ones
+        os.environ["OPS_AUTOTUNE"] = "1"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(see DESIGN.md for how the mesh would be decomposed)", file=sys.stderr)
@@ -340,7 +352,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"cloverleaf_2d clover_bm16_short.in {n}x{n} cells, "
                                f"{args.steps} hydro steps", "cells": n * n,
-                   "halo_depth": 2, "parallelism": "single GPU"},
+                   "halo_depth": 2, "parallelism": "single GPU",
+                   "autotune": not args.no_autotune},
         "roofline": roof,
         "step_bandwidth": whole_step_bw(stats, secs),
         "checks": checks,

@@ -50,6 +50,7 @@ typedef struct ops_hip_loop {
   /* bookkeeping */
   int kernel_id;
   int timed;
+  int tune_cand; /* autotuning candidate measured by this launch, -1 otherwise */
   double bytes;  /* ops_compute_transfer bytes of this launch */
 } ops_hip_loop;
 

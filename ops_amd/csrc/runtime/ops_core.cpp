@@ -110,6 +110,7 @@ static void parse_arg(OPS_instance *in, const char *a) {
   if ((p = strstr(a, "OPS_DEVICE="))) in->device_id = atoi(p + 11);
   if ((p = strstr(a, "OPS_DUMP_DIR="))) in->dump_dir = std::string(p + 13);
   if ((p = strstr(a, "OPS_HIP_TIMING="))) in->timing = atoi(p + 15);
+  if ((p = strstr(a, "OPS_AUTOTUNE="))) g_autotune = atoi(p + 13);
   // other reference tokens (OPS_TILING, OPS_CACHE_SIZE, OPS_FORCE_DECOMP_*, checkpointing)
   // are accepted and ignored
 }
@@ -120,6 +121,8 @@ extern "C" void ops_init(const int argc, const char *const argv[], const int dia
   in->diags = diags_level;
   const char *env = getenv("OPS_DUMP_DIR");
   if (env) in->dump_dir = env;
+  const char *aenv = getenv("OPS_AUTOTUNE");
+  if (aenv) g_autotune = atoi(aenv);
   const char *tenv = getenv("OPS_HIP_TIMING");
   if (tenv) in->timing = atoi(tenv);
   for (int i = 1; i < argc; i++)

@@ -39,6 +39,7 @@ struct DevState {
   struct Pending {
     int kid;
     hipEvent_t a, b;
+    int cand;  // autotuning candidate measured by this launch, -1 for plain timing
   };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
@@ -46,6 +47,7 @@ struct DevState {
 };
 DevState g_dev;
 int g_timing = 0;
+bool g_capturing = false;
 
 hipEvent_t get_event() {
   if (!g_dev.pool.empty()) {
@@ -59,6 +61,8 @@ hipEvent_t get_event() {
 }
 
 }  // namespace
+
+int g_autotune = 0;
 
 extern "C" void ops_hip_check(int status, const char *what, const char *file, int line) {
   if (status != (int)hipSuccess) {
@@ -189,6 +193,43 @@ extern "C" void ops_hip_device_sync(void) { ops_dev_sync(); }
 
 extern "C" void ops_hip_set_timing(int on) { g_timing = on; }
 
+int ops_tune_next(int kid, int ncand, int reps) {
+  auto &kt = ops_kernel_table();
+  if (!g_autotune || g_capturing || kid < 0 || kid >= (int)kt.size()) return -1;
+  ops_kernel_entry &e = kt[kid];
+  if (e.tune_state < 0) return -1;
+  if (e.tune_state == 2) return e.tune_choice;
+  if (ncand > 8) ncand = 8;
+  e.tune_state = 1;
+  return e.tune_calls % ncand;
+  (void)reps;
+}
+
+void ops_tune_begin(int kid, int cand) {
+  hipEvent_t a = get_event(), b = get_event();
+  HIPCHECK(hipEventRecord(a, g_dev.stream));
+  g_dev.pending.push_back({kid, a, b, cand});
+}
+
+void ops_tune_end(int kid, int ncand, int reps) {
+  auto &kt = ops_kernel_table();
+  ops_kernel_entry &e = kt[kid];
+  HIPCHECK(hipEventRecord(g_dev.pending.back().b, g_dev.stream));
+  if (++e.tune_calls < ncand * reps) return;
+  ops_dev_timing_flush();
+  int best = 0;
+  double best_ms = 1e300;
+  for (int c = 0; c < ncand && c < 8; c++) {
+    const double avg = e.cand_n[c] ? e.cand_ms[c] / e.cand_n[c] : 1e300;
+    if (avg < best_ms) {
+      best_ms = avg;
+      best = c;
+    }
+  }
+  e.tune_choice = best;
+  e.tune_state = 2;
+}
+
 void ops_dev_timing_flush() {
   if (g_dev.pending.empty()) return;
   HIPCHECK(hipEventSynchronize(g_dev.pending.back().b));
@@ -197,6 +238,10 @@ void ops_dev_timing_flush() {
     float ms = 0.f;
     HIPCHECK(hipEventElapsedTime(&ms, p.a, p.b));
     kt[p.kid].total_ms += ms;
+    if (p.cand >= 0 && p.cand < 8) {
+      kt[p.kid].cand_ms[p.cand] += ms;
+      kt[p.kid].cand_n[p.cand]++;
+    }
     g_dev.pool.push_back(p.a);
     g_dev.pool.push_back(p.b);
   }
@@ -364,6 +409,31 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       bz = 1;
     }
   }
+  // autotuning: workgroup shapes tried on the kernel's first launches. Loops with a sum
+  // reduction are left alone, since the shape sets the order of the partial sums.
+  lp->tune_cand = -1;
+  if (g_autotune && kernel_id >= 0 && kernel_id < (int)kt.size() && kt[kernel_id].tune_state >= 0 &&
+      kt[kernel_id].bx == 0 && in->block_size_x <= 0 && in->block_size_y <= 0) {
+    bool sum_red = false;
+    for (int i = 0; i < nargs; i++)
+      if (args[i].argtype == OPS_ARG_GBL && args[i].acc == OPS_INC) sum_red = true;
+    if (sum_red) {
+      kt[kernel_id].tune_state = -1;
+    } else {
+      static const unsigned shapes2[][3] = {{64, 4, 1}, {128, 2, 1}, {256, 1, 1}, {64, 2, 1}, {32, 8, 1}};
+      static const unsigned shapes3[][3] = {{64, 4, 1}, {64, 2, 2}, {32, 4, 2}, {64, 1, 4}, {128, 2, 1}};
+      static const unsigned shapes1[][3] = {{256, 1, 1}, {128, 1, 1}, {64, 1, 1}};
+      const int nc = dim >= 3 ? 5 : (dim == 2 ? 5 : 3);
+      const int c = ops_tune_next(kernel_id, nc, 2);
+      if (c >= 0) {
+        const unsigned *sh = dim >= 3 ? shapes3[c] : (dim == 2 ? shapes2[c] : shapes1[c]);
+        bx = sh[0];
+        by = sh[1];
+        bz = sh[2];
+        if (kt[kernel_id].tune_state == 1) lp->tune_cand = c;
+      }
+    }
+  }
   if ((bx * by * bz) % OPS_WAVE != 0 || bx * by * bz > OPS_HIP_BLOCK_THREADS)
     ops_throw(OPS_INVALID_ARGUMENT,
               "ops_par_loop: workgroup shape must be a whole number of 64-lane wavefronts "
@@ -401,10 +471,13 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       lp->red_value[i] = h->data_d;
       lp->red_init[i] = h->dirty_d ? 0 : 1;
     }
-  if (g_timing) {
+  if (lp->tune_cand >= 0) {
+    ops_tune_begin(kernel_id, lp->tune_cand);
+    lp->timed = 1;
+  } else if (g_timing) {
     hipEvent_t a = get_event(), b = get_event();
     HIPCHECK(hipEventRecord(a, g_dev.stream));
-    g_dev.pending.push_back({kernel_id, a, b});
+    g_dev.pending.push_back({kernel_id, a, b, -1});
     lp->timed = 1;
   }
   return 1;
@@ -479,7 +552,9 @@ extern "C" void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs) {
   if (lp->nblocks > 0)
     for (int i = 0; i < nargs; i++)
       if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) finish_reduction(lp, args[i], i);
-  if (lp->timed) {
+  if (lp->timed && lp->tune_cand >= 0) {
+    ops_tune_end(lp->kernel_id, lp->ndim >= 2 ? 5 : 3, 2);
+  } else if (lp->timed) {
     HIPCHECK(hipEventRecord(g_dev.pending.back().b, g_dev.stream));
     if (g_dev.pending.size() >= 8192) ops_dev_timing_flush();
   }
@@ -725,12 +800,15 @@ extern "C" int ops_hip_graph_begin(void) {
   if (!g_dev.live) return 1;
   ops_trid_flush();  // work deferred before the capture runs now, not inside the graph
   if (g_timing) ops_throw(OPS_RUNTIME_ERROR, "ops_hip_graph_begin: per-kernel timing is on");
-  return hipStreamBeginCapture(g_dev.stream, hipStreamCaptureModeThreadLocal) == hipSuccess ? 0 : 1;
+  if (hipStreamBeginCapture(g_dev.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return 1;
+  g_capturing = true;
+  return 0;
 }
 
 extern "C" void *ops_hip_graph_end(void) {
   ops_trid_flush();
   hipGraph_t g = nullptr;
+  g_capturing = false;
   if (hipStreamEndCapture(g_dev.stream, &g) != hipSuccess || g == nullptr) return nullptr;
   hipGraphExec_t e = nullptr;
   const hipError_t st = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);

@@ -16,6 +16,13 @@ struct ops_kernel_entry {
   double bytes = 0.0;     // ops_compute_transfer bytes summed over calls
   double host_s = 0.0;    // host wall time spent in begin..end (stub overhead)
   int bx = 0, by = 0, bz = 0;  // tuned workgroup shape, 0 = default
+  // autotuning (OPS_AUTOTUNE): candidates are tried in turn on real launches, each timed
+  // with HIP events, and the fastest average is kept
+  int tune_state = 0;  // 0 not started, 1 measuring, 2 decided, -1 not tunable
+  int tune_calls = 0;  // measured launches so far
+  int tune_choice = -1;
+  double cand_ms[8] = {0};
+  int cand_n[8] = {0};
 };
 
 struct ops_const_entry {
@@ -70,6 +77,14 @@ void *ops_hip_get_stream_raw();
 // launches the deferred tridiagonal batch, if any (every launch, copy and synchronisation
 // of the device layer calls it first)
 void ops_trid_flush();
+// Autotuning seam. ops_tune_next returns the candidate (0..ncand-1) the next launch of
+// kernel kid should use while it is being measured, the chosen one afterwards, or -1 when
+// tuning is off (or the kernel is not tunable). A measured launch is bracketed by
+// ops_tune_begin/ops_tune_end, which decides once every candidate has `reps` samples.
+int ops_tune_next(int kid, int ncand, int reps);
+void ops_tune_begin(int kid, int cand);
+void ops_tune_end(int kid, int ncand, int reps);
+extern int g_autotune;
 void ops_dev_halo_copy(ops_halo h);
 // packed slab copy between a dat (device) and a dense buffer; dir 0: dat->buf, 1: buf->dat
 void ops_dev_slab_copy(ops_dat dat, char *buf, int buf_on_device, const int *range, int dir);

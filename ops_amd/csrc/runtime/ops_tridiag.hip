@@ -34,8 +34,8 @@
 
 namespace {
 
-constexpr int MAXSYS = 16;       // systems per launch (kernel-argument size)
-constexpr int XL = 64, XR = 16;  // x-lines: lines per workgroup, rows per LDS tile
+constexpr int MAXSYS = 16;  // systems per launch (kernel-argument size)
+constexpr int XL = 64;      // x-lines: lines per workgroup (rows per LDS tile: XR)
 
 template <typename T> struct Sys {
   T *a, *b, *c, *d, *u;  // box origins
@@ -111,7 +111,9 @@ __global__ __launch_bounds__(256) void thomas_strided(const Batch<T> B, T *__res
 }
 
 // x-lines: 64 lines of one system per workgroup, rows staged through LDS in chunks of XR
-template <typename T, bool INC>
+// (16, 8 or 4: taller tiles coalesce better, shorter ones leave LDS for more workgroups
+// per CU; the autotuner picks)
+template <typename T, bool INC, int XR>
 __global__ __launch_bounds__(64) void thomas_x(const Batch<T> B, T *__restrict__ scratch) {
   __shared__ T sa[XL][XR + 1], sb[XL][XR + 1], sc[XL][XR + 1], sd[XL][XR + 1];
   const int lane = threadIdx.x;
@@ -260,9 +262,21 @@ void launch_chunk(const std::vector<Call> &calls, size_t first, size_t count) {
   if (B.nsys == 0) return;
   T *scr_p = (T *)scratch(sizeof(T) * (size_t)scr);
   hipStream_t st = (hipStream_t)ops_hip_get_stream_raw();
-  if (sd == 0)
-    hipLaunchKernelGGL((thomas_x<T, INC>), dim3((unsigned)(line / XL)), dim3(XL), 0, st, B, scr_p);
-  else
+  if (sd == 0) {
+    static const int kid = ops_hip_register_kernel("ops_tridiag_x");
+    const int c = ops_tune_next(kid, 3, 2);
+    const dim3 grid((unsigned)(line / XL));
+    if (c > 0 && ops_kernel_table()[kid].tune_state == 1) ops_tune_begin(kid, c);
+    else if (c == 0 && ops_kernel_table()[kid].tune_state == 1) ops_tune_begin(kid, 0);
+    const bool measured = c >= 0 && ops_kernel_table()[kid].tune_state == 1;
+    if (c == 1)
+      hipLaunchKernelGGL((thomas_x<T, INC, 8>), grid, dim3(XL), 0, st, B, scr_p);
+    else if (c == 2)
+      hipLaunchKernelGGL((thomas_x<T, INC, 4>), grid, dim3(XL), 0, st, B, scr_p);
+    else
+      hipLaunchKernelGGL((thomas_x<T, INC, 16>), grid, dim3(XL), 0, st, B, scr_p);
+    if (measured) ops_tune_end(kid, 3, 2);
+  } else
     hipLaunchKernelGGL((thomas_strided<T, INC>), dim3((unsigned)((line + 255) / 256)), dim3(256), 0,
                        st, B, scr_p);
   ops_hip_check_launch("ops_tridMultiDimBatch");
