@@ -29,6 +29,19 @@ SIGNATURES = {
         "tealeaf_state": (_i, [_p]),
         "tealeaf_fetch": (ctypes.c_long, [ctypes.c_char_p, _p]),
         "tealeaf_finalize": (_i, []),
+        "tealeaf_set_guess": (_i, [_i, _i]),
+        "tealeaf_capture_next": (_i, [_i]),
+        "tealeaf_guess_state": (_i, [_p]),
+        "tea_unet_init": (_i, [ctypes.c_ulonglong]),
+        "tea_unet_num_params": (ctypes.c_long, []),
+        "tea_unet_get_params": (_i, [_p]),
+        "tea_unet_set_params": (_i, [_p]),
+        "tea_unet_forward_host": (_i, [_i, _i, _i, _p, _p]),
+        "tea_unet_backward_host": (_i, [_i, _i, _i, _p, _p, _p]),
+        "tea_unet_dataset_reserve": (_i, [_i, _i, _i]),
+        "tea_unet_dataset_set": (_i, [_i, _p, _p]),
+        "tea_unet_train": (_i, [_i, _i, ctypes.c_float, ctypes.c_float, ctypes.c_ulonglong, _p]),
+        "tea_unet_free": (_i, []),
     },
     "apitest": {
         "api_mblock": (_i, [_i, _pp, _i, _p, _p]),

@@ -1,0 +1,1246 @@
+// Learned warm start for the TeaLeaf implicit solve: a 4-level 2-D U-Net trained on the GPU
+// by its own forward and backward kernels (no reference counterpart; interface in
+// tealeaf_unet.h).
+//
+// Network (channels C = {16, 32, 48, 64}, every convolution 3 x 3, zero padding):
+//   enc0 = relu(conv(in))                  H   x W,   16 channels (in: 16, 4 used)
+//   enc1 = relu(conv(pool(enc0)))          H/2,       32
+//   enc2 = relu(conv(pool(enc1)))          H/4,       48
+//   enc3 = relu(conv(pool(enc2)))          H/8,       64
+//   dec2 = relu(conv([up(enc3), enc2]))    H/4,       48   (112 input channels)
+//   dec1 = relu(conv([up(dec2), enc1]))    H/2,       32   (80)
+//   dec0 = relu(conv([up(dec1), enc0]))    H,         16   (48)
+//   out  = conv(dec0)                      H,         16   (channel 0 is the prediction)
+// pool = 2 x 2 average, up = nearest-neighbour copy, [a, b] = channel concatenation (the
+// decoder convolutions read both sources directly; nothing is copied).
+//
+// Storage: activations NHWC in BF16, weights [Cout][3][3][Cin] in BF16 (an FP32 master copy
+// is updated by Adam), gradients FP32. Convolutions are implicit GEMMs on MFMA
+// (v_mfma_f32_16x16x16bf16_1k, FP32 accumulation): M = pixels, N = output channels,
+// K = 9 taps x input channels. The lane layout of the instruction (A: lane l holds
+// A[l % 16][4 (l / 16) + j], B: B[4 (l / 16) + j][l % 16], D: D[4 (l / 16) + j][l % 16],
+// j = 0..3) was checked on the device against a host product.
+//   forward / input gradient: a 256-lane workgroup covers 64 pixels of one row, each
+//     wavefront a 16-pixel M tile and all output channels (up to 7 N tiles: the input
+//     gradient of the first decoder layer has 112 channels);
+//   weight gradient: one wavefront per (tap, band of rows), all (Cout, Cin) tiles in
+//     registers, K = pixels; bands are summed by a second kernel in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ops_hip_kernel.h"
+#include "ops_hip_rt.h"
+#include "tealeaf_native.h"
+#include "tealeaf_unet.h"
+
+#define CK(x) ops_hip_check((int)(x), #x, __FILE__, __LINE__)
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+__host__ __device__ inline bf16 f2bf(float f) {
+  // round to nearest even
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+__host__ __device__ inline float bf2f(bf16 h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+constexpr int NL = 8;  // convolution layers
+// layer table: input channels (source 1 = upsampled, source 2 = same resolution), output
+// channels, resolution level
+struct LayerSpec {
+  int c1, c2, cout, level, relu;
+};
+constexpr LayerSpec SPEC[NL] = {
+    {0, 16, 16, 0, 1},   // enc0 (input padded to 16)
+    {0, 16, 32, 1, 1},   // enc1 on pool(enc0)
+    {0, 32, 48, 2, 1},   // enc2 on pool(enc1)
+    {0, 48, 64, 3, 1},   // enc3 on pool(enc2)
+    {64, 48, 48, 2, 1},  // dec2 on [up(enc3), enc2]
+    {48, 32, 32, 1, 1},  // dec1 on [up(dec2), enc1]
+    {32, 16, 16, 0, 1},  // dec0 on [up(dec1), enc0]
+    {0, 16, 16, 0, 0},   // out on dec0
+};
+
+// ---------------------------------------------------------------- convolution kernels
+
+struct ConvArgs {
+  const bf16 *src1;  // coarse source (H/2 x W/2 x c1), read at (y/2, x/2); may be null
+  const bf16 *src2;  // same-resolution source (H x W x c2)
+  int c1, c2, cout, H, W;
+  const bf16 *w;     // [cout][9][c1 + c2]
+  const float *bias; // [cout] or null
+  bf16 *out;         // [H][W][cout] or null
+  float *out32;      // [H][W][cout] or null
+  int relu;
+};
+
+// forward convolution (and the input-gradient convolution with transformed weights)
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int y = blockIdx.y, b = blockIdx.z;
+  const int H = a.H, W = a.W, cin = a.c1 + a.c2;
+  const int xm = blockIdx.x * 64 + wv * 16 + m;  // this lane's A-row pixel
+  const int nt = a.cout >> 4;  // <= 7, checked by the host
+  f4 acc[7];
+#pragma unroll
+  for (int t = 0; t < 7; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  const int Hc = H >> 1, Wc = W >> 1;
+  for (int tap = 0; tap < 9; tap++) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int xs = xm + dx, ys = y + dy;
+    const bool inb = xs >= 0 && xs < W && ys >= 0 && ys < H;
+    for (int c0 = 0; c0 < cin; c0 += 16) {
+      const int c = c0 + 4 * kq;
+      s4 av = s4{0, 0, 0, 0};
+      if (inb) {
+        const bf16 *p = (c < a.c1)
+                            ? a.src1 + (((long)b * Hc + (ys >> 1)) * Wc + (xs >> 1)) * a.c1 + c
+                            : a.src2 + (((long)b * H + ys) * W + xs) * a.c2 + (c - a.c1);
+        av = *(const s4 *)p;
+      }
+#pragma unroll
+      for (int t = 0; t < 7; t++) {
+        if (t < nt) {
+          const s4 bv = *(const s4 *)(a.w + ((long)(16 * t + m) * 9 + tap) * cin + c);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // D[pixel 4 kq + j][channel m] of each N tile
+#pragma unroll
+  for (int t = 0; t < 7; t++) {
+    if (t >= nt) continue;
+    const int co = 16 * t + m;
+    const float bia = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int px = blockIdx.x * 64 + wv * 16 + 4 * kq + j;
+      if (px >= W) continue;
+      float v = acc[t][j] + bia;
+      if (a.relu && v < 0.f) v = 0.f;
+      const long o = (((long)b * H + y) * W + px) * a.cout + co;
+      if (a.out) a.out[o] = f2bf(v);
+      if (a.out32) a.out32[o] = v;
+    }
+  }
+}
+
+struct WgradArgs {
+  const bf16 *dy;    // masked output gradient [B][H][W][cout] (BF16)
+  const bf16 *src1;  // the layer's input sources, as in ConvArgs
+  const bf16 *src2;
+  int c1, c2, cout, H, W, rows, band;  // rows = B*H, band = rows per wavefront
+  float *part;       // [nbands][9][cout][cin]
+};
+
+// partial weight gradient of one band of rows for one tap
+__global__ __launch_bounds__(64) void k_wgrad(WgradArgs a) {
+  const int lane = threadIdx.x, m = lane & 15, kq = lane >> 4;
+  const int band = blockIdx.x, tap = blockIdx.y;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const int H = a.H, W = a.W, cin = a.c1 + a.c2, Hc = H >> 1, Wc = W >> 1;
+  const int ntc = a.cout >> 4, ntn = cin >> 4;  // ntc <= 4, ntn <= 7
+  f4 acc[4][7];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 7; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int r0 = band * a.band, r1 = min(a.rows, r0 + a.band);
+  for (int r = r0; r < r1; r++) {
+    const int b = r / H, y = r % H, ys = y + dy;
+    for (int x0 = 0; x0 < W; x0 += 16) {
+      // A = dy^T (cout x pixels): lane holds channel 16 tc + m, pixels x0 + 4 kq + j
+      s4 av[4];
+#pragma unroll
+      for (int tc = 0; tc < 4; tc++) {
+        av[tc] = s4{0, 0, 0, 0};
+        if (tc < ntc) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int px = x0 + 4 * kq + j;
+            if (px < W) av[tc][j] = (short)a.dy[(((long)b * H + y) * W + px) * a.cout + 16 * tc + m];
+          }
+        }
+      }
+#pragma unroll
+      for (int tn = 0; tn < 7; tn++) {
+        if (tn >= ntn) continue;
+        // B = shifted input (pixels x cin): lane holds channel 16 tn + m, pixels x0 + 4 kq + j
+        s4 bv = s4{0, 0, 0, 0};
+        const int c = 16 * tn + m;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int xs = x0 + 4 * kq + j + dx;
+          if (x0 + 4 * kq + j < W && xs >= 0 && xs < W && ys >= 0 && ys < H) {
+            const bf16 v = (c < a.c1) ? a.src1[(((long)b * Hc + (ys >> 1)) * Wc + (xs >> 1)) * a.c1 + c]
+                                      : a.src2[(((long)b * H + ys) * W + xs) * a.c2 + (c - a.c1)];
+            bv[j] = (short)v;
+          }
+        }
+#pragma unroll
+        for (int tc = 0; tc < 4; tc++)
+          if (tc < ntc) acc[tc][tn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[tc], bv, acc[tc][tn], 0, 0, 0);
+      }
+    }
+  }
+  // D[co = 16 tc + 4 kq + j][ci = 16 tn + m]
+  float *out = a.part + ((long)band * 9 + tap) * a.cout * cin;
+#pragma unroll
+  for (int tc = 0; tc < 4; tc++)
+#pragma unroll
+    for (int tn = 0; tn < 7; tn++)
+      if (tc < ntc && tn < ntn)
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[(long)(16 * tc + 4 * kq + j) * cin + 16 * tn + m] = acc[tc][tn][j];
+}
+
+// dW[co][tap][ci] = sum over bands (fixed order)
+__global__ __launch_bounds__(256) void k_wgrad_sum(const float *part, int nbands, int cout, int cin,
+                                                   float *dw) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;  // over [9][cout][cin]
+  const long n = 9L * cout * cin;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int k = 0; k < nbands; k++) s += part[(long)k * n + i];
+  const int tap = (int)(i / ((long)cout * cin)), rem = (int)(i % ((long)cout * cin));
+  const int co = rem / cin, ci = rem % cin;
+  dw[((long)co * 9 + tap) * cin + ci] = s;
+}
+
+// masked gradient in BF16 for the MFMA operands, and per-workgroup partial bias gradient
+__global__ __launch_bounds__(256) void k_mask(const float *g, const bf16 *y, int relu, long n,
+                                              int cout, bf16 *gm, float *gmf) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float v = g[i];
+  if (relu && bf2f(y[i]) <= 0.f) v = 0.f;
+  gm[i] = f2bf(v);
+  gmf[i] = v;
+  (void)cout;
+}
+
+// bias gradient: db[co] = sum over pixels of the masked gradient (one workgroup per
+// channel, fixed order)
+__global__ __launch_bounds__(256) void k_bias_grad(const float *gmf, long npix, int cout, float *db) {
+  __shared__ float lds[4];
+  const int co = blockIdx.x;
+  float s = 0.f;
+  for (long p = threadIdx.x; p < npix; p += 256) s += gmf[p * cout + co];
+  s = ops_block_reduce<ops_red_inc<float>>(s, lds);
+  if (threadIdx.x == 0) db[co] = s;
+}
+
+// weights for the input gradient: wt[ci][8 - tap][co] = w[co][tap][ci]
+__global__ __launch_bounds__(256) void k_wtrans(const bf16 *w, int cout, int cin, bf16 *wt) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= 9 * cout * cin) return;
+  const int co = i / (9 * cin), rem = i % (9 * cin), tap = rem / cin, ci = rem % cin;
+  wt[((long)ci * 9 + (8 - tap)) * cout + co] = w[i];
+}
+
+// 2 x 2 average pooling, BF16 -> BF16 (FP32 sum)
+__global__ __launch_bounds__(256) void k_pool(const bf16 *in, int B, int H, int W, int C, bf16 *out) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  const int Hc = H / 2, Wc = W / 2;
+  const long n = (long)B * Hc * Wc * C;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  const long p = i / C;
+  const int X = (int)(p % Wc), Y = (int)((p / Wc) % Hc), b = (int)(p / ((long)Wc * Hc));
+  const long base = (((long)b * H + 2 * Y) * W + 2 * X) * C + c;
+  const float s = (bf2f(in[base]) + bf2f(in[base + C])) + (bf2f(in[base + (long)W * C]) + bf2f(in[base + (long)W * C + C]));
+  out[i] = f2bf(0.25f * s);
+}
+
+// adjoint of the pooling: gfine += 0.25 * gcoarse(parent)
+__global__ __launch_bounds__(256) void k_unpool_add(const float *gc, int B, int H, int W, int C, float *gf) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  const long n = (long)B * H * W * C;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  const long p = i / C;
+  const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((long)W * H));
+  gf[i] += 0.25f * gc[(((long)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+}
+
+// split the input gradient of a decoder layer ([up(src1), src2] at H x W):
+// gsrc2 += g[..., c1:], gsrc1(coarse) = sum of the 2 x 2 children of g[..., :c1]
+__global__ __launch_bounds__(256) void k_split_up(const float *g, int B, int H, int W, int c1, int c2,
+                                                  float *g1, float *g2) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  const int cin = c1 + c2;
+  const long n2 = (long)B * H * W * c2, n1 = (long)B * (H / 2) * (W / 2) * c1;
+  if (i < n2) {
+    const int c = (int)(i % c2);
+    const long p = i / c2;
+    g2[i] += g[p * cin + c1 + c];
+  }
+  if (i < n1) {
+    const int c = (int)(i % c1);
+    const long p = i / c1;
+    const int X = (int)(p % (W / 2)), Y = (int)((p / (W / 2)) % (H / 2)), b = (int)(p / ((long)(W / 2) * (H / 2)));
+    const long f = (((long)b * H + 2 * Y) * W + 2 * X) * cin + c;
+    g1[i] = (g[f] + g[f + cin]) + (g[f + (long)W * cin] + g[f + (long)W * cin + cin]);
+  }
+}
+
+// Loss on channel 0 of the output: the error e = prediction - target in the norm
+// e^T (I + beta C) e, C the conduction part of the sample's operator (A = I + C), relative
+// to the target's, averaged over the batch. beta = 0 gives the relative mean squared
+// error, beta = 1 the energy norm of A (the error measure the CG iteration count depends
+// on, which weights high-frequency noise by the operator's large eigenvalues).
+// coef[pixel] = {rx Kx, ry Ky} of the faces on the low side of the pixel (faces on the
+// mesh boundary carry zero); norm[2 b] = t^T t, norm[2 b + 1] = t^T C t. Gradient
+// 2 (I + beta C) e / (norm_b * batch) on channel 0, zero on channels 1..15; partial losses
+// per workgroup.
+__global__ __launch_bounds__(256) void k_loss(const float *pred, const float *target, const float *coef,
+                                              const double *norm, float beta, int B, int H, int W, int cout,
+                                              float *grad, float *part) {
+  __shared__ float lds[4];
+  const long npix = (long)B * H * W;
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  float l = 0.f;
+  if (p < npix) {
+    const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((long)W * H));
+    auto err = [&](long q) { return pred[q * cout] - target[q]; };
+    const float e = err(p);
+    float ce = 0.f;
+    if (beta != 0.f) {
+      if (x > 0) ce += coef[2 * p] * (e - err(p - 1));
+      if (x + 1 < W) ce += coef[2 * (p + 1)] * (e - err(p + 1));
+      if (y > 0) ce += coef[2 * p + 1] * (e - err(p - W));
+      if (y + 1 < H) ce += coef[2 * (p + W) + 1] * (e - err(p + W));
+    }
+    const float me = e + beta * ce;
+    const float inv = (float)(1.0 / (norm[2 * b] + (double)beta * norm[2 * b + 1]));
+    l = e * me * inv;
+    grad[p * cout] = 2.f * me * inv / (float)B;
+    for (int c = 1; c < cout; c++) grad[p * cout + c] = 0.f;
+  }
+  l = ops_block_reduce<ops_red_inc<float>>(l, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = l;
+}
+
+// Adam on the FP32 master copy, BF16 copy refreshed
+__global__ __launch_bounds__(256) void k_adam(float *w, const float *g, float *m1, float *m2, bf16 *wb,
+                                              long n, float lr, float b1, float b2, float eps,
+                                              float c1, float c2) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.f - b1) * gi, v = b2 * m2[i] + (1.f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = v;
+  const float wn = w[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+  w[i] = wn;
+  if (wb) wb[i] = f2bf(wn);
+}
+
+__global__ __launch_bounds__(256) void k_add(const float *a, long n, float *b) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] += a[i];
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(const float *a, bf16 *b, long n) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = f2bf(a[i]);
+}
+
+__global__ __launch_bounds__(256) void k_fill(float *a, long n, float v) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+unsigned g1d(long n) { return (unsigned)((n + 255) / 256); }
+
+// ---------------------------------------------------------------- host side
+
+struct Net {
+  bool live = false;
+  // parameters: per layer weights (cout*9*cin) then bias (cout), FP32 master, Adam moments
+  long woff[NL], boff[NL], nparam = 0;
+  float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+  bf16 *wb = nullptr, *wt = nullptr;  // BF16 weights, transformed weights (largest layer)
+  long step = 0;
+  // activations for a batch of B images of H x W
+  int B = 0, H = 0, W = 0;
+  bf16 *act[NL] = {};   // layer outputs (BF16)
+  bf16 *pooled[4] = {}; // pooled inputs of enc1..enc3 (index level)
+  bf16 *in = nullptr;   // input, 16 channels
+  float *out32 = nullptr, *target = nullptr, *coef = nullptr;
+  double *norm = nullptr;  // per image of the batch, {t^T t, t^T C t}
+  float *grad[NL] = {}; // gradients w.r.t. layer outputs (FP32)
+  float *gin = nullptr; // input gradient of a layer (largest)
+  float *gpool = nullptr;
+  bf16 *gm = nullptr;   // masked output gradient, BF16
+  float *gmf = nullptr; // same, FP32
+  float *wpart = nullptr;
+  size_t wpart_cap = 0;
+  float *lpart = nullptr;
+  char *pinned = nullptr;
+};
+Net N;
+
+hipStream_t stream() { return (hipStream_t)ops_hip_get_stream(); }
+void sync() { CK(hipStreamSynchronize(stream())); }
+
+int cin_of(int l) { return SPEC[l].c1 + SPEC[l].c2; }
+
+template <typename T> T *dalloc(size_t n) {
+  T *p = nullptr;
+  CK(hipMalloc((void **)&p, n * sizeof(T)));
+  CK(hipMemset(p, 0, n * sizeof(T)));
+  return p;
+}
+template <typename T> void dfree(T *&p) {
+  if (p) CK(hipFree(p));
+  p = nullptr;
+}
+
+void free_acts() {
+  for (int l = 0; l < NL; l++) {
+    dfree(N.act[l]);
+    dfree(N.grad[l]);
+  }
+  for (int l = 0; l < 4; l++) dfree(N.pooled[l]);
+  dfree(N.in);
+  dfree(N.out32);
+  dfree(N.target);
+  dfree(N.coef);
+  dfree(N.norm);
+  dfree(N.gin);
+  dfree(N.gpool);
+  dfree(N.gm);
+  dfree(N.gmf);
+  dfree(N.lpart);
+  N.B = N.H = N.W = 0;
+}
+
+void ensure_acts(int B, int H, int W) {
+  if (!N.live) throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: call tea_unet_init first");
+  if (B == N.B && H == N.H && W == N.W) return;
+  if (B <= 0 || H <= 0 || W <= 0 || H % 8 || W % 8)
+    throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: B > 0, H and W positive multiples of 8");
+  sync();
+  free_acts();
+  N.B = B;
+  N.H = H;
+  N.W = W;
+  auto pix = [&](int level) { return (long)B * (H >> level) * (W >> level); };
+  long maxg = 0;
+  for (int l = 0; l < NL; l++) {
+    N.act[l] = dalloc<bf16>(pix(SPEC[l].level) * SPEC[l].cout);
+    N.grad[l] = dalloc<float>(pix(SPEC[l].level) * SPEC[l].cout);
+    maxg = std::max(maxg, pix(SPEC[l].level) * (long)std::max(cin_of(l), SPEC[l].cout));
+  }
+  for (int lv = 1; lv < 4; lv++) N.pooled[lv] = dalloc<bf16>(pix(lv) * SPEC[lv].c2);
+  N.in = dalloc<bf16>(pix(0) * 16);
+  N.out32 = dalloc<float>(pix(0) * 16);
+  N.target = dalloc<float>(pix(0));
+  N.coef = dalloc<float>(pix(0) * 2);
+  N.norm = dalloc<double>(2 * B);
+  N.gin = dalloc<float>(maxg);
+  N.gpool = dalloc<float>(maxg);
+  N.gm = dalloc<bf16>(maxg);
+  N.gmf = dalloc<float>(maxg);
+  N.lpart = dalloc<float>(g1d(pix(0)) + 1);
+}
+
+
+// k_conv assumes channel counts in multiples of 16, at most 7 output tiles, and src1 present
+// exactly when c1 > 0
+void launch_conv(const ConvArgs &a, int B) {
+  if (a.c1 % 16 || a.c2 % 16 || a.c2 <= 0 || a.cout % 16 || a.cout <= 0 || a.cout > 112 ||
+      (a.c1 > 0) != (a.src1 != nullptr) || !a.src2 || !a.w || a.H <= 0 || a.W <= 0 || (a.c1 && (a.H % 2 || a.W % 2)))
+    throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: convolution shape outside the kernel's limits");
+  hipLaunchKernelGGL(k_conv, dim3((unsigned)((a.W + 63) / 64), (unsigned)a.H, (unsigned)B), dim3(256), 0, stream(), a);
+}
+
+// sources of layer l
+void sources(int l, const bf16 *&s1, const bf16 *&s2) {
+  s1 = nullptr;
+  switch (l) {
+    case 0: s2 = N.in; break;
+    case 1: s2 = N.pooled[1]; break;
+    case 2: s2 = N.pooled[2]; break;
+    case 3: s2 = N.pooled[3]; break;
+    case 4: s1 = N.act[3]; s2 = N.act[2]; break;
+    case 5: s1 = N.act[4]; s2 = N.act[1]; break;
+    case 6: s1 = N.act[5]; s2 = N.act[0]; break;
+    default: s2 = N.act[6]; break;
+  }
+}
+
+void forward() {
+  hipStream_t st = stream();
+  const int B = N.B, H = N.H, W = N.W;
+  for (int l = 0; l < NL; l++) {
+    const int lv = SPEC[l].level, h = H >> lv, w = W >> lv;
+    if (l >= 1 && l <= 3) {  // pool the previous encoder output
+      const long n = (long)B * h * w * SPEC[l].c2;
+      hipLaunchKernelGGL(k_pool, dim3(g1d(n)), dim3(256), 0, st, N.act[l - 1], B, 2 * h, 2 * w,
+                         SPEC[l].c2, N.pooled[lv]);
+    }
+    ConvArgs a;
+    sources(l, a.src1, a.src2);
+    a.c1 = SPEC[l].c1;
+    a.c2 = SPEC[l].c2;
+    a.cout = SPEC[l].cout;
+    a.H = h;
+    a.W = w;
+    a.w = N.wb + N.woff[l];
+    a.bias = N.p + N.boff[l];
+    a.out = N.act[l];
+    a.out32 = (l == NL - 1) ? N.out32 : nullptr;
+    a.relu = SPEC[l].relu;
+    launch_conv(a, B);
+  }
+  ops_hip_check_launch("tea_unet forward");
+}
+
+// gradients of every parameter from N.grad[NL-1] (output gradient)
+void backward() {
+  hipStream_t st = stream();
+  const int B = N.B, H = N.H, W = N.W;
+  // zero the gradients that accumulate (skip connections, pooled paths)
+  for (int l = 0; l < NL - 1; l++) {
+    const long n = (long)B * (H >> SPEC[l].level) * (W >> SPEC[l].level) * SPEC[l].cout;
+    hipLaunchKernelGGL(k_fill, dim3(g1d(n)), dim3(256), 0, st, N.grad[l], n, 0.f);
+  }
+  for (int l = NL - 1; l >= 0; l--) {
+    const int lv = SPEC[l].level, h = H >> lv, w = W >> lv, cout = SPEC[l].cout, cin = cin_of(l);
+    const long npix = (long)B * h * w, n = npix * cout;
+    // masked gradient
+    hipLaunchKernelGGL(k_mask, dim3(g1d(n)), dim3(256), 0, st, N.grad[l], N.act[l], SPEC[l].relu, n, cout,
+                       N.gm, N.gmf);
+    hipLaunchKernelGGL(k_bias_grad, dim3(cout), dim3(256), 0, st, N.gmf, npix, cout, N.g + N.boff[l]);
+    // weight gradient
+    WgradArgs wa;
+    wa.dy = N.gm;
+    sources(l, wa.src1, wa.src2);
+    wa.c1 = SPEC[l].c1;
+    wa.c2 = SPEC[l].c2;
+    wa.cout = cout;
+    wa.H = h;
+    wa.W = w;
+    wa.rows = B * h;
+    wa.band = std::max(1, (B * h + 255) / 256);
+    const int nbands = (B * h + wa.band - 1) / wa.band;
+    const size_t need = (size_t)nbands * 9 * cout * cin;
+    if (need > N.wpart_cap) {
+      sync();
+      dfree(N.wpart);
+      N.wpart = dalloc<float>(need);
+      N.wpart_cap = need;
+    }
+    wa.part = N.wpart;
+    hipLaunchKernelGGL(k_wgrad, dim3((unsigned)nbands, 9), dim3(64), 0, st, wa);
+    hipLaunchKernelGGL(k_wgrad_sum, dim3(g1d(9L * cout * cin)), dim3(256), 0, st, N.wpart, nbands, cout, cin,
+                       N.g + N.woff[l]);
+    if (l == 0) break;  // no gradient w.r.t. the network input
+    // input gradient: convolution of the masked gradient with the transformed weights
+    hipLaunchKernelGGL(k_wtrans, dim3(g1d(9L * cout * cin)), dim3(256), 0, st, N.wb + N.woff[l], cout, cin, N.wt);
+    ConvArgs a;
+    a.src1 = nullptr;
+    a.src2 = N.gm;
+    a.c1 = 0;
+    a.c2 = cout;
+    a.cout = cin;
+    a.H = h;
+    a.W = w;
+    a.w = N.wt;
+    a.bias = nullptr;
+    a.out = nullptr;
+    a.out32 = N.gin;
+    a.relu = 0;
+    launch_conv(a, B);
+    // route it to the layers that produced the input
+    if (l >= 1 && l <= 3) {
+      // input = pool(act[l-1]): back through the average
+      const long nf = (long)B * (2 * h) * (2 * w) * SPEC[l].c2;
+      hipLaunchKernelGGL(k_unpool_add, dim3(g1d(nf)), dim3(256), 0, st, N.gin, B, 2 * h, 2 * w, SPEC[l].c2,
+                         N.grad[l - 1]);
+    } else if (l >= 4 && l <= 6) {
+      // input = [up(act[s1]), act[s2]]
+      const int s1 = (l == 4) ? 3 : l - 1, s2 = (l == 4) ? 2 : (l == 5 ? 1 : 0);
+      const long n2 = npix * SPEC[l].c2;
+      const long n1 = (long)B * (h / 2) * (w / 2) * SPEC[l].c1;
+      hipLaunchKernelGGL(k_split_up, dim3(g1d(std::max(n1, n2))), dim3(256), 0, st, N.gin, B, h, w, SPEC[l].c1,
+                         SPEC[l].c2, N.gpool, N.grad[s2]);
+      hipLaunchKernelGGL(k_add, dim3(g1d(n1)), dim3(256), 0, st, N.gpool, n1, N.grad[s1]);
+    } else {
+      // output layer: input = act[6]
+      hipLaunchKernelGGL(k_add, dim3(g1d(npix * cin)), dim3(256), 0, st, N.gin, npix * cin, N.grad[6]);
+    }
+  }
+  ops_hip_check_launch("tea_unet backward");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- dataset and training
+
+namespace {
+
+struct Data {
+  int count = 0, H = 0, W = 0;
+  bf16 *in = nullptr;       // [count][H][W][4] features
+  float *target = nullptr;  // [count][H][W]
+  float *coef = nullptr;    // [count][H][W][2] operator face coefficients {rx Kx, ry Ky}
+  double *stats = nullptr;  // [count][3] normalisation of each sample
+  double *norm = nullptr;   // [count][2] {t^T t, t^T C t} of each target (A = I + C)
+};
+Data D;
+
+// batch input: 16 channels (4 features, zeros) from the packed 4-channel samples
+__global__ __launch_bounds__(256) void k_expand(const bf16 *src, long npix, bf16 *dst) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= npix * 16) return;
+  const int c = (int)(i & 15);
+  dst[i] = c < 4 ? src[(i >> 4) * 4 + c] : (bf16)0;
+}
+
+__global__ __launch_bounds__(256) void k_pack_in(const float *src, long npix, bf16 *dst) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < npix * 4) dst[i] = f2bf(src[i]);
+}
+
+uint64_t mix(uint64_t &s) {  // splitmix64
+  s += 0x9E3779B97F4A7C15ull;
+  uint64_t z = s;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+float *scratch_f(size_t n) {
+  static float *p = nullptr;
+  static size_t cap = 0;
+  if (n > cap) {
+    sync();
+    dfree(p);
+    p = dalloc<float>(n);
+    cap = n;
+  }
+  return p;
+}
+
+// batch loss from the per-workgroup partials of k_loss
+double read_loss(long npix, int B) {
+  hipStream_t st = stream();
+  const unsigned nb = g1d(npix);
+  std::vector<float> h(nb);
+  CK(hipMemcpyAsync(h.data(), N.lpart, nb * sizeof(float), hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  double s = 0;
+  for (float v : h) s += v;
+  return s / (double)B;
+}
+
+void adam_step(float lr) {
+  N.step++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)N.step), c2 = 1.f - powf(b2, (float)N.step);
+  hipLaunchKernelGGL(k_adam, dim3(g1d(N.nparam)), dim3(256), 0, stream(), N.p, N.g, N.m1, N.m2, nullptr,
+                     N.nparam, lr, b1, b2, 1e-8f, c1, c2);
+  for (int l = 0; l < NL; l++) {
+    const long nw = 9L * SPEC[l].cout * cin_of(l);
+    hipLaunchKernelGGL(k_to_bf16, dim3(g1d(nw)), dim3(256), 0, stream(), N.p + N.woff[l], N.wb + N.woff[l], nw);
+  }
+}
+
+void refresh_bf16() {
+  for (int l = 0; l < NL; l++) {
+    const long nw = 9L * SPEC[l].cout * cin_of(l);
+    hipLaunchKernelGGL(k_to_bf16, dim3(g1d(nw)), dim3(256), 0, stream(), N.p + N.woff[l], N.wb + N.woff[l], nw);
+  }
+}
+
+template <typename F> int guarded(const char *what, F f) {
+  try {
+    f();
+    return 0;
+  } catch (std::exception &e) {
+    fprintf(stderr, "%s: %s\n", what, e.what());
+    return 1;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- TeaLeaf states as samples
+//
+// A TeaLeaf solve finds u with A u = u0, A = I + rx Dx(Kx) + ry Dy(Ky) (Kx, Ky from the
+// density), starting from u = u0 with residual r0 = u0 - A u0. The change it makes is
+// delta = u - u0 = A^-1 r0. The network sees, per pixel,
+//   channel 0: (log10(density) - 0.5) / 1.5
+//   channel 1: energy / max(energy)          (energy = u0 / density)
+//   channel 2: u0 / max(u0)
+//   channel 3: r0 / rms(r0)
+// and predicts delta / rms(r0): the problem is linear in (u0, r0), so rms(r0) carries the
+// magnitude and the network only the shape. A mesh of n x n cells is seen through f x f
+// averages (n = f x network size; the statistics are taken on the averages); the
+// prediction returns to the mesh by bilinear interpolation between coarse cell centres.
+
+namespace {
+
+struct Mesh {
+  const double *rho, *u0, *r;  // dat base pointers, element (x, y) at (x + 1) + (y + 1) * stride
+  int n, stride, f;            // n x n cells, viewed as (n / f)^2 network pixels
+};
+
+__device__ __forceinline__ long cell_at(const Mesh &m, int x, int y) { return (x + 1) + (long)(y + 1) * m.stride; }
+
+// f x f averages of the four raw features: raw[pixel][4]
+__global__ __launch_bounds__(256) void k_raw(Mesh m, float *raw) {
+  const int nc = m.n / m.f;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)nc * nc) return;
+  const int X = (int)(i % nc), Y = (int)(i / nc);
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
+  for (int dy = 0; dy < m.f; dy++)
+    for (int dx = 0; dx < m.f; dx++) {
+      const long c = cell_at(m, X * m.f + dx, Y * m.f + dy);
+      const double rho = m.rho[c], u = m.u0[c];
+      f0 += (log10(rho) - 0.5) / 1.5;
+      f1 += u / rho;
+      f2 += u;
+      f3 += m.r[c];
+    }
+  const double w = 1.0 / ((double)m.f * m.f);
+  raw[4 * i] = (float)(f0 * w);
+  raw[4 * i + 1] = (float)(f1 * w);
+  raw[4 * i + 2] = (float)(f2 * w);
+  raw[4 * i + 3] = (float)(f3 * w);
+}
+
+constexpr int STAT_BLOCKS = 512;
+
+// per-workgroup max |energy|, max |u0|, sum r0^2 over the raw features
+__global__ __launch_bounds__(256) void k_stats_part(const float *raw, long npix, double *part) {
+  __shared__ double lds[4];
+  double a = 0.0, b = 0.0, q = 0.0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < npix; i += (long)gridDim.x * 256) {
+    a = fmax(a, fabs((double)raw[4 * i + 1]));
+    b = fmax(b, fabs((double)raw[4 * i + 2]));
+    q += (double)raw[4 * i + 3] * raw[4 * i + 3];
+  }
+  a = ops_block_reduce<ops_red_max<double>>(a, lds);
+  b = ops_block_reduce<ops_red_max<double>>(b, lds);
+  q = ops_block_reduce<ops_red_inc<double>>(q, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = a;
+    part[3 * blockIdx.x + 1] = b;
+    part[3 * blockIdx.x + 2] = q;
+  }
+}
+
+// stats = {max energy, max u0, rms r0}, each 1 where it would be zero
+__global__ __launch_bounds__(256) void k_stats_fin(const double *part, int nb, long npix, double *stats) {
+  __shared__ double lds[4];
+  double a = 0.0, b = 0.0, q = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    a = fmax(a, part[3 * i]);
+    b = fmax(b, part[3 * i + 1]);
+    q += part[3 * i + 2];
+  }
+  a = ops_block_reduce<ops_red_max<double>>(a, lds);
+  b = ops_block_reduce<ops_red_max<double>>(b, lds);
+  q = ops_block_reduce<ops_red_inc<double>>(q, lds);
+  if (threadIdx.x == 0) {
+    const double rms = sqrt(q / (double)npix);
+    stats[0] = a > 0.0 ? a : 1.0;
+    stats[1] = b > 0.0 ? b : 1.0;
+    stats[2] = rms > 0.0 ? rms : 1.0;
+  }
+}
+
+// normalised network input: 4 features, zeros up to `cs` channels
+__global__ __launch_bounds__(256) void k_pack(const float *raw, long npix, const double *stats, int cs, bf16 *dst) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= npix) return;
+  bf16 *o = dst + i * cs;
+  o[0] = f2bf(raw[4 * i]);
+  o[1] = f2bf((float)(raw[4 * i + 1] / stats[0]));
+  o[2] = f2bf((float)(raw[4 * i + 2] / stats[1]));
+  o[3] = f2bf((float)(raw[4 * i + 3] / stats[2]));
+  for (int c = 4; c < cs; c++) o[c] = 0;
+}
+
+// training target u - u0 (brought to unit rms afterwards: the warm start fits the scale
+// itself) and the operator's face coefficients {rx Kx, ry Ky} (the low-side faces of each
+// cell), on a mesh seen at its own size (f = 1)
+__global__ __launch_bounds__(256) void k_target(Mesh m, const double *u, const double *kx, const double *ky,
+                                                double rx, double ry, float *dst, float *coef) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)m.n * m.n) return;
+  const long c = cell_at(m, (int)(i % m.n), (int)(i / m.n));
+  dst[i] = (float)(u[c] - m.u0[c]);
+  coef[2 * i] = (float)(rx * kx[c]);
+  coef[2 * i + 1] = (float)(ry * ky[c]);
+}
+
+// per-workgroup {t^T t, t^T C t} of one sample (C as in k_loss)
+__global__ __launch_bounds__(256) void k_tnorm_part(const float *t, const float *coef, int H, int W, double *part) {
+  __shared__ double lds[4];
+  const long npix = (long)H * W;
+  double v0 = 0.0, v1 = 0.0;
+  for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < npix; p += (long)gridDim.x * 256) {
+    const int x = (int)(p % W), y = (int)(p / W);
+    const double e = t[p];
+    double ce = 0.0;
+    if (x > 0) ce += coef[2 * p] * (e - t[p - 1]);
+    if (x + 1 < W) ce += coef[2 * (p + 1)] * (e - t[p + 1]);
+    if (y > 0) ce += coef[2 * p + 1] * (e - t[p - W]);
+    if (y + 1 < H) ce += coef[2 * (p + W) + 1] * (e - t[p + W]);
+    v0 += e * e;
+    v1 += e * ce;
+  }
+  v0 = ops_block_reduce<ops_red_inc<double>>(v0, lds);
+  v1 = ops_block_reduce<ops_red_inc<double>>(v1, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = v0;
+    part[2 * blockIdx.x + 1] = v1;
+  }
+}
+
+// norm = the summed partials (t^T t = 1 for a zero target)
+__global__ __launch_bounds__(256) void k_tnorm_fin(const double *part, int nb, double *norm) {
+  __shared__ double lds[4];
+  double v0 = 0.0, v1 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    v0 += part[2 * i];
+    v1 += part[2 * i + 1];
+  }
+  v0 = ops_block_reduce<ops_red_inc<double>>(v0, lds);
+  v1 = ops_block_reduce<ops_red_inc<double>>(v1, lds);
+  if (threadIdx.x == 0) {
+    norm[0] = v0 > 0.0 ? v0 : 1.0;
+    norm[1] = v1;
+  }
+}
+
+// t /= rms(t), rms from norm[0] = t^T t
+__global__ __launch_bounds__(256) void k_unit_rms(float *t, long npix, const double *norm) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < npix) t[i] = (float)(t[i] / sqrt(norm[0] / (double)npix));
+}
+
+// u = u0 + s * bilinear(prediction) written to pf (the (n + 2)^2 layout of the mesh, zero
+// halo): the network's change on every cell
+__global__ __launch_bounds__(256) void k_apply(Mesh m, const float *pred, int ps, const double *stats, double *pf) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)m.n * m.n) return;
+  const int x = (int)(i % m.n), y = (int)(i / m.n), nc = m.n / m.f;
+  const double cx = (x + 0.5) / m.f - 0.5, cy = (y + 0.5) / m.f - 0.5;
+  const int x0 = (int)floor(cx), y0 = (int)floor(cy);
+  const double wx = cx - x0, wy = cy - y0;
+  const int xa = max(x0, 0), xb = min(x0 + 1, nc - 1), ya = max(y0, 0), yb = min(y0 + 1, nc - 1);
+  const double v = (1.0 - wy) * ((1.0 - wx) * pred[((long)ya * nc + xa) * ps] + wx * pred[((long)ya * nc + xb) * ps]) +
+                   wy * ((1.0 - wx) * pred[((long)yb * nc + xa) * ps] + wx * pred[((long)yb * nc + xb) * ps]);
+  pf[cell_at(m, x, y)] = stats[2] * v;
+}
+
+// Galerkin correction from the network's change p: u = u0 + alpha (p - m), m the mean of p
+// and alpha = (p - m)^T r0 / (p - m)^T A (p - m), the energy-norm optimum over the span of
+// p and the constant field (the constant's own coefficient is zero: A 1 = 1 with
+// zero-conductance boundary faces, and r0 sums to zero). The plain start is alpha = 0, so
+// the energy-norm error never grows. A is the solve's operator, the 5-point form of
+// tea_leaf_cg_calc_w_reduce_kernel. Partials {sum p, sum r0, p^T r0, p^T A p}.
+__global__ __launch_bounds__(256) void k_galerkin_part(Mesh m, const double *pf, const double *kx, const double *ky,
+                                                       double rx, double ry, double *part) {
+  __shared__ double lds[4];
+  double sp = 0.0, sr = 0.0, pr = 0.0, pap = 0.0;
+  const long nn = (long)m.n * m.n;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < nn; i += (long)gridDim.x * 256) {
+    const long c = cell_at(m, (int)(i % m.n), (int)(i / m.n)), s = m.stride;
+    const double p = pf[c], r = m.r[c];
+    const double ap = (1.0 + ry * (ky[c + s] + ky[c]) + rx * (kx[c + 1] + kx[c])) * p -
+                      ry * (ky[c + s] * pf[c + s] + ky[c] * pf[c - s]) - rx * (kx[c + 1] * pf[c + 1] + kx[c] * pf[c - 1]);
+    sp += p;
+    sr += r;
+    pr += p * r;
+    pap += p * ap;
+  }
+  sp = ops_block_reduce<ops_red_inc<double>>(sp, lds);
+  sr = ops_block_reduce<ops_red_inc<double>>(sr, lds);
+  pr = ops_block_reduce<ops_red_inc<double>>(pr, lds);
+  pap = ops_block_reduce<ops_red_inc<double>>(pap, lds);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = sp;
+    part[4 * blockIdx.x + 1] = sr;
+    part[4 * blockIdx.x + 2] = pr;
+    part[4 * blockIdx.x + 3] = pap;
+  }
+}
+
+// stats[3] = alpha, stats[4] = m, stats[5] = the drop in the squared energy-norm error
+// against the plain start, ((p - m)^T r0)^2 / (p - m)^T A (p - m)
+__global__ __launch_bounds__(256) void k_galerkin_fin(const double *part, int nb, long ncell, double *stats) {
+  __shared__ double lds[4];
+  double sp = 0.0, sr = 0.0, pr = 0.0, pap = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    sp += part[4 * i];
+    sr += part[4 * i + 1];
+    pr += part[4 * i + 2];
+    pap += part[4 * i + 3];
+  }
+  sp = ops_block_reduce<ops_red_inc<double>>(sp, lds);
+  sr = ops_block_reduce<ops_red_inc<double>>(sr, lds);
+  pr = ops_block_reduce<ops_red_inc<double>>(pr, lds);
+  pap = ops_block_reduce<ops_red_inc<double>>(pap, lds);
+  if (threadIdx.x == 0) {
+    const double m = sp / (double)ncell;
+    const double num = pr - m * sr, den = pap - m * m * (double)ncell;
+    stats[3] = den > 0.0 ? num / den : 0.0;
+    stats[4] = m;
+    stats[5] = den > 0.0 ? num * num / den : 0.0;
+  }
+}
+
+// u = u0 + alpha (p - m) on every cell
+__global__ __launch_bounds__(256) void k_update(Mesh m, const double *pf, const double *stats, double *u) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)m.n * m.n) return;
+  const long c = cell_at(m, (int)(i % m.n), (int)(i / m.n));
+  u[c] = m.u0[c] + stats[3] * (pf[c] - stats[4]);
+}
+
+struct SampleScratch {
+  double *part = nullptr;
+  double *stats = nullptr;  // guess statistics {max energy, max u0, rms r0, alpha, mean of p, energy drop}
+  float *raw = nullptr;
+  long raw_cap = 0;
+  double *pf = nullptr;     // the network's change on the mesh, (n + 2)^2, zero halo
+  long pf_cap = 0;
+};
+SampleScratch SS;
+
+void scratch_init() {
+  if (!SS.part) {
+    SS.part = dalloc<double>(4 * STAT_BLOCKS);
+    SS.stats = dalloc<double>(6);
+  }
+}
+
+// norm[index] = {t^T t, t^T C t} of dataset sample `index`
+void sample_norm(int index) {
+  const long npix = (long)D.H * D.W;
+  scratch_init();
+  hipLaunchKernelGGL(k_tnorm_part, dim3(STAT_BLOCKS), dim3(256), 0, stream(), D.target + index * npix,
+                     D.coef + index * npix * 2, D.H, D.W, SS.part);
+  hipLaunchKernelGGL(k_tnorm_fin, dim3(1), dim3(256), 0, stream(), SS.part, STAT_BLOCKS, D.norm + 2L * index);
+}
+
+const double *dev_of(ops_dat d) {
+  ops_memspace ms = OPS_DEVICE;
+  ops_dat_get_raw_pointer(d, 0, nullptr, &ms);  // uploads a host-newer copy
+  ops_dat_release_raw_data(d, 0, OPS_READ);
+  return (const double *)d->data_d;
+}
+
+void check_field(ops_dat d, int n) {
+  if (d->size[0] != n + 2 || d->size[1] != n + 2 || d->elem_size != 8 || d->x_pad)
+    throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: expected n x n double fields with one halo layer");
+}
+
+// network input of a mesh (stats written to `stats`, 3 doubles)
+Mesh features(ops_dat rho, ops_dat u0, ops_dat r, int n, int net, double *stats, int cs, bf16 *dst) {
+  if (net <= 0 || net % 8 || n % net)
+    throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: mesh size must be a multiple of the network size (a multiple of 8)");
+  for (ops_dat d : {rho, u0, r}) check_field(d, n);
+  const long npix = (long)net * net;
+  scratch_init();
+  if (npix * 4 > SS.raw_cap) {
+    sync();
+    dfree(SS.raw);
+    SS.raw = dalloc<float>(npix * 4);
+    SS.raw_cap = npix * 4;
+  }
+  Mesh m{dev_of(rho), dev_of(u0), dev_of(r), n, n + 2, n / net};
+  hipStream_t st = stream();
+  hipLaunchKernelGGL(k_raw, dim3(g1d(npix)), dim3(256), 0, st, m, SS.raw);
+  hipLaunchKernelGGL(k_stats_part, dim3(STAT_BLOCKS), dim3(256), 0, st, SS.raw, npix, SS.part);
+  hipLaunchKernelGGL(k_stats_fin, dim3(1), dim3(256), 0, st, SS.part, STAT_BLOCKS, npix, stats);
+  hipLaunchKernelGGL(k_pack, dim3(g1d(npix)), dim3(256), 0, st, SS.raw, npix, stats, cs, dst);
+  return m;
+}
+
+}  // namespace
+
+void tea_unet_capture_features(int index, ops_dat rho, ops_dat u0, ops_dat r, int n) {
+  if (index < 0 || index >= D.count) throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: dataset index");
+  const long npix = (long)D.H * D.W;
+  features(rho, u0, r, n, D.H, D.stats + 3L * index, 4, D.in + index * npix * 4);
+  ops_hip_check_launch("tea_unet capture (features)");
+}
+
+void tea_unet_capture_target(int index, ops_dat u0, ops_dat u, ops_dat Kx, ops_dat Ky, double rx, double ry, int n) {
+  if (index < 0 || index >= D.count) throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: dataset index");
+  if (n != D.H || n != D.W) throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: training samples are captured at the dataset size");
+  for (ops_dat d : {u0, u, Kx, Ky}) check_field(d, n);
+  const Mesh m{nullptr, dev_of(u0), nullptr, n, n + 2, 1};
+  const long npix = (long)D.H * D.W;
+  hipLaunchKernelGGL(k_target, dim3(g1d(npix)), dim3(256), 0, stream(), m, dev_of(u), dev_of(Kx), dev_of(Ky), rx, ry,
+                     D.target + index * npix, D.coef + index * npix * 2);
+  sample_norm(index);
+  hipLaunchKernelGGL(k_unit_rms, dim3(g1d(npix)), dim3(256), 0, stream(), D.target + index * npix, npix,
+                     D.norm + 2L * index);
+  sample_norm(index);
+  ops_hip_check_launch("tea_unet capture (target)");
+}
+
+void tea_unet_guess_dats(ops_dat rho, ops_dat u0, ops_dat r, ops_dat Kx, ops_dat Ky, double rx, double ry, ops_dat u,
+                         int n, int net_cells, double *out2) {
+  if (!N.live) throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: no network (tea_unet_init)");
+  if (net_cells <= 0 || net_cells % 8 || n % net_cells)
+    throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: mesh size must be a multiple of the network size (a multiple of 8)");
+  for (ops_dat d : {u, Kx, Ky}) check_field(d, n);
+  ensure_acts(1, net_cells, net_cells);
+  scratch_init();
+  const long cells = (long)(n + 2) * (n + 2);
+  if (cells > SS.pf_cap) {
+    sync();
+    dfree(SS.pf);
+    SS.pf = dalloc<double>(cells);  // zeroed: the halo stays zero
+    SS.pf_cap = cells;
+  }
+  const Mesh m = features(rho, u0, r, n, net_cells, SS.stats, 16, N.in);
+  forward();
+  hipStream_t st = stream();
+  const double *kx = dev_of(Kx), *ky = dev_of(Ky);
+  hipLaunchKernelGGL(k_apply, dim3(g1d((long)n * n)), dim3(256), 0, st, m, N.out32, 16, SS.stats, SS.pf);
+  hipLaunchKernelGGL(k_galerkin_part, dim3(STAT_BLOCKS), dim3(256), 0, st, m, SS.pf, kx, ky, rx, ry, SS.part);
+  hipLaunchKernelGGL(k_galerkin_fin, dim3(1), dim3(256), 0, st, SS.part, STAT_BLOCKS, (long)n * n, SS.stats);
+  ops_memspace ms = OPS_DEVICE;
+  ops_dat_get_raw_pointer(u, 0, nullptr, &ms);
+  hipLaunchKernelGGL(k_update, dim3(g1d((long)n * n)), dim3(256), 0, st, m, SS.pf, SS.stats, (double *)u->data_d);
+  ops_dat_release_raw_data(u, 0, OPS_WRITE);
+  ops_hip_check_launch("tea_unet warm start");
+  if (out2) {
+    double h[6];
+    CK(hipMemcpy(h, SS.stats, sizeof h, hipMemcpyDeviceToHost));
+    out2[0] = h[3];
+    out2[1] = h[5];
+  }
+}
+
+extern "C" int tea_unet_init(unsigned long long seed) {
+  return guarded("tea_unet_init", [&] {
+    if (!N.live) {
+      long off = 0;
+      long maxw = 0;
+      for (int l = 0; l < NL; l++) {
+        N.woff[l] = off;
+        off += 9L * SPEC[l].cout * cin_of(l);
+        N.boff[l] = off;
+        off += SPEC[l].cout;
+        maxw = std::max(maxw, 9L * SPEC[l].cout * cin_of(l));
+      }
+      N.nparam = off;
+      N.p = dalloc<float>(off);
+      N.g = dalloc<float>(off);
+      N.m1 = dalloc<float>(off);
+      N.m2 = dalloc<float>(off);
+      N.wb = dalloc<bf16>(off);
+      N.wt = dalloc<bf16>(maxw);
+      N.live = true;
+    }
+    // He initialisation from splitmix64 (uniform with the normal's variance), zero biases
+    std::vector<float> h(N.nparam, 0.f);
+    uint64_t s = seed;
+    for (int l = 0; l < NL; l++) {
+      const long nw = 9L * SPEC[l].cout * cin_of(l);
+      const int fan_in = 9 * (l == 0 ? 4 : cin_of(l));
+      const float a = sqrtf(6.f / (float)fan_in) * (l == NL - 1 ? 0.1f : 1.f);
+      for (long i = 0; i < nw; i++) {
+        const double u = (double)(mix(s) >> 11) * (1.0 / 9007199254740992.0);
+        float v = (float)((2.0 * u - 1.0) * a);
+        if (l == 0 && (i % cin_of(l)) >= 4) v = 0.f;  // padded input channels
+        h[N.woff[l] + i] = v;
+      }
+    }
+    CK(hipMemcpy(N.p, h.data(), N.nparam * sizeof(float), hipMemcpyHostToDevice));
+    CK(hipMemset(N.m1, 0, N.nparam * sizeof(float)));
+    CK(hipMemset(N.m2, 0, N.nparam * sizeof(float)));
+    N.step = 0;
+    refresh_bf16();
+    sync();
+  });
+}
+
+extern "C" long tea_unet_num_params(void) { return N.nparam; }
+
+extern "C" int tea_unet_get_params(float *host) {
+  return guarded("tea_unet_get_params", [&] {
+    if (!N.live) throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: call tea_unet_init first");
+    sync();
+    CK(hipMemcpy(host, N.p, N.nparam * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int tea_unet_set_params(const float *host) {
+  return guarded("tea_unet_set_params", [&] {
+    if (!N.live) throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: call tea_unet_init first");
+    sync();
+    CK(hipMemcpy(N.p, host, N.nparam * sizeof(float), hipMemcpyHostToDevice));
+    refresh_bf16();
+    sync();
+  });
+}
+
+// in: [B][H][W][16] FP32 (rounded to BF16), out: [B][H][W] prediction (channel 0)
+extern "C" int tea_unet_forward_host(int B, int H, int W, const float *in, float *out) {
+  return guarded("tea_unet_forward_host", [&] {
+    ensure_acts(B, H, W);
+    const long npix = (long)B * H * W;
+    float *tmp = scratch_f(npix * 16);
+    CK(hipMemcpy(tmp, in, npix * 16 * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_to_bf16, dim3(g1d(npix * 16)), dim3(256), 0, stream(), tmp, N.in, npix * 16);
+    forward();
+    std::vector<float> o(npix * 16);
+    sync();
+    CK(hipMemcpy(o.data(), N.out32, npix * 16 * sizeof(float), hipMemcpyDeviceToHost));
+    for (long p = 0; p < npix; p++) out[p] = o[p * 16];
+  });
+}
+
+// gradients of every parameter (FP32, the get_params order) for d loss / d prediction =
+// grad_out ([B][H][W]); runs the forward pass on `in` first
+extern "C" int tea_unet_backward_host(int B, int H, int W, const float *in, const float *grad_out,
+                                      float *gparams) {
+  return guarded("tea_unet_backward_host", [&] {
+    ensure_acts(B, H, W);
+    const long npix = (long)B * H * W;
+    float *tmp = scratch_f(npix * 16);
+    CK(hipMemcpy(tmp, in, npix * 16 * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_to_bf16, dim3(g1d(npix * 16)), dim3(256), 0, stream(), tmp, N.in, npix * 16);
+    forward();
+    std::vector<float> g(npix * 16, 0.f);
+    for (long p = 0; p < npix; p++) g[p * 16] = grad_out[p];
+    sync();
+    CK(hipMemcpy(N.grad[NL - 1], g.data(), npix * 16 * sizeof(float), hipMemcpyHostToDevice));
+    backward();
+    sync();
+    CK(hipMemcpy(gparams, N.g, N.nparam * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int tea_unet_dataset_reserve(int count, int H, int W) {
+  return guarded("tea_unet_dataset_reserve", [&] {
+    if (count <= 0 || H <= 0 || W <= 0 || H % 8 || W % 8)
+      throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: dataset count > 0, H and W positive multiples of 8");
+    sync();
+    dfree(D.in);
+    dfree(D.target);
+    D.count = count;
+    D.H = H;
+    D.W = W;
+    dfree(D.stats);
+    dfree(D.coef);
+    dfree(D.norm);
+    D.in = dalloc<bf16>((size_t)count * H * W * 4);
+    D.target = dalloc<float>((size_t)count * H * W);
+    D.coef = dalloc<float>((size_t)count * H * W * 2);
+    D.stats = dalloc<double>((size_t)count * 3);
+    D.norm = dalloc<double>((size_t)count * 2);
+  });
+}
+
+// in4: [H][W][4] features, target: [H][W] (host FP32)
+extern "C" int tea_unet_dataset_set(int index, const float *in4, const float *target) {
+  return guarded("tea_unet_dataset_set", [&] {
+    if (index < 0 || index >= D.count) throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: dataset index");
+    const long npix = (long)D.H * D.W;
+    float *tmp = scratch_f(npix * 4);
+    CK(hipMemcpy(tmp, in4, npix * 4 * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_pack_in, dim3(g1d(npix * 4)), dim3(256), 0, stream(), tmp, npix, D.in + index * npix * 4);
+    CK(hipMemcpy(D.target + index * npix, target, npix * sizeof(float), hipMemcpyHostToDevice));
+    CK(hipMemsetAsync(D.coef + index * npix * 2, 0, npix * 2 * sizeof(float), stream()));
+    sample_norm(index);
+    sync();
+  });
+}
+
+// Adam over the dataset in batches, sample order shuffled by splitmix64(seed) each epoch;
+// loss_per_epoch[e] = mean over the epoch's batches of the batch loss (k_loss)
+extern "C" int tea_unet_train(int epochs, int batch, float lr, float beta, unsigned long long seed,
+                              float *loss_per_epoch) {
+  return guarded("tea_unet_train", [&] {
+    if (D.count <= 0) throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet: empty dataset");
+    if (batch < 1 || batch > D.count) batch = D.count < 4 ? D.count : 4;
+    ensure_acts(batch, D.H, D.W);
+    const long spix = (long)D.H * D.W, npix = spix * batch;
+    std::vector<int> order(D.count);
+    uint64_t s = seed;
+    hipStream_t st = stream();
+    for (int e = 0; e < epochs; e++) {
+      for (int i = 0; i < D.count; i++) order[i] = i;
+      for (int i = D.count - 1; i > 0; i--) std::swap(order[i], order[(int)(mix(s) % (uint64_t)(i + 1))]);
+      double lsum = 0;
+      int nb = 0;
+      for (int i0 = 0; i0 + batch <= D.count; i0 += batch) {
+        for (int k = 0; k < batch; k++) {
+          const int idx = order[i0 + k];
+          hipLaunchKernelGGL(k_expand, dim3(g1d(spix * 16)), dim3(256), 0, st, D.in + (long)idx * spix * 4, spix,
+                             N.in + k * spix * 16);
+          CK(hipMemcpyAsync(N.target + k * spix, D.target + (long)idx * spix, spix * sizeof(float),
+                            hipMemcpyDeviceToDevice, st));
+          CK(hipMemcpyAsync(N.coef + k * spix * 2, D.coef + (long)idx * spix * 2, spix * 2 * sizeof(float),
+                            hipMemcpyDeviceToDevice, st));
+          CK(hipMemcpyAsync(N.norm + 2 * k, D.norm + 2L * idx, 2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        forward();
+        hipLaunchKernelGGL(k_loss, dim3(g1d(npix)), dim3(256), 0, st, N.out32, N.target, N.coef, N.norm, beta, batch, D.H,
+                           D.W, 16, N.grad[NL - 1], N.lpart);
+        backward();
+        adam_step(lr);
+        lsum += read_loss(npix, batch);
+        nb++;
+      }
+      if (loss_per_epoch) loss_per_epoch[e] = (float)(lsum / (nb > 0 ? nb : 1));
+    }
+    sync();
+  });
+}
+
+extern "C" int tea_unet_free(void) {
+  return guarded("tea_unet_free", [&] {
+    sync();
+    free_acts();
+    dfree(D.in);
+    dfree(D.target);
+    dfree(D.stats);
+    dfree(D.coef);
+    dfree(D.norm);
+    D.count = 0;
+    dfree(N.p);
+    dfree(N.g);
+    dfree(N.m1);
+    dfree(N.m2);
+    dfree(N.wb);
+    dfree(N.wt);
+    dfree(N.wpart);
+    N.wpart_cap = 0;
+    N.live = false;
+  });
+}

@@ -45,11 +45,16 @@ struct Tea {
   int end_step = 640000, max_iters = 1000, coefficient = CONDUCTIVITY, use_cg = 0;
   int reflective = 0, summary_frequency = 10, check_result = 0, test_problem = 0;
   int solver = 0;
+  int guess = 0, guess_cells = 0;  // learned warm start mode / network size
+  int capture = -1;                // dataset sample the next solve is stored as, -1 none
   int step = 0;
   double time = 0.0, dt = 0.0;
   // last solve
   int last_iters = 0;
   double last_error = 0, last_initial = 0, last_exact = -1;
+  double last_warm = -1;  // residual of the learned warm start, -1 without one
+  double last_alpha[2] = {0, 0};  // its Galerkin scaling, energy-norm error drop
+  int last_guess_used = 0;
   long total_iters = 0;
   ops_block blk;
   ops_dat density, energy0, energy1, u, u0, r, w, Kx, Ky, p, z, volume, xarea, yarea;
@@ -385,6 +390,30 @@ void solve() {
   calc_residual(rx, ry);
   double initial = norm2(T.r);
   initial = sqrt(fabs(initial));
+  T.last_warm = -1;
+  T.last_guess_used = 0;
+  if (T.capture >= 0) tea_unet_capture_features(T.capture, T.density, T.u0, T.r, grid.x_cells);
+  if (T.guess) {
+    // learned warm start: u = u0 + alpha * the network's predicted change, alpha the Galerkin
+    // scaling (energy-norm optimal along that direction). The stop threshold keeps
+    // the residual of the standard start u = u0. Mode 1 keeps the warm start only when its
+    // own residual is below that one, otherwise u = u0 is restored (the plain start); mode
+    // 2 always keeps it.
+    tea_unet_guess_dats(T.density, T.u0, T.r, T.Kx, T.Ky, rx, ry, T.u, grid.x_cells, T.guess_cells, T.last_alpha);
+    set_fields(f, {FIELD_U});
+    update_halo(f, 1);
+    calc_residual(rx, ry);
+    T.last_warm = sqrt(fabs(norm2(T.r)));
+    if (T.guess == 2 || T.last_warm < initial) {
+      T.last_guess_used = 1;
+    } else {
+      ops_par_loop(tea_copy, "tea_leaf_warm_start_reset_kernel", T.blk, 2, in,
+                   ops_arg_dat(T.u0, 1, T.S00, "double", OPS_READ),
+                   ops_arg_dat(T.u, 1, T.S00, "double", OPS_WRITE));
+      update_halo(f, 1);
+      calc_residual(rx, ry);
+    }
+  }
   double error = 0.0;
   int n = 1;
   if (T.use_cg && T.solver != 0) {
@@ -479,6 +508,10 @@ void solve() {
       if (fabs(error) < T.eps * initial) break;
     }
   }
+  if (T.capture >= 0) {
+    tea_unet_capture_target(T.capture, T.u0, T.u, T.Kx, T.Ky, rx, ry, grid.x_cells);
+    T.capture = -1;
+  }
   T.last_iters = n > T.max_iters ? T.max_iters : n;
   T.last_error = error;
   T.last_initial = initial;
@@ -558,6 +591,36 @@ extern "C" int tealeaf_init(int argc, const char **argv, const char *deck) {
 extern "C" int tealeaf_set_solver(int solver) {
   if (solver < 0 || solver > 3) return 1;
   T.solver = solver;
+  return 0;
+}
+
+// learned warm start for the implicit solve: modes 1 and 2 evaluate the network
+// (tea_unet_*) on a net_cells^2 view of the mesh before each solve (1: kept only when its
+// residual beats the standard start's, 2: always kept), mode 0 turns it off
+extern "C" int tealeaf_set_guess(int mode, int net_cells) {
+  if (mode < 0 || mode > 2 || (mode && net_cells <= 0)) return 1;
+  T.guess = mode;
+  T.guess_cells = net_cells;
+  return 0;
+}
+
+// store the next solve as training sample `index` of the network's dataset (features of
+// density, u0 and the starting residual; target u)
+extern "C" int tealeaf_capture_next(int index) {
+  if (index < 0) return 1;
+  T.capture = index;
+  return 0;
+}
+
+// last solve: residual of the standard start, residual of the learned warm start (-1 when
+// off), 1 if the warm start was kept, its Galerkin scaling, and the drop in the squared
+// energy-norm error it makes against the standard start
+extern "C" int tealeaf_guess_state(double *out5) {
+  out5[0] = T.last_initial;
+  out5[1] = T.last_warm;
+  out5[2] = T.last_guess_used;
+  out5[3] = T.last_alpha[0];
+  out5[4] = T.last_alpha[1];
   return 0;
 }
 

@@ -45,4 +45,17 @@ void tea_native_cg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_d
 void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat Kx, ops_dat Ky,
                      const tea_native_solve_params *P, tea_native_result *R);
 
+// Learned warm start (the U-Net of tealeaf_unet.h). capture_features stores the network
+// input of the solve about to start (density, u0 and its residual r = u0 - A u0) as
+// dataset sample `index`; capture_target stores its result u and the operator's face
+// coefficients (rx Kx, ry Ky). guess overwrites the interior of u with u0 + alpha p, p the
+// network's predicted change (less its mean) evaluated on a net_cells^2 view of the n x n
+// mesh and alpha the Galerkin scaling p^T r / p^T A p; out2 (when non-null) receives alpha
+// and the drop in the squared energy-norm error against the plain start. All need n x n
+// double fields with one halo layer.
+void tea_unet_capture_features(int index, ops_dat rho, ops_dat u0, ops_dat r, int n);
+void tea_unet_capture_target(int index, ops_dat u0, ops_dat u, ops_dat Kx, ops_dat Ky, double rx, double ry, int n);
+void tea_unet_guess_dats(ops_dat rho, ops_dat u0, ops_dat r, ops_dat Kx, ops_dat Ky, double rx, double ry, ops_dat u,
+                         int n, int net_cells, double *out2);
+
 #endif

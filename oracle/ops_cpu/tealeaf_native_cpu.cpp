@@ -348,3 +348,14 @@ void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops
   R->outer = (int)lv.size();
   R->error = err;
 }
+
+// The learned warm start runs only on the device build; the host build rejects it.
+void tea_unet_capture_features(int, ops_dat, ops_dat, ops_dat, int) {
+  throw OPSException(OPS_NOT_IMPLEMENTED, "tealeaf learned warm start: device build only");
+}
+void tea_unet_capture_target(int, ops_dat, ops_dat, ops_dat, ops_dat, double, double, int) {
+  throw OPSException(OPS_NOT_IMPLEMENTED, "tealeaf learned warm start: device build only");
+}
+void tea_unet_guess_dats(ops_dat, ops_dat, ops_dat, ops_dat, ops_dat, double, double, ops_dat, int, int, double *) {
+  throw OPSException(OPS_NOT_IMPLEMENTED, "tealeaf learned warm start: device build only");
+}

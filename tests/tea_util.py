@@ -20,6 +20,8 @@ def lib(kind):
         _cpu = ctypes.CDLL(str(ROOT / "oracle" / "_build" / "libtealeaf_cpu.so"),
                            mode=ctypes.RTLD_LOCAL)
         for fn, (res, args) in SIGNATURES["tealeaf"].items():
+            if fn.startswith("tea_unet_"):  # the network lives in the device build only
+                continue
             f = getattr(_cpu, fn)
             f.restype = res
             f.argtypes = args
