@@ -51,6 +51,9 @@ typedef struct ops_hip_loop {
   int kernel_id;
   int timed;
   int tune_cand; /* autotuning candidate measured by this launch, -1 otherwise */
+  int alias;     /* 1 if one dat is passed to two arguments and at least one writes it: the
+                    generated stub then launches the kernel variant without no-alias
+                    pointer parameters */
   double bytes;  /* ops_compute_transfer bytes of this launch */
 } ops_hip_loop;
 

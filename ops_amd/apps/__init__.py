@@ -48,6 +48,7 @@ SIGNATURES = {
         "api_halo3d": (_i, [_i, _pp, _p, _p, _p, _i, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p]),
         "api_tridiag": (_i, [_i, _pp, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
         "api_reduce": (_i, [_i, _pp, _i, _i, _p, _p, _p]),
+        "api_alias": (_i, [_i, _pp, _i, _i, _p, _p]),
         "api_random": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
     },
     "adi": {

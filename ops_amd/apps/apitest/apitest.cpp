@@ -223,6 +223,33 @@ extern "C" int api_tridiag(int argc, const char **argv, int ndim, int solvedim, 
 // runs twice over the two halves (rows [0, ny/2) and [ny/2, ny)) before one
 // ops_reduction_result per handle. out: sum, min, max, isum, imin, imax (as doubles),
 // pair[0], pair[1].
+// out[0 .. nx*ny): v after v = 2 v + 1 with v passed as both the read and the written
+// argument; out[nx*ny ..): u = 2 v + 1 into a second dat
+extern "C" int api_alias(int argc, const char **argv, int nx, int ny, const double *vals, double *out) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block blk = ops_decl_block(2, "alias");
+    int size[2] = {nx, ny}, base[2] = {0, 0}, dm[2] = {0, 0}, dp[2] = {0, 0};
+    double *none = nullptr;
+    ops_dat V = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "v");
+    ops_dat U = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "u");
+    int full[4] = {0, nx, 0, ny};
+    ops_dat_set_data_slab_memspace(V, 0, (char *)vals, full, OPS_HOST);
+    int s00[] = {0, 0};
+    ops_stencil S00 = ops_decl_stencil(2, 1, s00, "00");
+    ops_par_loop(api_twice_plus_one, "api_twice_plus_one", blk, 2, full,
+                 ops_arg_dat(V, 1, S00, "double", OPS_READ), ops_arg_dat(V, 1, S00, "double", OPS_WRITE));
+    ops_par_loop(api_twice_plus_one, "api_twice_plus_one", blk, 2, full,
+                 ops_arg_dat(V, 1, S00, "double", OPS_READ), ops_arg_dat(U, 1, S00, "double", OPS_WRITE));
+    ops_dat_fetch_data_slab_memspace(V, 0, (char *)out, full, OPS_HOST);
+    ops_dat_fetch_data_slab_memspace(U, 0, (char *)(out + (long)nx * ny), full, OPS_HOST);
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_alias", e);
+  }
+}
+
 extern "C" int api_reduce(int argc, const char **argv, int nx, int ny, const double *vals,
                           const int *ivals, double *out) {
   try {

@@ -21,4 +21,8 @@ void api_reduce(const ACC<double> &v, const ACC<int> &w, double *sum, double *mn
   pair[1] = pair[1] + 1.0;
 }
 
+// one dat read and written at the same point through two arguments (the runtime then uses
+// the kernel variant without no-alias pointers)
+void api_twice_plus_one(const ACC<double> &in, ACC<double> &out) { out(0, 0) = 2.0 * in(0, 0) + 1.0; }
+
 #endif

@@ -324,6 +324,12 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   lp->stream = (void *)g_dev.stream;
   lp->timed = 0;
   lp->bytes = 0;
+  lp->alias = 0;
+  for (int i = 0; i < nargs; i++)
+    for (int j = i + 1; j < nargs; j++)
+      if (args[i].argtype == OPS_ARG_DAT && args[j].argtype == OPS_ARG_DAT && args[i].opt && args[j].opt &&
+          args[i].dat == args[j].dat && (args[i].acc != OPS_READ || args[j].acc != OPS_READ))
+        lp->alias = 1;
   int start[OPS_MAX_DIM] = {0, 0, 0, 0, 0}, end[OPS_MAX_DIM] = {1, 1, 1, 1, 1};
   long total = 1;
   for (int d = 0; d < 3; d++) {

@@ -169,8 +169,31 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
             L.append(f"  {t} *rp{i};")
     L.append("};")
     L.append("")
+    if noalias_variant(loop):
+        # the same body twice: with every dat as a no-alias pointer parameter (the usual
+        # launch: loads need not wait behind the stores of other arguments), and reading
+        # the pointers from the struct (a dat passed to two arguments, one of them writing)
+        L.append(kernel_body(loop, params, ndim, k, True))
+        L.append("")
+        L.append(kernel_body(loop, params, ndim, k + "_alias", False))
+    else:
+        L.append(kernel_body(loop, params, ndim, k, False))
+    return "\n".join(L)
+
+
+def noalias_variant(loop: Loop) -> bool:
+    """a no-alias kernel variant pays when some dat is written and another dat is accessed"""
+    dats = [a for a in loop.args if a.kind == "dat"]
+    return len(dats) >= 2 and any(a.acc != "OPS_READ" for a in dats)
+
+
+def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: bool) -> str:
+    k = loop.stub
+    L = []
     red = [i for i, a in enumerate(loop.args) if a.kind == "reduce"]
-    L.append(f"__global__ __launch_bounds__(256) void ops_kern_{k}(const ops_kp_{k} p) {{")
+    extra = "".join(f", {params[i].ctype} *__restrict__ q{i}" for i, a in enumerate(loop.args)
+                    if a.kind == "dat") if noalias else ""
+    L.append(f"__global__ __launch_bounds__(256) void ops_kern_{name}(const ops_kp_{k} p{extra}) {{")
     for i in red:
         a = loop.args[i]
         t = CTYPES.get(a.typ, a.typ)
@@ -198,7 +221,7 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
                                      "must be ACC<T>& for an ops_arg_dat argument")
             t = pr.ctype
             off = f"x * p.sx{i} + y * p.sy{i} + z * p.sz{i}"
-            ptr = f"p.p{i} + ({off})"
+            ptr = f"{'q' if noalias else 'p.p'}{i} + ({off})"
             mlit = cparse.int_literal(a.dim)
             if mlit == 1:
                 if ndim == 1:
@@ -263,8 +286,16 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
         elif a.kind == "reduce":
             t = CTYPES.get(a.typ, a.typ)
             L.append(f"    p.rp{i} = ({t} *)lp.red_part[{i}];")
-    L.append(f"    hipLaunchKernelGGL({ns}::ops_kern_{k}, dim3(lp.grid[0], lp.grid[1], lp.grid[2]),")
-    L.append("                       dim3(lp.block[0], lp.block[1], lp.block[2]), 0, (hipStream_t)lp.stream, p);")
+    launch = ("hipLaunchKernelGGL({}, dim3(lp.grid[0], lp.grid[1], lp.grid[2]), "
+              "dim3(lp.block[0], lp.block[1], lp.block[2]), 0, (hipStream_t)lp.stream, p{});")
+    if noalias_variant(loop):
+        ptrs = "".join(f", p.p{i}" for i, a in enumerate(loop.args) if a.kind == "dat")
+        L.append("    if (lp.alias)")
+        L.append("      " + launch.format(f"{ns}::ops_kern_{k}_alias", ""))
+        L.append("    else")
+        L.append("      " + launch.format(f"{ns}::ops_kern_{k}", ptrs))
+    else:
+        L.append("    " + launch.format(f"{ns}::ops_kern_{k}", ""))
     L.append("    ops_hip_check_launch(name);")
     L.append("  }")
     L.append(f"  ops_hip_loop_end(&lp, args, {n});")

@@ -262,6 +262,23 @@ def test_reductions(kind, shape):
     assert out[7] == nx * ny
 
 
+# ---------------------------------------------------------------- one dat, two arguments
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_dat_read_and_written_through_two_arguments(kind):
+    """v = 2 v + 1 with v passed as both the read and the written argument (the device
+    build's kernel variant without no-alias pointers), then u = 2 v + 1 into another dat
+    (the no-alias variant)."""
+    L = lib("apitest", kind)
+    nx, ny = 67, 19
+    v = np.random.default_rng(5).integers(-1000, 1000, nx * ny).astype(np.float64)
+    out = np.zeros(2 * nx * ny)
+    argc, av = argv()
+    assert L.api_alias(argc, av, nx, ny, p(v), p(out)) == 0
+    assert np.array_equal(out[:nx * ny], 2 * v + 1)
+    assert np.array_equal(out[nx * ny:], 2 * (2 * v + 1) + 1)
+
+
 # ---------------------------------------------------------------- random numbers
 
 def mt_uniform(seed, n):
