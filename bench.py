@@ -5,7 +5,8 @@ steps, FP64), through libops_hip.so and the translated application. A step is on
 time step of the whole mesh; a cell-update is one cell advanced by one step.
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--cpu-threads T]
-                    [--no-tealeaf] [--tea-cells N] [--tea-steps S]
+                    [--no-tealeaf] [--tea-cells N] [--tea-steps S] [--no-learned]
+                    [--unet-pairs P] [--unet-epochs E] [--no-adi] [--adi-steps S]
 
 Prints one JSON line. `value` is measured with the inputs resident in HBM: the deck is
 generated on the device before the timed region; the K timed steps are bracketed by
@@ -15,7 +16,10 @@ the kernel with the largest total time from its ops_compute_transfer bytes per l
 `cpu_baseline` runs the same application on the oracle's host backend (OpenMP) for a
 bounded sample on this machine's cores. `tealeaf` reports the second half of the metric:
 TeaLeaf 4096^2 time-to-solution of plain FP64 CG against the multigrid-preconditioned
-solver (FP64 outer CG, FP32 V-cycle), with residuals and field differences.
+solver (FP64 outer CG, FP32 V-cycle), with residuals and field differences;
+`tealeaf_learned` the learned warm start (data generation, U-Net training throughput
+against the BF16 matrix-core peak, iterations and time with and without it); `adi` the
+batched tridiagonal application.
 """
 import argparse
 import ctypes
@@ -31,6 +35,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "cell-updates/s + % HBM roofline; TeaLeaf time-to-tol learned+mixed vs FP64 CG"
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X published dense BF16 matrix throughput
 GOLD_KE_BM16 = 0.307475452287895  # apps/c/CloverLeaf/field_summary.cpp:94
 
 BM16_DECK = """*clover
@@ -233,6 +238,50 @@ def tealeaf_gpu(args):
     }
 
 
+def tealeaf_learned_gpu(args):
+    """Learned warm start: training pairs at 1024^2 from seeded rectangle decks (seeds
+    20261203 + k for k >= 1, so the benchmark's own layout, seed 20261203, is not among
+    them), each deck run for --unet-steps-per-deck implicit steps with the FP64 multigrid
+    PCG to 1e-10; Adam training of the U-Net on the GPU; then the 4096^2 benchmark solve
+    (multigrid PCG, --tea-steps steps, tl_eps 1e-10) without and with the warm start, the
+    network evaluated on 4 x 4 averages. The warm start is kept only when its residual is
+    below the plain start's; the stop threshold is the plain start's either way."""
+    import numpy as np
+    import ops_amd
+    from ops_amd.apps.tealeaf import learned
+    from ops_amd.apps.tealeaf.decks import tea_deck
+    L = ops_amd.app("tealeaf")
+    net = 1024
+    assert L.tea_unet_init(7) == 0
+    pairs, epochs = args.unet_pairs, args.unet_epochs
+    gen_s = learned.generate(L, pairs, cells=net, steps_per_deck=args.unet_steps_per_deck, seed0=20261204)
+    log(f"[bench] learned: {pairs} pairs at {net}^2 in {gen_s:.1f} s")
+    loss, train_s = learned.train(L, epochs, batch=4, lr=1e-3, seed=1)
+    flops = learned.training_flops_per_pixel() * net * net * pairs * epochs
+    log(f"[bench] learned: {epochs} epochs in {train_s:.1f} s, loss {loss}")
+    n, steps = args.tea_cells, args.tea_steps
+    deck = tea_deck(n, steps, eps=1e-10, max_iters=20000)
+    plain = learned.solve_run(L, deck, 3, steps, fetch=("u",))
+    warm = learned.solve_run(L, deck, 3, steps, net_cells=net, fetch=("u",), mode=1)
+    assert L.tea_unet_free() == 0
+    diff = float(np.max(np.abs(warm["u"] - plain["u"])) / np.max(np.abs(plain["u"])))
+    log(f"[bench] learned: iterations {sum(plain['iters'])} plain, {sum(warm['iters'])} warm")
+    return {
+        "workload": f"tealeaf_2d {n}x{n}, {steps} steps, tl_eps 1e-10, FP64 multigrid PCG; "
+                    f"U-Net trained on {pairs} pairs at {net}^2, {epochs} epochs",
+        "data_generation_s": round(gen_s, 2),
+        "training": {"s": round(train_s, 2), "loss_per_epoch": [float(v) for v in loss],
+                     "TFLOPs": round(flops / train_s / 1e12, 2),
+                     "frac_of_bf16_mfma_peak": round(flops / train_s / 1e12 / BF16_MFMA_PEAK_TFLOPS, 5)},
+        "iterations": {"plain": plain["iters"], "warm_start": warm["iters"],
+                       "saved": sum(plain["iters"]) - sum(warm["iters"])},
+        "warm_start_kept": warm["kept"],
+        "initial_residual": {"plain": plain["initial"], "warm_start": warm["warm"]},
+        "time_to_solution_s": {"plain": round(plain["s"], 4), "warm_start": round(warm["s"], 4)},
+        "field_max_rel_diff": diff,
+    }
+
+
 def adi_gpu(args):
     """Batched ADI: 48 blocks with extents in {64..256} (splitmix64 seed 5), u ~ U[0,1),
     lambda 1, 100 steps of preproc + x, y, z tridiagonal solves, one step replayed as a
@@ -310,6 +359,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-tealeaf", action="store_true")
     ap.add_argument("--no-adi", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--unet-pairs", type=int, default=2000)
+    ap.add_argument("--unet-epochs", type=int, default=2)
+    ap.add_argument("--unet-steps-per-deck", type=int, default=10)
     ap.add_argument("--adi-steps", type=int, default=100)
     ap.add_argument("--tea-cells", type=int, default=4096)
     ap.add_argument("--tea-steps", type=int, default=10)
@@ -360,6 +413,8 @@ def main():
     }
     if not args.no_tealeaf:
         out["tealeaf"] = tealeaf_gpu(args)
+    if not args.no_tealeaf and not args.no_learned:
+        out["tealeaf_learned"] = tealeaf_learned_gpu(args)
     if not args.no_adi:
         out["adi"] = adi_gpu(args)
     if not args.no_cpu:

@@ -16,6 +16,24 @@ import numpy as np
 
 from .decks import tea_deck
 
+# the network's convolutions (native/tealeaf_unet.hip): input channels from the upsampled
+# source, from the same-resolution source, output channels, resolution level (1/2^level)
+UNET_LAYERS = [(0, 16, 16, 0), (0, 16, 32, 1), (0, 32, 48, 2), (0, 48, 64, 3),
+               (64, 48, 48, 2), (48, 32, 32, 1), (32, 16, 16, 0), (0, 16, 16, 0)]
+
+
+def forward_flops_per_pixel():
+    """multiply-adds x 2 of one forward pass per input pixel: sum over the 3 x 3
+    convolutions of 2 * 9 * Cin * Cout at each layer's resolution"""
+    return sum(2 * 9 * (c1 + c2) * co / 4 ** lv for c1, c2, co, lv in UNET_LAYERS)
+
+
+def training_flops_per_pixel():
+    """forward + weight gradient (the same products) + input gradient (every layer but the
+    first)"""
+    c1, c2, co, lv = UNET_LAYERS[0]
+    return 3 * forward_flops_per_pixel() - 2 * 9 * (c1 + c2) * co / 4 ** lv
+
 
 def _run_deck(L, text, solver):
     import ops_amd

@@ -238,15 +238,32 @@ __global__ __launch_bounds__(256) void k_mask(const float *g, const bf16 *y, int
   (void)cout;
 }
 
-// bias gradient: db[co] = sum over pixels of the masked gradient (one workgroup per
-// channel, fixed order)
-__global__ __launch_bounds__(256) void k_bias_grad(const float *gmf, long npix, int cout, float *db) {
-  __shared__ float lds[4];
-  const int co = blockIdx.x;
+// bias gradient, first pass: each workgroup sums the masked gradient of its pixel slice per
+// channel (thread t owns channel t % cout and every (256 / cout)-th pixel), bpart[wg][co]
+constexpr int BIAS_WG = 512;
+__global__ __launch_bounds__(256) void k_bias_part(const float *gmf, long npix, int cout, float *bpart) {
+  __shared__ float lds[256];
+  const int lanes = 256 / cout, t = threadIdx.x, c = t % cout, q = t / cout;
+  const long per = (npix + BIAS_WG - 1) / BIAS_WG, p0 = blockIdx.x * per, p1 = min(npix, p0 + per);
   float s = 0.f;
-  for (long p = threadIdx.x; p < npix; p += 256) s += gmf[p * cout + co];
-  s = ops_block_reduce<ops_red_inc<float>>(s, lds);
-  if (threadIdx.x == 0) db[co] = s;
+  if (q < lanes)
+    for (long p = p0 + q; p < p1; p += lanes) s += gmf[p * cout + c];
+  lds[t] = s;
+  __syncthreads();
+  if (t < cout) {
+    float v = 0.f;
+    for (int k = 0; k < lanes; k++) v += lds[k * cout + t];
+    bpart[blockIdx.x * cout + t] = v;
+  }
+}
+
+// second pass: db[co] = sum over workgroups (fixed order)
+__global__ __launch_bounds__(64) void k_bias_fin(const float *bpart, int cout, float *db) {
+  const int c = threadIdx.x;
+  if (c >= cout) return;
+  float v = 0.f;
+  for (int k = 0; k < BIAS_WG; k++) v += bpart[k * cout + c];
+  db[c] = v;
 }
 
 // weights for the input gradient: wt[ci][8 - tap][co] = w[co][tap][ci]
@@ -396,6 +413,7 @@ struct Net {
   float *wpart = nullptr;
   size_t wpart_cap = 0;
   float *lpart = nullptr;
+  float *bpart = nullptr;  // bias-gradient partials [BIAS_WG][64]
   char *pinned = nullptr;
 };
 Net N;
@@ -432,6 +450,7 @@ void free_acts() {
   dfree(N.gm);
   dfree(N.gmf);
   dfree(N.lpart);
+  dfree(N.bpart);
   N.B = N.H = N.W = 0;
 }
 
@@ -463,6 +482,7 @@ void ensure_acts(int B, int H, int W) {
   N.gm = dalloc<bf16>(maxg);
   N.gmf = dalloc<float>(maxg);
   N.lpart = dalloc<float>(g1d(pix(0)) + 1);
+  N.bpart = dalloc<float>((size_t)BIAS_WG * 64);
 }
 
 
@@ -532,7 +552,8 @@ void backward() {
     // masked gradient
     hipLaunchKernelGGL(k_mask, dim3(g1d(n)), dim3(256), 0, st, N.grad[l], N.act[l], SPEC[l].relu, n, cout,
                        N.gm, N.gmf);
-    hipLaunchKernelGGL(k_bias_grad, dim3(cout), dim3(256), 0, st, N.gmf, npix, cout, N.g + N.boff[l]);
+    hipLaunchKernelGGL(k_bias_part, dim3(BIAS_WG), dim3(256), 0, st, N.gmf, npix, cout, N.bpart);
+    hipLaunchKernelGGL(k_bias_fin, dim3(1), dim3(64), 0, st, N.bpart, cout, N.g + N.boff[l]);
     // weight gradient
     WgradArgs wa;
     wa.dy = N.gm;
