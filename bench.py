@@ -301,7 +301,7 @@ def adi_gpu(args):
     assert L.adi_init(argc, argv, len(sizes), sz.ctypes.data_as(ctypes.c_void_p), 1, 5, 1.0) == 0
     assert L.adi_step(3) == 0
     assert L.adi_finalize() == 0
-    os.environ["OPS_ADI_GRAPH"] = "1"
+    os.environ["OPS_ADI_GRAPH"] = "0" if args.adi_eager else "1"
     assert L.adi_init(argc, argv, len(sizes), sz.ctypes.data_as(ctypes.c_void_p), 1, 5, 1.0) == 0
     assert L.adi_step(2) == 0  # eager step + graph capture + one replay
     R.ops_hip_device_sync()
@@ -364,6 +364,8 @@ def main():
     ap.add_argument("--unet-epochs", type=int, default=2)
     ap.add_argument("--unet-steps-per-deck", type=int, default=10)
     ap.add_argument("--adi-steps", type=int, default=100)
+    ap.add_argument("--adi-eager", action="store_true",
+                    help="time eager launches instead of the captured step (for profilers)")
     ap.add_argument("--tea-cells", type=int, default=4096)
     ap.add_argument("--tea-steps", type=int, default=10)
     ap.add_argument("--ingest-pmc", metavar="DIR", default=None,
@@ -378,8 +380,8 @@ def main():
         # the warm-up launches and keeps the fastest for the timed ones
         os.environ["OPS_AUTOTUNE"] = "1"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the mesh would be decomposed)", file=sys.stderr)
+        print("bench.py: runs on one GPU only; the mesh is not decomposed across GPUs, "
+              "so --gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
     import ops_amd
     R = ops_amd.runtime()
@@ -413,7 +415,7 @@ def main():
     }
     if not args.no_tealeaf:
         out["tealeaf"] = tealeaf_gpu(args)
-    if not args.no_tealeaf and not args.no_learned:
+    if not args.no_learned:
         out["tealeaf_learned"] = tealeaf_learned_gpu(args)
     if not args.no_adi:
         out["adi"] = adi_gpu(args)

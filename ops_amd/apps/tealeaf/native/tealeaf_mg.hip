@@ -467,7 +467,11 @@ void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops
   const void *key[8] = {G.mem, u64, r64, pa, w64, kx, ky, (const void *)(long)n};
   bool same = G.graph[0] != nullptr;
   for (int k = 0; k < 8 && same; k++) same = key[k] == G.graph_key[k];
-  if (!same) {
+  // OPS_TEA_GRAPH=0: launch every iteration's kernels eagerly (the same kernels and order;
+  // for tools that do not follow graph launches)
+  const char *ge = getenv("OPS_TEA_GRAPH");
+  const bool use_graph = !(ge && atoi(ge) == 0);
+  if (use_graph && !same) {
     drop_graphs();
     for (int par = 0; par < 2; par++) {
       hipGraph_t gr;
@@ -486,7 +490,10 @@ void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops
   CK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   while (issued < P->max_iters) {
     for (int k = 0; k < POLL && issued < P->max_iters; k++) {
-      CK(hipGraphLaunch(G.graph[(issued + 1) & 1], st));
+      if (use_graph)
+        CK(hipGraphLaunch(G.graph[(issued + 1) & 1], st));
+      else
+        iteration((issued + 1) & 1);
       issued++;
     }
     if (pending) {
