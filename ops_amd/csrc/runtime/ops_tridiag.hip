@@ -72,6 +72,11 @@ template <typename T> __device__ __forceinline__ int find_sys(const Batch<T> &B,
   return k;
 }
 
+// rows per chunk of the lane-serial sweeps: a chunk's loads are all issued before its
+// arithmetic, so a lane waits for memory once per chunk rather than once per row (the
+// recurrence itself is unchanged, row by row)
+constexpr int TU = 8;
+
 // y- and z-lines (and x-lines of tiny systems are not special-cased): one lane per line
 template <typename T, bool INC>
 __global__ __launch_bounds__(256) void thomas_strided(const Batch<T> B, T *__restrict__ scratch) {
@@ -86,27 +91,54 @@ __global__ __launch_bounds__(256) void thomas_strided(const Batch<T> B, T *__res
   const int n = S.n[sd];
   const long long L = S.nlines;
   T *cp = scratch + S.scr, *dp = cp + (long long)n * L;
-  const T *a = S.a, *b = S.b, *c = S.c;
+  const T *__restrict__ a = S.a, *__restrict__ b = S.b, *__restrict__ c = S.c;
   T *d = S.d, *u = S.u;
   T cprev = c[o] / b[o];
   T dprev = d[o] / b[o];
   cp[l] = cprev;
   dp[l] = dprev;
-  for (int i = 1; i < n; i++) {
-    const long long kk = o + i * st;
-    const T m = T(1) / (b[kk] - a[kk] * cprev);
-    cprev = c[kk] * m;
-    dprev = (d[kk] - a[kk] * dprev) * m;
-    cp[i * L + l] = cprev;
-    dp[i * L + l] = dprev;
+  for (int i0 = 1; i0 < n; i0 += TU) {
+    T ra[TU], rb[TU], rc[TU], rd[TU];
+#pragma unroll
+    for (int j = 0; j < TU; j++)
+      if (i0 + j < n) {
+        const long long kk = o + (long long)(i0 + j) * st;
+        ra[j] = a[kk];
+        rb[j] = b[kk];
+        rc[j] = c[kk];
+        rd[j] = d[kk];
+      }
+#pragma unroll
+    for (int j = 0; j < TU; j++)
+      if (i0 + j < n) {
+        const T m = T(1) / (rb[j] - ra[j] * cprev);
+        cprev = rc[j] * m;
+        dprev = (rd[j] - ra[j] * dprev) * m;
+        cp[(long long)(i0 + j) * L + l] = cprev;
+        dp[(long long)(i0 + j) * L + l] = dprev;
+      }
   }
   T x = dprev;
   long long kk = o + (long long)(n - 1) * st;
   if (INC) u[kk] += x; else d[kk] = x;
-  for (int i = n - 2; i >= 0; i--) {
-    x = dp[i * L + l] - cp[i * L + l] * x;
-    kk = o + i * st;
-    if (INC) u[kk] += x; else d[kk] = x;
+  for (int i1 = n - 2; i1 >= 0; i1 -= TU) {
+    // rows i1, i1 - 1, ..., i1 - TU + 1
+    T rc[TU], rd[TU], ru[TU];
+#pragma unroll
+    for (int j = 0; j < TU; j++)
+      if (i1 - j >= 0) {
+        const long long i = i1 - j;
+        rc[j] = cp[i * L + l];
+        rd[j] = dp[i * L + l];
+        if (INC) ru[j] = u[o + i * st];
+      }
+#pragma unroll
+    for (int j = 0; j < TU; j++)
+      if (i1 - j >= 0) {
+        x = rd[j] - rc[j] * x;
+        const long long kj = o + (long long)(i1 - j) * st;
+        if (INC) u[kj] = ru[j] + x; else d[kj] = x;
+      }
   }
 }
 
@@ -170,12 +202,20 @@ __global__ __launch_bounds__(64) void thomas_x(const Batch<T> B, T *__restrict__
     const int r0 = ci * XR;
     const int rows = (n - r0) < XR ? (n - r0) : XR;
     if (own) {
-      for (int ri = rows - 1; ri >= 0; ri--) {
-        const int i = r0 + ri;
-        if (i == n - 1) x = dp[i * L + myl];
-        else x = dp[i * L + myl] - cp[i * L + myl] * x;
-        sd[lane][ri] = x;
-      }
+      T rc[XR], rd[XR];
+#pragma unroll
+      for (int ri = 0; ri < XR; ri++)
+        if (ri < rows) {
+          rc[ri] = cp[(long long)(r0 + ri) * L + myl];
+          rd[ri] = dp[(long long)(r0 + ri) * L + myl];
+        }
+#pragma unroll
+      for (int ri = XR - 1; ri >= 0; ri--)
+        if (ri < rows) {
+          if (r0 + ri == n - 1) x = rd[ri];
+          else x = rd[ri] - rc[ri] * x;
+          sd[lane][ri] = x;
+        }
     }
     __syncthreads();
     for (int e = lane; e < XL * XR; e += XL) {
