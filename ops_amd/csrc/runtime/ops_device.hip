@@ -409,10 +409,18 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       if (in->block_size_y > 0) by = in->block_size_y;
       bz = OPS_HIP_BLOCK_THREADS / (bx * by);
       if (bz < 1) bz = 1;
-    } else {
+    } else if (dim == 2) {
       by = OPS_HIP_BLOCK_THREADS / bx;
       if (in->block_size_y > 0) by = in->block_size_y;
       bz = 1;
+    } else {
+      // 1-D: no second dimension to stack along, so the workgroup is one row of lanes
+      bx = OPS_HIP_BLOCK_THREADS;
+      by = bz = 1;
+      if (in->block_size_x > 0) {
+        bx = in->block_size_x;
+        by = OPS_HIP_BLOCK_THREADS / bx;  // whole wavefronts, as in 2-D
+      }
     }
   }
   // autotuning: workgroup shapes tried on the kernel's first launches. Loops with a sum
@@ -426,16 +434,20 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
     if (sum_red) {
       kt[kernel_id].tune_state = -1;
     } else {
-      static const unsigned shapes2[][3] = {{64, 4, 1}, {128, 2, 1}, {256, 1, 1}, {64, 2, 1}, {32, 8, 1}};
-      static const unsigned shapes3[][3] = {{64, 4, 1}, {64, 2, 2}, {32, 4, 2}, {64, 1, 4}, {128, 2, 1}};
-      static const unsigned shapes1[][3] = {{256, 1, 1}, {128, 1, 1}, {64, 1, 1}};
+      // candidate 0 is the heuristic shape above (narrow along x for narrow ranges, such as
+      // the halo strips), so tuning never ends worse than the default
+      static const unsigned shapes2[][3] = {{0, 0, 0}, {64, 4, 1}, {128, 2, 1}, {256, 1, 1}, {32, 8, 1}};
+      static const unsigned shapes3[][3] = {{0, 0, 0}, {64, 4, 1}, {64, 2, 2}, {32, 4, 2}, {64, 1, 4}};
+      static const unsigned shapes1[][3] = {{0, 0, 0}, {256, 1, 1}, {128, 1, 1}};
       const int nc = dim >= 3 ? 5 : (dim == 2 ? 5 : 3);
       const int c = ops_tune_next(kernel_id, nc, 2);
       if (c >= 0) {
         const unsigned *sh = dim >= 3 ? shapes3[c] : (dim == 2 ? shapes2[c] : shapes1[c]);
-        bx = sh[0];
-        by = sh[1];
-        bz = sh[2];
+        if (sh[0] != 0) {
+          bx = sh[0];
+          by = sh[1];
+          bz = sh[2];
+        }
         if (kt[kernel_id].tune_state == 1) lp->tune_cand = c;
       }
     }

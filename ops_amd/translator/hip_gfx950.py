@@ -150,6 +150,16 @@ def const_dim(a: Arg, loop: Loop) -> str:
 
 # loop points per lane along x (blockDim.x apart)
 VX = int(os.environ.get("OPS_GEN_VX", "1"))
+# loops whose only reductions are MIN/MAX (order-free, so results are unchanged) take this
+# many points per lane: the per-workgroup reduction tree is then paid once per 4 x 256 points
+RED_VX = int(os.environ.get("OPS_GEN_RED_VX", "4"))
+
+
+def loop_vx(loop: Loop) -> int:
+    reds = [a for a in loop.args if a.kind == "reduce"]
+    if reds and all(a.acc in ("OPS_MIN", "OPS_MAX") for a in reds):
+        return max(VX, RED_VX)
+    return VX
 
 
 def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
@@ -204,11 +214,12 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
     L.append("  ops_block_coords(bx, by, bz);")
     L.append("  const int y = (int)(by * blockDim.y + threadIdx.y);")
     L.append("  const int z = (int)(bz * blockDim.z + threadIdx.z);")
-    if VX > 1:
-        # VX points per lane, blockDim.x apart: every pass of the wavefront stays contiguous
+    vx = loop_vx(loop)
+    if vx > 1:
+        # vx points per lane, blockDim.x apart: every pass of the wavefront stays contiguous
         L.append("#pragma unroll")
-        L.append(f"  for (int v = 0; v < {VX}; v++) {{")
-        L.append(f"  const int x = (int)((bx * {VX} + v) * blockDim.x + threadIdx.x);")
+        L.append(f"  for (int v = 0; v < {vx}; v++) {{")
+        L.append(f"  const int x = (int)((bx * {vx} + v) * blockDim.x + threadIdx.x);")
     else:
         L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
     L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
@@ -250,7 +261,7 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
             call.append(f"idx{i}")
     L.append(f"    {loop.kernel}({', '.join(call)});")
     L.append("  }")
-    if VX > 1:
+    if vx > 1:
         L.append("  }")
     for i in red:
         a = loop.args[i]
@@ -270,7 +281,7 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     L.append(f"  static int kid = ops_hip_register_kernel(\"{loop.kernel}\");")
     L.append(f"  ops_arg args[{max(n, 1)}] = {{{', '.join(f'arg{i}' for i in range(n)) or 'ops_arg_idx()'}}};")
     L.append("  ops_hip_loop lp;")
-    L.append(f"  lp.vx = {VX};")
+    L.append(f"  lp.vx = {loop_vx(loop)};")
     L.append(f"  if (ops_hip_loop_begin(&lp, kid, name, block, dim, range, args, {n})) {{")
     L.append(f"    {ns}::ops_kp_{k} p;")
     L.append("    p.cnt0 = lp.count[0]; p.cnt1 = lp.count[1]; p.cnt2 = lp.count[2];")
