@@ -51,6 +51,9 @@ typedef struct ops_hip_loop {
   int kernel_id;
   int timed;
   int tune_cand; /* autotuning candidate measured by this launch, -1 otherwise */
+  unsigned prolong_mask; /* set by the stub: bit i = argument i was translated with
+                            prolong addressing (coarse index = loop index / mgrid stride) */
+  int mg[OPS_HIP_MAX_ARGS][3]; /* prolong arguments: the mgrid stride per dimension (1 otherwise) */
   int alias;     /* 1 if one dat is passed to two arguments and at least one writes it: the
                     generated stub then launches the kernel variant without no-alias
                     pointer parameters */

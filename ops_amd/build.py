@@ -39,6 +39,7 @@ APP_SOURCES = {
     "cloverleaf": ["clover.cpp"],
     "tealeaf": ["tealeaf.cpp"],
     "adi": ["adi.cpp"],
+    "mgrid": ["mgrid.cpp"],
 }
 
 

@@ -52,10 +52,11 @@ void ops_check_loop_bounds(const char *name, int dim, const int *start, const in
     const ops_arg &a = args[i];
     if (a.argtype != OPS_ARG_DAT || !a.opt || a.dat == nullptr || a.stencil == nullptr) continue;
     const ops_stencil s = a.stencil;
-    if (s->type != 0) continue;  // restrict/prolong stencils map coordinates between grids
     const ops_dat dat = a.dat;
     for (int d = 0; d < dim && d < dat->block->dims; d++) {
       int stride = d < s->dims ? s->stride[d] : 1;
+      // multigrid stencils: restrict reads index * m, prolong index / m (C division)
+      const int m = (s->type != 0 && d < s->dims) ? s->mgrid_stride[d] : 1;
       int pmin = 0, pmax = 0;
       if (d < s->dims)
         for (int p = 0; p < s->points; p++) {
@@ -66,6 +67,13 @@ void ops_check_loop_bounds(const char *name, int dim, const int *start, const in
       long off = -(long)dat->base[d] - dat->d_m[d];
       long lo = (long)start[d] * stride + pmin + off;
       long hi = (long)(end[d] - 1) * stride + pmax + off;
+      if (s->type == 2) {
+        lo = (long)start[d] * stride * m + pmin + off;
+        hi = (long)(end[d] - 1) * stride * m + pmax + off;
+      } else if (s->type == 1) {
+        lo = (long)(start[d] / m) * stride + pmin + off;
+        hi = (long)((end[d] - 1) / m) * stride + pmax + off;
+      }
       if (stride == 0) {
         lo = pmin + off;
         hi = pmax + off;

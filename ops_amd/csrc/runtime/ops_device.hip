@@ -360,11 +360,24 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
                       " that is locked by ops_dat_get_raw_pointer");
       if (dat->dirty_hd == 1) ops_put_data(dat);
       const int *sst = a.stencil->stride;
+      // multigrid stencils (templates/cpp/mpi_openmp/loop_host.cpp.j2:230-237): a restrict
+      // argument is read at loop index * mgrid stride (affine: the stride scales the step), a
+      // prolong argument at loop index / mgrid stride (the kernel divides; the pointer here
+      // is the dat's index 0 and the steps are per coarse index)
+      const int stype = a.stencil->type;
+      if (((lp->prolong_mask >> i) & 1u) != (stype == 1 ? 1u : 0u))
+        ops_throw(OPS_INVALID_ARGUMENT,
+                  std::string("ops_par_loop ") + (name ? name : "") + ": argument " + std::to_string(i) +
+                      (stype == 1 ? " has a prolong stencil but was not translated as one"
+                                  : " was translated for a prolong stencil but has another"));
+      auto mgs = [&](int d) { return (stype != 0 && d < a.stencil->dims) ? a.stencil->mgrid_stride[d] : 1; };
       size_t esz = in->soa ? (size_t)dat->type_size : (size_t)dat->elem_size;
       long long off = 0, cum = 1;
       for (int d = 0; d < dim; d++) {
         int s = (d < a.stencil->dims) ? sst[d] : 1;
-        off += (long long)esz * cum * ((long long)start[d] * s - dat->base[d] - dat->d_m[d]);
+        if (stype == 2) s *= mgs(d);
+        const long long first = stype == 1 ? 0 : (long long)start[d] * s;
+        off += (long long)esz * cum * (first - dat->base[d] - dat->d_m[d]);
         cum *= dat->size[d];
       }
       lp->ptr[i] = dat->data_d + off;
@@ -372,7 +385,9 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       long long cumd = 1;
       for (int d = 0; d < 3; d++) {
         int s = (d < dim && d < a.stencil->dims) ? sst[d] : 0;
+        if (stype == 2 && d < dim) s *= mgs(d);
         lp->step[i][d] = (int)(s * cumd * comp);
+        lp->mg[i][d] = (stype == 1 && d < dim) ? mgs(d) : 1;
         cumd *= dat->size[d];
       }
       lp->xdim[i] = dat->size[0];

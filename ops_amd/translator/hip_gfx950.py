@@ -43,10 +43,12 @@ class Arg:
     typ: str = ""
     acc: str = "OPS_READ"
     opt: bool = False
+    prolong: bool = False     # stencil declared with ops_decl_prolong_stencil
 
     def key(self):
         return (self.kind, self.typ, self.acc if self.kind != "dat" else self.acc, self.dim
-                if self.kind != "dat" else (self.dim if cparse.int_literal(self.dim) == 1 else "m"))
+                if self.kind != "dat" else (self.dim if cparse.int_literal(self.dim) == 1 else "m"),
+                self.prolong)
 
 
 @dataclass
@@ -172,6 +174,8 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
         if a.kind == "dat":
             t = params[i].ctype
             L.append(f"  {t} *p{i}; int sx{i}, sy{i}, sz{i}, xd{i}, yd{i}, zd{i}, md{i};")
+            if a.prolong:
+                L.append(f"  int mx{i}, my{i}, mz{i};")
         elif a.kind == "gbl":
             L.append(f"  ops_gbl<{CTYPES.get(a.typ, a.typ)}, {const_dim(a, loop)}> g{i};")
         elif a.kind == "reduce":
@@ -232,6 +236,11 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
                                      "must be ACC<T>& for an ops_arg_dat argument")
             t = pr.ctype
             off = f"x * p.sx{i} + y * p.sy{i} + z * p.sz{i}"
+            if a.prolong:
+                # coarse index = global loop index / mgrid stride (C division, as the
+                # reference's generated loops: templates/cpp/mpi_openmp/loop_host.cpp.j2:234-237)
+                off = (f"((p.st0 + x) / p.mx{i}) * p.sx{i} + ((p.st1 + y) / p.my{i}) * p.sy{i}"
+                       f" + ((p.st2 + z) / p.mz{i}) * p.sz{i}")
             ptr = f"{'q' if noalias else 'p.p'}{i} + ({off})"
             mlit = cparse.int_literal(a.dim)
             if mlit == 1:
@@ -282,6 +291,8 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     L.append(f"  ops_arg args[{max(n, 1)}] = {{{', '.join(f'arg{i}' for i in range(n)) or 'ops_arg_idx()'}}};")
     L.append("  ops_hip_loop lp;")
     L.append(f"  lp.vx = {loop_vx(loop)};")
+    mask = sum(1 << i for i, a in enumerate(loop.args) if a.kind == "dat" and a.prolong)
+    L.append(f"  lp.prolong_mask = {mask}u;")
     L.append(f"  if (ops_hip_loop_begin(&lp, kid, name, block, dim, range, args, {n})) {{")
     L.append(f"    {ns}::ops_kp_{k} p;")
     L.append("    p.cnt0 = lp.count[0]; p.cnt1 = lp.count[1]; p.cnt2 = lp.count[2];")
@@ -292,6 +303,8 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
             L.append(f"    p.p{i} = ({t} *)lp.ptr[{i}]; p.sx{i} = lp.step[{i}][0]; p.sy{i} = lp.step[{i}][1]; "
                      f"p.sz{i} = lp.step[{i}][2];")
             L.append(f"    p.xd{i} = lp.xdim[{i}]; p.yd{i} = lp.ydim[{i}]; p.zd{i} = lp.zdim[{i}]; p.md{i} = lp.mdim[{i}];")
+            if a.prolong:
+                L.append(f"    p.mx{i} = lp.mg[{i}][0]; p.my{i} = lp.mg[{i}][1]; p.mz{i} = lp.mg[{i}][2];")
         elif a.kind == "gbl":
             L.append(f"    memcpy(p.g{i}.v, arg{i}.data, sizeof(p.g{i}.v));")
         elif a.kind == "reduce":
@@ -340,6 +353,18 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
     for (_, _, cs, _) in scanned.values():
         for c in cs:
             consts.setdefault(c.name, c)
+    # stencils declared as prolong stencils (their loops divide the loop index by the mgrid
+    # stride; the runtime checks every launch against the stencil it is given)
+    prolong = set()
+    for (text, _, _, _) in scanned.values():
+        masked = cparse.mask(text)
+        for m in re.finditer(r"\b([A-Za-z_]\w*)\s*=\s*ops_decl_prolong_stencil\s*\(", masked):
+            prolong.add(m.group(1))
+    for (_, lps, _, _) in scanned.values():
+        for lp in lps:
+            for a in lp.args:
+                if a.kind == "dat" and a.stencil.strip() in prolong:
+                    a.prolong = True
     # stubs: one per (kernel, argument signature)
     variants: Dict[str, List[tuple]] = {}
     loops_all: List[Loop] = []

@@ -9,6 +9,10 @@
  *   - ops_arg_gbl READ passes the user's pointer, ops_arg_reduce the handle's running
  *     value (:117-126), ops_arg_idx the loop index (:127-134);
  *   - writes mark the dat host-dirty (:299-301).
+ * Multigrid stencils follow the reference's generated loops, which the sequential header
+ * does not implement (templates/cpp/mpi_openmp/loop_host.cpp.j2:230-237): a restrict
+ * argument is addressed at loop index * mgrid stride, a prolong argument at loop index /
+ * mgrid stride (C division).
  * The outer loop (y in 2D, y*z in 3D) is split over OpenMP threads, as the reference's
  * mpi_openmp target does (templates/cpp/mpi_openmp/loop_host.cpp.j2:150-330); reductions
  * go to per-thread copies combined in thread order, and with one thread the kernel
@@ -54,6 +58,7 @@ template <typename T> struct Param<ACC<T>> {
     T *base;
     long sx, sy, sz;
     int xd, yd, zd, md;
+    int prolong, mg[3], st[3];
   };
   static St prepare(const ops_arg &a, const int *start, int ndim) {
     St s{};
@@ -61,16 +66,27 @@ template <typename T> struct Param<ACC<T>> {
     ops_dat d = a.dat;
     int soa = ops_cpu_soa();
     long esz = soa ? d->type_size : d->elem_size;
+    const int type = a.stencil->type;  // 0 regular, 1 prolong, 2 restrict
+    int st[3] = {1, 1, 1};
+    for (int i = 0; i < 3; i++) s.mg[i] = 1;
+    for (int i = 0; i < ndim && i < 3; i++) {
+      const int m = (type != 0 && i < a.stencil->dims) ? a.stencil->mgrid_stride[i] : 1;
+      st[i] = a.stencil->stride[i] * (type == 2 ? m : 1);
+      s.mg[i] = type == 1 ? m : 1;
+      s.st[i] = start[i];
+    }
+    s.prolong = type == 1;
     long off = 0, cum = 1;
     for (int i = 0; i < ndim; i++) {
-      off += esz * cum * ((long)start[i] * a.stencil->stride[i] - d->base[i] - d->d_m[i]);
+      const long first = type == 1 ? 0 : (long)start[i] * st[i];
+      off += esz * cum * (first - d->base[i] - d->d_m[i]);
       cum *= d->size[i];
     }
     s.base = (T *)(d->data + off);
     long comp = soa ? 1 : d->dim;
-    s.sx = a.stencil->stride[0] * comp;
-    s.sy = ndim > 1 ? (long)a.stencil->stride[1] * d->size[0] * comp : 0;
-    s.sz = ndim > 2 ? (long)a.stencil->stride[2] * d->size[0] * d->size[1] * comp : 0;
+    s.sx = st[0] * comp;
+    s.sy = ndim > 1 ? (long)st[1] * d->size[0] * comp : 0;
+    s.sz = ndim > 2 ? (long)st[2] * d->size[0] * d->size[1] * comp : 0;
     s.xd = d->size[0];
     s.yd = d->size[1];
     s.zd = d->size[2];
@@ -79,7 +95,12 @@ template <typename T> struct Param<ACC<T>> {
   }
   static ACC<T> &get(const St &s, const ops_arg &, Ctx &c, const int *lc, const int *, int) {
     static_assert(sizeof(ACC<T>) <= sizeof(c.acc), "ACC view does not fit its slot");
-    T *p = s.base ? s.base + (lc[0] * s.sx + lc[1] * s.sy + lc[2] * s.sz) : nullptr;
+    T *p = nullptr;
+    if (s.base && s.prolong)
+      p = s.base + ((s.st[0] + lc[0]) / s.mg[0]) * s.sx + ((s.st[1] + lc[1]) / s.mg[1]) * s.sy +
+          ((s.st[2] + lc[2]) / s.mg[2]) * s.sz;
+    else if (s.base)
+      p = s.base + (lc[0] * s.sx + lc[1] * s.sy + lc[2] * s.sz);
 #if defined(OPS_1D)
     return *(s.md == 1 ? new (c.acc) ACC<T>(p) : new (c.acc) ACC<T>(s.md, s.xd, p));
 #elif defined(OPS_2D)

@@ -68,11 +68,11 @@ extern "C" void ops_cpu_before_loop(ops_arg *args, int n) {
 extern "C" void ops_cpu_check_bounds(int dim, const int *range, const ops_arg *args, int n) {
   for (int i = 0; i < n; i++) {
     const ops_arg &a = args[i];
-    if (a.argtype != OPS_ARG_DAT || !a.opt || !a.dat || !a.stencil || a.stencil->type != 0)
-      continue;
+    if (a.argtype != OPS_ARG_DAT || !a.opt || !a.dat || !a.stencil) continue;
     const ops_stencil s = a.stencil;
     for (int d = 0; d < dim && d < a.dat->block->dims; d++) {
       const int st = d < s->dims ? s->stride[d] : 1;
+      const int m = (s->type != 0 && d < s->dims) ? s->mgrid_stride[d] : 1;
       int lo_p = 0, hi_p = 0;
       for (int p = 0; d < s->dims && p < s->points; p++) {
         const int v = s->stencil[p * s->dims + d];
@@ -80,8 +80,15 @@ extern "C" void ops_cpu_check_bounds(int dim, const int *range, const ops_arg *a
         hi_p = (p == 0 || v > hi_p) ? v : hi_p;
       }
       const long shift = -(long)a.dat->base[d] - a.dat->d_m[d];
-      const long first = (st ? (long)range[2 * d] * st : 0) + lo_p + shift;
-      const long last = (st ? (long)(range[2 * d + 1] - 1) * st : 0) + hi_p + shift;
+      long first = (st ? (long)range[2 * d] * st : 0) + lo_p + shift;
+      long last = (st ? (long)(range[2 * d + 1] - 1) * st : 0) + hi_p + shift;
+      if (s->type == 2) {  // restrict: index * m
+        first = (long)range[2 * d] * st * m + lo_p + shift;
+        last = (long)(range[2 * d + 1] - 1) * st * m + hi_p + shift;
+      } else if (s->type == 1) {  // prolong: index / m
+        first = (long)(range[2 * d] / m) * st + lo_p + shift;
+        last = (long)((range[2 * d + 1] - 1) / m) * st + hi_p + shift;
+      }
       if (first < 0 || last >= a.dat->size[d])
         fail(OPS_INVALID_ARGUMENT, std::string("ops_par_loop: dat ") + a.dat->name +
                                        " addressed outside its allocation in dimension " +
