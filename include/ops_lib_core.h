@@ -154,6 +154,9 @@ typedef struct ops_dat_core {
   int is_hdf5;
   char const *hdf5_file;
   int e_dat;                /* 1 if some dimension has extent 1 (edge dat) */
+  int edat_acc;             /* access of the last loop that wrote this edge dat: a low-dim
+                               reduction (INC/MIN/MAX) with a different access first fills
+                               it with that reduction's identity */
   size_t mem;               /* bytes allocated */
   size_t base_offset;       /* bytes from allocation start to loop index 0 */
   int stride[OPS_MAX_DIM];
@@ -421,10 +424,36 @@ public:
     return ptr[off(d) + (x + sizex * (y + sizey * z)) * step()];
   }
 #endif
+  // low-dimensional reductions (ops_lib_core.h:1396-1900 of the reference): many loop
+  // points combine into one element of a dat with size 1 in the collapsed dimensions;
+  // atomics on the device, plain updates on the host (the host backend runs such loops on
+  // one thread)
+#if defined(OPS_1D)
+  OPS_HD void combine_max(int x, const T v) { cmax((*this)(x), v); }
+  OPS_HD void combine_min(int x, const T v) { cmin((*this)(x), v); }
+  OPS_HD void combine_inc(int x, const T v) { cinc((*this)(x), v); }
+#elif defined(OPS_2D)
+  OPS_HD void combine_max(int x, int y, const T v) { cmax((*this)(x, y), v); }
+  OPS_HD void combine_min(int x, int y, const T v) { cmin((*this)(x, y), v); }
+  OPS_HD void combine_inc(int x, int y, const T v) { cinc((*this)(x, y), v); }
+#elif defined(OPS_3D)
+  OPS_HD void combine_max(int x, int y, int z, const T v) { cmax((*this)(x, y, z), v); }
+  OPS_HD void combine_min(int x, int y, int z, const T v) { cmin((*this)(x, y, z), v); }
+  OPS_HD void combine_inc(int x, int y, int z, const T v) { cinc((*this)(x, y, z), v); }
+#endif
   OPS_HD void next(int offset) { ptr += offset; }
   OPS_HD T *raw() const { return ptr; }
 
 private:
+#if defined(__HIP_DEVICE_COMPILE__)
+  __device__ static void cmax(T &x, T v) { atomicMax(&x, v); }
+  __device__ static void cmin(T &x, T v) { atomicMin(&x, v); }
+  __device__ static void cinc(T &x, T v) { atomicAdd(&x, v); }
+#else
+  OPS_HD static void cmax(T &x, T v) { if (v > x) x = v; }
+  OPS_HD static void cmin(T &x, T v) { if (v < x) x = v; }
+  OPS_HD static void cinc(T &x, T v) { x += v; }
+#endif
 #ifdef OPS_SOA
   OPS_HD int step() const { return 1; }
   OPS_HD long off(int d) const { return (long)d * plane(); }

@@ -51,6 +51,9 @@ SIGNATURES = {
         "api_alias": (_i, [_i, _pp, _i, _i, _p, _p]),
         "api_random": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
     },
+    "lowdim": {
+        "lowdim_run": (_i, [_i, _pp, _i, _i, _i, _p]),
+    },
     "mgrid": {
         "mgrid_run": (_i, [_i, _pp, _i, _p, _p, _p, _p, _p]),
     },

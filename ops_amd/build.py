@@ -40,6 +40,7 @@ APP_SOURCES = {
     "tealeaf": ["tealeaf.cpp"],
     "adi": ["adi.cpp"],
     "mgrid": ["mgrid.cpp"],
+    "lowdim": ["lowdim.cpp"],
 }
 
 

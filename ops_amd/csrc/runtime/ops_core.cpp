@@ -227,6 +227,7 @@ static void init_dat_metadata(ops_dat dat, ops_block block, int dim, const int *
   dat->type_size = type_size;
   dat->elem_size = type_size * dim;
   dat->e_dat = 0;
+  dat->edat_acc = OPS_READ;
   for (int n = 0; n < block->dims; n++) {
     if (block_size[n] != 1) {
       dat->size[n] = block_size[n] - d_m[n] + d_p[n];

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -313,6 +314,42 @@ static double transfer_bytes(int dims, const int *start, const int *end, const o
   return size * a->dat->elem_size * ((a->acc == OPS_READ || a->acc == OPS_WRITE) ? 1.0 : 2.0);
 }
 
+// ---- low-dimensional reductions (ops_dummy_singlenode.cpp:204-345): a dat argument with
+// INC/MIN/MAX access into an edge dat with a collapsed dimension starts, the first time
+// that access follows another write, from the reduction's identity over the whole allocation
+
+template <typename T> __global__ __launch_bounds__(256) void lowdim_fill(T *p, size_t n, T v) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename T> static void fill_identity(ops_dat d, int acc) {
+  const T v = acc == OPS_INC ? T(0)
+                             : (acc == OPS_MAX ? std::numeric_limits<T>::lowest() : std::numeric_limits<T>::max());
+  const size_t n = d->mem / sizeof(T);
+  hipLaunchKernelGGL((lowdim_fill<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g_dev.stream, (T *)d->data_d,
+                     n, v);
+  ops_hip_check_launch("low-dim reduction initialisation");
+}
+
+static bool lowdim_reduction(const ops_arg &a) {
+  if (a.argtype != OPS_ARG_DAT || !a.opt || !a.dat || !a.stencil || !a.dat->e_dat) return false;
+  if (a.acc != OPS_INC && a.acc != OPS_MIN && a.acc != OPS_MAX) return false;
+  for (int d = 0; d < a.stencil->dims && d < OPS_MAX_DIM; d++)
+    if (a.dat->size[d] == 1) return true;
+  return false;
+}
+
+static void lowdim_init(const ops_arg &a) {
+  if (!lowdim_reduction(a) || a.dat->edat_acc == a.acc) return;
+  ops_dat d = a.dat;
+  d->edat_acc = a.acc;
+  if (!strcmp(d->type, "double")) fill_identity<double>(d, a.acc);
+  else if (!strcmp(d->type, "float")) fill_identity<float>(d, a.acc);
+  else if (!strcmp(d->type, "int")) fill_identity<int>(d, a.acc);
+  else ops_throw(OPS_NOT_IMPLEMENTED, std::string("low-dim reduction into a dat of type ") + d->type);
+}
+
 extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *name,
                                   ops_block block, int dim, const int *range, ops_arg *args,
                                   int nargs) {
@@ -359,6 +396,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
                   std::string("Error: ops_par_loop called with dat ") + dat->name +
                       " that is locked by ops_dat_get_raw_pointer");
       if (dat->dirty_hd == 1) ops_put_data(dat);
+      lowdim_init(a);
       const int *sst = a.stencil->stride;
       // multigrid stencils (templates/cpp/mpi_openmp/loop_host.cpp.j2:230-237): a restrict
       // argument is read at loop index * mgrid stride (affine: the stride scales the step), a
@@ -598,7 +636,10 @@ extern "C" void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs) {
   }
   for (int i = 0; i < nargs; i++) {
     ops_arg &a = args[i];
-    if (a.argtype == OPS_ARG_DAT && a.opt && a.acc != OPS_READ) a.dat->dirty_hd = 2;
+    if (a.argtype == OPS_ARG_DAT && a.opt && a.acc != OPS_READ) {
+      a.dat->dirty_hd = 2;
+      if (a.dat->e_dat) a.dat->edat_acc = a.acc;
+    }
     if (a.argtype == OPS_ARG_GBL && a.acc != OPS_READ && lp->nblocks > 0)
       ((ops_reduction)a.data)->dirty_d = 1;
   }

@@ -40,6 +40,7 @@ extern "C" void ops_cpu_before_loop(ops_arg *args, int nargs);
 extern "C" void ops_cpu_check_bounds(int dim, const int *range, const ops_arg *args, int nargs);
 extern "C" void ops_cpu_after_loop(ops_arg *args, int nargs);
 extern "C" int ops_cpu_soa(void);
+extern "C" int ops_cpu_lowdim_reduction(const ops_arg *a);
 
 namespace ops_cpu {
 
@@ -157,6 +158,10 @@ void run(std::index_sequence<I...>, void (*kernel)(P...), ops_block block, int d
   nth = omp_get_max_threads();
   if ((long)cnt[1] * cnt[2] < nth) nth = 1;
 #endif
+  // low-dim reductions combine many points into one element: one thread keeps them in
+  // loop order
+  for (int i = 0; i < N; i++)
+    if (ops_cpu_lowdim_reduction(&args[i])) nth = 1;
   const long outer = (long)cnt[1] * cnt[2];
   if (nth == 1) {
     Ctx c[N > 0 ? N : 1];

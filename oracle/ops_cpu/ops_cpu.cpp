@@ -22,6 +22,7 @@
 #include <sys/time.h>
 
 #include <random>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -58,10 +59,38 @@ static char *dup(const char *s) {
 
 extern "C" const char *ops_get_last_error(void) { return g_err.c_str(); }
 extern "C" int ops_cpu_soa(void) { return g ? g->soa : 0; }
+// a dat argument that reduces into an edge dat along a collapsed dimension
+// (ops_dummy_singlenode.cpp:247-252)
+extern "C" int ops_cpu_lowdim_reduction(const ops_arg *a) {
+  if (a->argtype != OPS_ARG_DAT || !a->opt || !a->dat || !a->stencil || !a->dat->e_dat) return 0;
+  if (a->acc != OPS_INC && a->acc != OPS_MIN && a->acc != OPS_MAX) return 0;
+  for (int d = 0; d < a->stencil->dims && d < OPS_MAX_DIM; d++)
+    if (a->dat->size[d] == 1) return 1;
+  return 0;
+}
+
+template <typename T> static void fill_identity(ops_dat d, int acc) {
+  T v = acc == OPS_INC ? T(0) : (acc == OPS_MAX ? std::numeric_limits<T>::lowest() : std::numeric_limits<T>::max());
+  T *p = (T *)d->data;
+  for (size_t i = 0; i < d->mem / sizeof(T); i++) p[i] = v;
+}
+
 extern "C" void ops_cpu_before_loop(ops_arg *args, int n) {
   for (int i = 0; i < n; i++)
     if (args[i].argtype == OPS_ARG_DAT && args[i].opt && args[i].dat->locked_hd)
       fail(OPS_RUNTIME_ERROR, "ops_par_loop on a dat locked by ops_dat_get_raw_pointer");
+  // the first low-dim reduction of a kind since the dat was last written otherwise starts
+  // from the reduction's identity over the whole allocation (ops_dummy_singlenode.cpp:248-345)
+  for (int i = 0; i < n; i++) {
+    if (!ops_cpu_lowdim_reduction(&args[i])) continue;
+    ops_dat d = args[i].dat;
+    if (d->edat_acc == args[i].acc) continue;
+    d->edat_acc = args[i].acc;
+    if (!strcmp(d->type, "double")) fill_identity<double>(d, args[i].acc);
+    else if (!strcmp(d->type, "float")) fill_identity<float>(d, args[i].acc);
+    else if (!strcmp(d->type, "int")) fill_identity<int>(d, args[i].acc);
+    else fail(OPS_NOT_IMPLEMENTED, std::string("low-dim reduction into a dat of type ") + d->type);
+  }
 }
 // Extent check of every dat argument (loop range plus stencil reach, per dimension);
 // an out-of-allocation access raises OPS_INVALID_ARGUMENT, as the device runtime does.
@@ -101,8 +130,10 @@ extern "C" void ops_cpu_check_bounds(int dim, const int *range, const ops_arg *a
 }
 extern "C" void ops_cpu_after_loop(ops_arg *args, int n) {
   for (int i = 0; i < n; i++)
-    if (args[i].argtype == OPS_ARG_DAT && args[i].opt && args[i].acc != OPS_READ)
+    if (args[i].argtype == OPS_ARG_DAT && args[i].opt && args[i].acc != OPS_READ) {
       args[i].dat->dirty_hd = 1;
+      if (args[i].dat->e_dat) args[i].dat->edat_acc = args[i].acc;  // ops_dummy_singlenode.cpp:204-213
+    }
 }
 
 static void dump_all(const char *dir);
