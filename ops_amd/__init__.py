@@ -42,7 +42,8 @@ def app(name: str):
     """lib<name>_hip.so with its extern "C" entry points declared."""
     if name not in _apps:
         runtime()
-        lib = _load(LIB_DIR / f"lib{name}_hip.so")
+        # local: two builds of one application (AoS and SoA) export the same stub names
+        lib = _load(LIB_DIR / f"lib{name}_hip.so", mode=ctypes.RTLD_LOCAL)
         from . import apps as _a
         _a.declare(name, lib)
         _apps[name] = lib

@@ -51,6 +51,12 @@ SIGNATURES = {
         "api_alias": (_i, [_i, _pp, _i, _i, _p, _p]),
         "api_random": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
     },
+    "multicomp": {
+        "multicomp_run": (_i, [_i, _pp, _i, _i, _p]),
+    },
+    "multicomp_soa": {
+        "multicomp_run": (_i, [_i, _pp, _i, _i, _p]),
+    },
     "lowdim": {
         "lowdim_run": (_i, [_i, _pp, _i, _i, _i, _p]),
     },

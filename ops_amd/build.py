@@ -41,6 +41,11 @@ APP_SOURCES = {
     "adi": ["adi.cpp"],
     "mgrid": ["mgrid.cpp"],
     "lowdim": ["lowdim.cpp"],
+    "multicomp": ["multicomp.cpp"],
+}
+# builds of an application's sources with extra defines: name -> (application, defines)
+APP_VARIANTS = {
+    "multicomp_soa": ("multicomp", ["-DOPS_SOA"]),
 }
 
 
@@ -91,9 +96,10 @@ def build_runtime(jobs: int = 8) -> Path:
 def build_app(name: str, jobs: int = 8) -> Path:
     sys.path.insert(0, str(ROOT))
     from ops_amd.translator import translate_app
-    app_dir = APPS / name
-    srcs = [str(app_dir / s) for s in APP_SOURCES[name]]
-    gen = app_dir / "_gen"
+    base, defines = APP_VARIANTS.get(name, (name, []))
+    app_dir = APPS / base
+    srcs = [str(app_dir / s) for s in APP_SOURCES[base]]
+    gen = app_dir / ("_gen" if base == name else f"_gen_{name}")
     res = translate_app(name, srcs, str(gen), str(app_dir))
     obj = PKG / "_build" / name
     obj.mkdir(parents=True, exist_ok=True)
@@ -103,30 +109,33 @@ def build_app(name: str, jobs: int = 8) -> Path:
         k = Path(res["kernels"])
         ko = obj / (k.stem + ".o")
         objs.append(ko)
-        futs.append(ex.submit(_compile_hip, k, ko, [f"-I{app_dir}"]))
+        futs.append(ex.submit(_compile_hip, k, ko, [f"-I{app_dir}", *defines]))
         for h in res["host"]:
             h = Path(h)
             ho = obj / (h.stem + ".o")
             objs.append(ho)
-            futs.append(ex.submit(_compile_cxx, h, ho, [f"-I{app_dir}", "-DOPS_APP_LIB"]))
+            futs.append(ex.submit(_compile_cxx, h, ho, [f"-I{app_dir}", "-DOPS_APP_LIB", *defines]))
         extra = app_dir / "native"
         if extra.exists():
             for s in sorted(extra.glob("*.hip")):
                 so = obj / (s.stem + ".o")
                 objs.append(so)
-                futs.append(ex.submit(_compile_hip, s, so, [f"-I{app_dir}"]))
+                futs.append(ex.submit(_compile_hip, s, so, [f"-I{app_dir}", *defines]))
         for f in futs:
             f.result()
     out = LIB / f"lib{name}_hip.so"
     tmp = LIB / f"lib{name}_hip.so.tmp"
+    # -Bsymbolic: an application's calls to its own stubs bind inside the library, whatever
+    # else is loaded (builds of one application's sources share stub names)
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", *map(str, objs), f"-L{LIB}",
-          "-lops_hip", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
+          "-lops_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic", "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 
 
 def available_apps():
-    return [a for a in APP_SOURCES if all((APPS / a / s).exists() for s in APP_SOURCES[a])]
+    apps = [a for a in APP_SOURCES if all((APPS / a / s).exists() for s in APP_SOURCES[a])]
+    return apps + [v for v, (a, _) in APP_VARIANTS.items() if a in apps]
 
 
 def build_all(apps=None, jobs: int = 8):

@@ -741,16 +741,23 @@ void ops_dev_halo_copy(ops_halo h) {
   halo_side(f, h->from_base, h->from_dir, h->iter_size, fs, &fb);
   halo_side(t, h->to_base, h->to_dir, h->iter_size, ts, &tb);
   int ncomp = f->dim < t->dim ? f->dim : t->dim;
-  // element strides in units of one component (AoS)
+  // element strides in units of one component: AoS, or SoA with each component one
+  // allocation plane further on (ops_host_singlenode.cpp:124-160)
+  const int soa = ops_instance_get()->soa;
+  long long fplane = 1, tplane = 1;
   for (int a = 0; a < OPS_MAX_DIM; a++) {
-    c.sstride[a] = fs[a] * f->dim;
-    c.dstride[a] = ts[a] * t->dim;
+    fplane *= f->size[a];
+    tplane *= t->size[a];
   }
-  c.sbase = fb * f->dim;
-  c.dbase = tb * t->dim;
+  for (int a = 0; a < OPS_MAX_DIM; a++) {
+    c.sstride[a] = soa ? fs[a] : fs[a] * f->dim;
+    c.dstride[a] = soa ? ts[a] : ts[a] * t->dim;
+  }
+  c.sbase = soa ? fb : fb * f->dim;
+  c.dbase = soa ? tb : tb * t->dim;
   c.ncomp = ncomp;
-  c.scomp = 1;
-  c.dcomp = 1;
+  c.scomp = soa ? fplane : 1;
+  c.dcomp = soa ? tplane : 1;
   c.total = pts * ncomp;
   if (pts <= 0) return;
   int fc = type_code(f->type), tc = type_code(t->type);

@@ -533,7 +533,10 @@ extern "C" void ops_halo_transfer(ops_halo_group grp) {
                   cum *= (size_t)d->size[a];
                 }
                 for (int comp = 0; comp < d->dim; comp++) {
-                  char *dp = d->data + doff * d->elem_size + (size_t)comp * d->type_size;
+                  // SoA: component comp a whole allocation plane further on
+                  // (ops_host_singlenode.cpp:124-160)
+                  char *dp = g->soa ? d->data + ((size_t)comp * cum + doff) * d->type_size
+                                    : d->data + doff * d->elem_size + (size_t)comp * d->type_size;
                   char *bp = buf.data() + bo * (size_t)esz + (size_t)comp * (mixed ? st : d->type_size);
                   if (!mixed) {
                     if (pass == 0) memcpy(bp, dp, (size_t)d->type_size);
@@ -637,9 +640,21 @@ static void slab(ops_dat d, char *buf, const int *range, int dir) {
               off += (size_t)(range[2 * a] + c[a] - d->d_m[a]) * cum;
               cum *= (size_t)d->size[a];
             }
-            char *dp = d->data + off * d->elem_size;
-            if (dir == 0) memcpy(buf + k, dp, (size_t)d->elem_size);
-            else memcpy(dp, buf + k, (size_t)d->elem_size);
+            if (g->soa && d->dim > 1) {
+              // component c of the point sits c planes further on; the buffer is packed
+              // with components innermost (ops_util.cpp:378-400)
+              size_t plane = 1;
+              for (int a = 0; a < nd; a++) plane *= (size_t)d->size[a];
+              for (int cc = 0; cc < d->dim; cc++) {
+                char *dp = d->data + (cc * plane + off) * d->type_size;
+                if (dir == 0) memcpy(buf + k + (size_t)cc * d->type_size, dp, (size_t)d->type_size);
+                else memcpy(dp, buf + k + (size_t)cc * d->type_size, (size_t)d->type_size);
+              }
+            } else {
+              char *dp = d->data + off * d->elem_size;
+              if (dir == 0) memcpy(buf + k, dp, (size_t)d->elem_size);
+              else memcpy(dp, buf + k, (size_t)d->elem_size);
+            }
             k += (size_t)d->elem_size;
           }
 }
