@@ -62,6 +62,7 @@ __host__ __device__ inline float bf2f(bf16 h) {
 }
 
 constexpr int NL = 8;  // convolution layers
+constexpr int WG_BANDS = 1024;  // row bands of the weight gradient (partial sums per band)
 // layer table: input channels (source 1 = upsampled, source 2 = same resolution), output
 // channels, resolution level
 struct LayerSpec {
@@ -144,20 +145,64 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   }
 }
 
+// channel-major copy of a layer input for the weight gradient: dst[b][c][y][x] for the
+// concatenation [up(src1), src2] (src1 read at (y/2, x/2)); a workgroup moves a 64-pixel
+// piece of one row through LDS so that both the NHWC reads and the CHW writes are contiguous
+constexpr int TP = 64;
+__global__ __launch_bounds__(256) void k_to_chw(const bf16 *src1, const bf16 *src2, int c1, int c2, int H, int W,
+                                                bf16 *dst) {
+  __shared__ bf16 t[112][TP + 2];
+  const int C = c1 + c2, x0 = blockIdx.x * TP, y = blockIdx.y, b = blockIdx.z;
+  const int Hc = H >> 1, Wc = W >> 1;
+  for (int e = threadIdx.x; e < TP * C; e += 256) {
+    const int px = e / C, c = e % C, x = x0 + px;
+    bf16 v = 0;
+    if (x < W)
+      v = c < c1 ? src1[(((long)b * Hc + (y >> 1)) * Wc + (x >> 1)) * c1 + c]
+                 : src2[(((long)b * H + y) * W + x) * c2 + (c - c1)];
+    t[c][px] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < TP * C; e += 256) {
+    const int c = e / TP, px = e % TP, x = x0 + px;
+    if (x < W) dst[(((long)b * C + c) * H + y) * W + x] = t[c][px];
+  }
+}
+
 struct WgradArgs {
-  const bf16 *dy;    // masked output gradient [B][H][W][cout] (BF16)
-  const bf16 *src1;  // the layer's input sources, as in ConvArgs
-  const bf16 *src2;
-  int c1, c2, cout, H, W, rows, band;  // rows = B*H, band = rows per wavefront
+  const bf16 *dyT;   // masked output gradient, channel-major [B][cout][H][W]
+  const bf16 *xT;    // layer input, channel-major [B][cin][H][W]
+  int cin, cout, H, W, rows, band;  // rows = B*H, band = rows per wavefront
   float *part;       // [nbands][9][cout][cin]
 };
 
-// partial weight gradient of one band of rows for one tap
+// 4 consecutive values of a channel-major row at pixels p, p+1, p+2, p+3, zero outside
+// [0, W): aligned 8-byte loads where the row allows them
+__device__ __forceinline__ s4 row4(const bf16 *row, int p, int W) {
+  if ((W & 3) == 0 && p >= 0 && (p & 3) == 0 && p + 3 < W) return *(const s4 *)(row + p);
+  if ((W & 3) == 0 && p >= 3 && ((p + 1) & 3) == 0 && p + 4 < W) {  // p = 4k - 1: reads p-3 .. p+4
+    const s4 lo = *(const s4 *)(row + p - 3), hi = *(const s4 *)(row + p + 1);
+    return s4{lo[3], hi[0], hi[1], hi[2]};
+  }
+  if ((W & 3) == 0 && p >= 1 && ((p - 1) & 3) == 0 && p + 3 < W) {  // p = 4k + 1: reads p-1 .. p+3
+    const s4 lo = *(const s4 *)(row + p - 1);
+    const short nx = (short)row[p + 3];
+    return s4{lo[1], lo[2], lo[3], nx};
+  }
+  s4 v = s4{0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (p + j >= 0 && p + j < W) v[j] = (short)row[p + j];
+  return v;
+}
+
+// partial weight gradient of one band of rows for one tap:
+// dW[co][tap][ci] += sum over pixels p of dy[p][co] x[p + offset(tap)][ci]
 __global__ __launch_bounds__(64) void k_wgrad(WgradArgs a) {
   const int lane = threadIdx.x, m = lane & 15, kq = lane >> 4;
   const int band = blockIdx.x, tap = blockIdx.y;
   const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-  const int H = a.H, W = a.W, cin = a.c1 + a.c2, Hc = H >> 1, Wc = W >> 1;
+  const int H = a.H, W = a.W, cin = a.cin;
   const int ntc = a.cout >> 4, ntn = cin >> 4;  // ntc <= 4, ntn <= 7
   f4 acc[4][7];
 #pragma unroll
@@ -167,35 +212,21 @@ __global__ __launch_bounds__(64) void k_wgrad(WgradArgs a) {
   const int r0 = band * a.band, r1 = min(a.rows, r0 + a.band);
   for (int r = r0; r < r1; r++) {
     const int b = r / H, y = r % H, ys = y + dy;
+    if (ys < 0 || ys >= H) continue;  // the shifted row is zero padding
     for (int x0 = 0; x0 < W; x0 += 16) {
-      // A = dy^T (cout x pixels): lane holds channel 16 tc + m, pixels x0 + 4 kq + j
+      const int p = x0 + 4 * kq;
+      // A = dy^T (cout x pixels): lane holds channel 16 tc + m, pixels p .. p + 3
       s4 av[4];
 #pragma unroll
-      for (int tc = 0; tc < 4; tc++) {
-        av[tc] = s4{0, 0, 0, 0};
-        if (tc < ntc) {
-#pragma unroll
-          for (int j = 0; j < 4; j++) {
-            const int px = x0 + 4 * kq + j;
-            if (px < W) av[tc][j] = (short)a.dy[(((long)b * H + y) * W + px) * a.cout + 16 * tc + m];
-          }
-        }
-      }
+      for (int tc = 0; tc < 4; tc++)
+        av[tc] = tc < ntc ? row4(a.dyT + (((long)b * a.cout + 16 * tc + m) * H + y) * W, p, W) : s4{0, 0, 0, 0};
 #pragma unroll
       for (int tn = 0; tn < 7; tn++) {
         if (tn >= ntn) continue;
-        // B = shifted input (pixels x cin): lane holds channel 16 tn + m, pixels x0 + 4 kq + j
-        s4 bv = s4{0, 0, 0, 0};
-        const int c = 16 * tn + m;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int xs = x0 + 4 * kq + j + dx;
-          if (x0 + 4 * kq + j < W && xs >= 0 && xs < W && ys >= 0 && ys < H) {
-            const bf16 v = (c < a.c1) ? a.src1[(((long)b * Hc + (ys >> 1)) * Wc + (xs >> 1)) * a.c1 + c]
-                                      : a.src2[(((long)b * H + ys) * W + xs) * a.c2 + (c - a.c1)];
-            bv[j] = (short)v;
-          }
-        }
+        // B = shifted input (pixels x cin): lane holds channel 16 tn + m, pixels p + dx ..;
+        // pixels past the row end carry no gradient (A is zero there), so only the
+        // shifted read itself needs the bounds
+        const s4 bv = row4(a.xT + (((long)b * cin + 16 * tn + m) * H + ys) * W, p + dx, W);
 #pragma unroll
         for (int tc = 0; tc < 4; tc++)
           if (tc < ntc) acc[tc][tn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[tc], bv, acc[tc][tn], 0, 0, 0);
@@ -409,6 +440,8 @@ struct Net {
   float *gin = nullptr; // input gradient of a layer (largest)
   float *gpool = nullptr;
   bf16 *gm = nullptr;   // masked output gradient, BF16
+  bf16 *gmT = nullptr;  // the same, channel-major (weight gradient)
+  bf16 *xT = nullptr;   // a layer's input, channel-major (weight gradient)
   float *gmf = nullptr; // same, FP32
   float *wpart = nullptr;
   size_t wpart_cap = 0;
@@ -448,6 +481,8 @@ void free_acts() {
   dfree(N.gin);
   dfree(N.gpool);
   dfree(N.gm);
+  dfree(N.gmT);
+  dfree(N.xT);
   dfree(N.gmf);
   dfree(N.lpart);
   dfree(N.bpart);
@@ -480,6 +515,8 @@ void ensure_acts(int B, int H, int W) {
   N.gin = dalloc<float>(maxg);
   N.gpool = dalloc<float>(maxg);
   N.gm = dalloc<bf16>(maxg);
+  N.gmT = dalloc<bf16>(maxg);
+  N.xT = dalloc<bf16>(maxg);
   N.gmf = dalloc<float>(maxg);
   N.lpart = dalloc<float>(g1d(pix(0)) + 1);
   N.bpart = dalloc<float>((size_t)BIAS_WG * 64);
@@ -554,17 +591,21 @@ void backward() {
                        N.gm, N.gmf);
     hipLaunchKernelGGL(k_bias_part, dim3(BIAS_WG), dim3(256), 0, st, N.gmf, npix, cout, N.bpart);
     hipLaunchKernelGGL(k_bias_fin, dim3(1), dim3(64), 0, st, N.bpart, cout, N.g + N.boff[l]);
-    // weight gradient
+    // weight gradient from channel-major copies of the masked gradient and the layer input
+    const bf16 *s1, *s2;
+    sources(l, s1, s2);
+    const dim3 tgrid((unsigned)((w + TP - 1) / TP), (unsigned)h, (unsigned)B);
+    hipLaunchKernelGGL(k_to_chw, tgrid, dim3(256), 0, st, nullptr, N.gm, 0, cout, h, w, N.gmT);
+    hipLaunchKernelGGL(k_to_chw, tgrid, dim3(256), 0, st, s1, s2, SPEC[l].c1, SPEC[l].c2, h, w, N.xT);
     WgradArgs wa;
-    wa.dy = N.gm;
-    sources(l, wa.src1, wa.src2);
-    wa.c1 = SPEC[l].c1;
-    wa.c2 = SPEC[l].c2;
+    wa.dyT = N.gmT;
+    wa.xT = N.xT;
+    wa.cin = cin;
     wa.cout = cout;
     wa.H = h;
     wa.W = w;
     wa.rows = B * h;
-    wa.band = std::max(1, (B * h + 255) / 256);
+    wa.band = std::max(1, (B * h + WG_BANDS - 1) / WG_BANDS);
     const int nbands = (B * h + wa.band - 1) / wa.band;
     const size_t need = (size_t)nbands * 9 * cout * cin;
     if (need > N.wpart_cap) {
