@@ -20,6 +20,15 @@ SIGNATURES = {
         "cloverleaf_fetch": (ctypes.c_long, [ctypes.c_char_p, _p]),
         "cloverleaf_finalize": (_i, []),
     },
+    "cloverleaf3d": {
+        "cloverleaf3d_init": (_i, [_i, _pp, ctypes.c_char_p]),
+        "cloverleaf3d_step": (_i, [_i]),
+        "cloverleaf3d_run": (_i, [_i]),
+        "cloverleaf3d_summary": (_i, [_p]),
+        "cloverleaf3d_state": (_i, [_p]),
+        "cloverleaf3d_fetch": (ctypes.c_long, [ctypes.c_char_p, _p]),
+        "cloverleaf3d_finalize": (_i, []),
+    },
     "tealeaf": {
         "tealeaf_init": (_i, [_i, _pp, ctypes.c_char_p]),
         "tealeaf_set_solver": (_i, [_i]),

@@ -42,6 +42,7 @@ APP_SOURCES = {
     "mgrid": ["mgrid.cpp"],
     "lowdim": ["lowdim.cpp"],
     "multicomp": ["multicomp.cpp"],
+    "cloverleaf3d": ["clover3d.cpp"],
 }
 # builds of an application's sources with extra defines: name -> (application, defines)
 APP_VARIANTS = {

@@ -22,6 +22,11 @@
 #include <vector>
 
 #include "ops_internal.h"
+#include "ops_macros.h"
+
+// host-side home of the API v1 accessor sizes: translated kernels receive their sizes as
+// a kernel parameter, so this is only referenced by the host copies of user kernels
+struct ops_v1_shape ops_v1;
 
 static OPS_instance *g_inst = nullptr;
 static std::string g_last_error;

@@ -118,6 +118,7 @@ class FuncDef:
     end: int          # one past the closing brace
     params: List[str]
     body_open: int
+    params_close: int = -1   # index of the ')' closing the parameter list
 
 
 _FUNC = re.compile(r"(?m)^[ \t]*((?:static\s+|inline\s+)*void)\s+([A-Za-z_]\w*)\s*\(")
@@ -143,7 +144,7 @@ def find_functions(text: str, masked: Optional[str] = None) -> List[FuncDef]:
         params = [text[s:e].strip() for s, e in split_top(masked, op + 1, cl)]
         if params == ["void"]:
             params = []
-        out.append(FuncDef(m.group(2), m.start(), be + 1, params, j))
+        out.append(FuncDef(m.group(2), m.start(), be + 1, params, j, cl))
     return out
 
 

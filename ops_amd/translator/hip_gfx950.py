@@ -138,6 +138,24 @@ def kernel_headers(app_dir: str, sources: List[str]) -> Dict[str, Tuple[str, cpa
     return out
 
 
+def is_v1_kernel(text: str, f: cparse.FuncDef) -> bool:
+    """API v1 kernel: only pointer parameters, dats addressed with the OPS_ACC<n> macros
+    (directly or through macros of the application's own)."""
+    return not any(re.search(r"\bACC\s*<", p) for p in f.params)
+
+
+def add_v1_param(text: str, f: cparse.FuncDef) -> str:
+    """Append the by-value `ops_v1_shape ops_v1` parameter that carries the sizes behind
+    the OPS_ACC<n> macros (it shadows the host-side global of ops_macros.h)."""
+    head = text[:f.params_close]
+    sep = ", " if f.params else ""
+    if not f.params:
+        # `void k()` or `void k(void)`: replace the (possibly void) list
+        op = text.rindex("(", f.start, f.params_close)
+        head = text[:op + 1]
+    return head + sep + "ops_v1_shape ops_v1" + text[f.params_close:]
+
+
 # ------------------------------------------------------------------ code generation
 
 def const_dim(a: Arg, loop: Loop) -> str:
@@ -164,7 +182,7 @@ def loop_vx(loop: Loop) -> int:
     return VX
 
 
-def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
+def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False) -> str:
     k = loop.stub
     L = []
     # kernel-argument struct
@@ -187,11 +205,11 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int) -> str:
         # the same body twice: with every dat as a no-alias pointer parameter (the usual
         # launch: loads need not wait behind the stores of other arguments), and reading
         # the pointers from the struct (a dat passed to two arguments, one of them writing)
-        L.append(kernel_body(loop, params, ndim, k, True))
+        L.append(kernel_body(loop, params, ndim, k, True, v1))
         L.append("")
-        L.append(kernel_body(loop, params, ndim, k + "_alias", False))
+        L.append(kernel_body(loop, params, ndim, k + "_alias", False, v1))
     else:
-        L.append(kernel_body(loop, params, ndim, k, False))
+        L.append(kernel_body(loop, params, ndim, k, False, v1))
     return "\n".join(L)
 
 
@@ -201,7 +219,8 @@ def noalias_variant(loop: Loop) -> bool:
     return len(dats) >= 2 and any(a.acc != "OPS_READ" for a in dats)
 
 
-def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: bool) -> str:
+def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: bool,
+                v1: bool = False) -> str:
     k = loop.stub
     L = []
     red = [i for i, a in enumerate(loop.args) if a.kind == "reduce"]
@@ -227,13 +246,21 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
     else:
         L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
     L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
+    if v1:
+        # sizes behind OPS_ACC<n>(...) for the dat arguments (xdim<n> = size[0], ...)
+        L.append("    ops_v1_shape v1s;")
+        for i, a in enumerate(loop.args):
+            if a.kind == "dat":
+                L.append(f"    v1s.xd[{i}] = p.xd{i}; v1s.yd[{i}] = p.yd{i}; "
+                         f"v1s.zd[{i}] = p.zd{i}; v1s.md[{i}] = p.md{i};")
     call = []
     for i, a in enumerate(loop.args):
         pr = params[i]
         if a.kind == "dat":
-            if not pr.is_acc:
+            if not pr.is_acc and not v1:
                 raise TranslateError(f"{loop.file}:{loop.line}: kernel {loop.kernel} parameter {i} "
-                                     "must be ACC<T>& for an ops_arg_dat argument")
+                                     "must be ACC<T>& (or a pointer addressed with OPS_ACC<n>) "
+                                     "for an ops_arg_dat argument")
             t = pr.ctype
             off = f"x * p.sx{i} + y * p.sy{i} + z * p.sz{i}"
             if a.prolong:
@@ -242,6 +269,10 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
                 off = (f"((p.st0 + x) / p.mx{i}) * p.sx{i} + ((p.st1 + y) / p.my{i}) * p.sy{i}"
                        f" + ((p.st2 + z) / p.mz{i}) * p.sz{i}")
             ptr = f"{'q' if noalias else 'p.p'}{i} + ({off})"
+            if not pr.is_acc:
+                # API v1: the pointer at this point; OPS_ACC<n> adds the stencil offset
+                call.append(f"({t} *)({ptr})")
+                continue
             mlit = cparse.int_literal(a.dim)
             if mlit == 1:
                 if ndim == 1:
@@ -268,6 +299,8 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
         else:
             L.append(f"    int idx{i}[3] = {{p.st0 + x, p.st1 + y, p.st2 + z}};")
             call.append(f"idx{i}")
+    if v1:
+        call.append("v1s")
     L.append(f"    {loop.kernel}({', '.join(call)});")
     L.append("  }")
     if vx > 1:
@@ -386,9 +419,17 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
         h = headers[lp.kernel][0]
         if h not in used_headers:
             used_headers.append(h)
+    used_kernels = {lp.kernel for lp in loops_all}
+    v1_kernels = set()
     hoisted, bodies = [], []
     for h in used_headers:
         text = Path(h).read_text()
+        # API v1 kernels take the accessor sizes as one more (by-value) parameter
+        fdefs = [f for f in cparse.find_functions(text)
+                 if f.name in used_kernels and is_v1_kernel(text, f)]
+        for f in sorted(fdefs, key=lambda f: f.start, reverse=True):
+            text = add_v1_param(text, f)
+            v1_kernels.add(f.name)
         keep = []
         for line in text.splitlines():
             if re.match(r"\s*#\s*include\b", line):
@@ -406,7 +447,8 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
          "#include <math.h>",
          "#include \"ops_lib_core.h\"",
          "#include \"ops_hip_rt.h\"",
-         "#include \"ops_hip_kernel.h\""]
+         "#include \"ops_hip_kernel.h\"",
+         "#include \"ops_macros.h\""]
     if (Path(app_dir) / "user_types.h").exists():
         H.append("#include \"user_types.h\"")
     H += hoisted
@@ -438,7 +480,7 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
         if len(params) != len(lp.args):
             raise TranslateError(f"{lp.file}:{lp.line}: kernel {lp.kernel} takes {len(params)} parameters "
                                  f"but the loop passes {len(lp.args)} arguments")
-        H.append(gen_kernel(lp, params, ndim))
+        H.append(gen_kernel(lp, params, ndim, lp.kernel in v1_kernels))
         H.append("")
         stub_defs.append(gen_stub(lp, params, ns))
     H.append(f"}}  // namespace {ns}")

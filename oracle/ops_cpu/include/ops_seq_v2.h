@@ -9,6 +9,10 @@
  *   - ops_arg_gbl READ passes the user's pointer, ops_arg_reduce the handle's running
  *     value (:117-126), ops_arg_idx the loop index (:127-134);
  *   - writes mark the dat host-dirty (:299-301).
+ * Kernels written against API v1 (plain pointer parameters, ops/c/include/ops_seq.h:129-296)
+ * receive the element pointer at the current point; the sizes behind their OPS_ACC<n>
+ * macros (size[0..2] and the component count of each dat argument) are set in `ops_v1`
+ * before the loop.
  * Multigrid stencils follow the reference's generated loops, which the sequential header
  * does not implement (templates/cpp/mpi_openmp/loop_host.cpp.j2:230-237): a restrict
  * argument is addressed at loop index * mgrid stride, a prolong argument at loop index /
@@ -30,6 +34,7 @@
 #include <vector>
 
 #include "ops_lib_core.h"
+#include "ops_macros.h"
 #include "ops_tridiag.h"
 
 #ifdef _OPENMP
@@ -54,54 +59,66 @@ struct Ctx {
 
 template <typename P> struct Param;
 
+// addressing of one dat argument at the loop start (ops_seq_v2.h:55-62 of the reference)
+struct DatSt {
+  char *base;
+  long sx, sy, sz;
+  int xd, yd, zd, md;
+  int prolong, mg[3], st[3];
+};
+
+inline DatSt dat_prepare(const ops_arg &a, const int *start, int ndim) {
+  DatSt s{};
+  if (a.argtype != OPS_ARG_DAT || !a.opt) return s;
+  ops_dat d = a.dat;
+  int soa = ops_cpu_soa();
+  long esz = soa ? d->type_size : d->elem_size;
+  const int type = a.stencil->type;  // 0 regular, 1 prolong, 2 restrict
+  int st[3] = {1, 1, 1};
+  for (int i = 0; i < 3; i++) s.mg[i] = 1;
+  for (int i = 0; i < ndim && i < 3; i++) {
+    const int m = (type != 0 && i < a.stencil->dims) ? a.stencil->mgrid_stride[i] : 1;
+    st[i] = a.stencil->stride[i] * (type == 2 ? m : 1);
+    s.mg[i] = type == 1 ? m : 1;
+    s.st[i] = start[i];
+  }
+  s.prolong = type == 1;
+  long off = 0, cum = 1;
+  for (int i = 0; i < ndim; i++) {
+    const long first = type == 1 ? 0 : (long)start[i] * st[i];
+    off += esz * cum * (first - d->base[i] - d->d_m[i]);
+    cum *= d->size[i];
+  }
+  s.base = d->data + off;
+  long comp = soa ? 1 : d->dim;
+  s.sx = st[0] * comp;
+  s.sy = ndim > 1 ? (long)st[1] * d->size[0] * comp : 0;
+  s.sz = ndim > 2 ? (long)st[2] * d->size[0] * d->size[1] * comp : 0;
+  s.xd = d->size[0];
+  s.yd = d->size[1];
+  s.zd = d->size[2];
+  s.md = d->dim;
+  return s;
+}
+
+// element pointer of a dat argument at loop-local index lc
+template <typename T> inline T *dat_at(const DatSt &s, const int *lc) {
+  if (!s.base) return nullptr;
+  T *b = (T *)s.base;
+  if (s.prolong)
+    return b + ((s.st[0] + lc[0]) / s.mg[0]) * s.sx + ((s.st[1] + lc[1]) / s.mg[1]) * s.sy +
+           ((s.st[2] + lc[2]) / s.mg[2]) * s.sz;
+  return b + (lc[0] * s.sx + lc[1] * s.sy + lc[2] * s.sz);
+}
+
 template <typename T> struct Param<ACC<T>> {
-  struct St {
-    T *base;
-    long sx, sy, sz;
-    int xd, yd, zd, md;
-    int prolong, mg[3], st[3];
-  };
+  using St = DatSt;
   static St prepare(const ops_arg &a, const int *start, int ndim) {
-    St s{};
-    if (a.argtype != OPS_ARG_DAT || !a.opt) return s;
-    ops_dat d = a.dat;
-    int soa = ops_cpu_soa();
-    long esz = soa ? d->type_size : d->elem_size;
-    const int type = a.stencil->type;  // 0 regular, 1 prolong, 2 restrict
-    int st[3] = {1, 1, 1};
-    for (int i = 0; i < 3; i++) s.mg[i] = 1;
-    for (int i = 0; i < ndim && i < 3; i++) {
-      const int m = (type != 0 && i < a.stencil->dims) ? a.stencil->mgrid_stride[i] : 1;
-      st[i] = a.stencil->stride[i] * (type == 2 ? m : 1);
-      s.mg[i] = type == 1 ? m : 1;
-      s.st[i] = start[i];
-    }
-    s.prolong = type == 1;
-    long off = 0, cum = 1;
-    for (int i = 0; i < ndim; i++) {
-      const long first = type == 1 ? 0 : (long)start[i] * st[i];
-      off += esz * cum * (first - d->base[i] - d->d_m[i]);
-      cum *= d->size[i];
-    }
-    s.base = (T *)(d->data + off);
-    long comp = soa ? 1 : d->dim;
-    s.sx = st[0] * comp;
-    s.sy = ndim > 1 ? (long)st[1] * d->size[0] * comp : 0;
-    s.sz = ndim > 2 ? (long)st[2] * d->size[0] * d->size[1] * comp : 0;
-    s.xd = d->size[0];
-    s.yd = d->size[1];
-    s.zd = d->size[2];
-    s.md = d->dim;
-    return s;
+    return dat_prepare(a, start, ndim);
   }
   static ACC<T> &get(const St &s, const ops_arg &, Ctx &c, const int *lc, const int *, int) {
     static_assert(sizeof(ACC<T>) <= sizeof(c.acc), "ACC view does not fit its slot");
-    T *p = nullptr;
-    if (s.base && s.prolong)
-      p = s.base + ((s.st[0] + lc[0]) / s.mg[0]) * s.sx + ((s.st[1] + lc[1]) / s.mg[1]) * s.sy +
-          ((s.st[2] + lc[2]) / s.mg[2]) * s.sz;
-    else if (s.base)
-      p = s.base + (lc[0] * s.sx + lc[1] * s.sy + lc[2] * s.sz);
+    T *p = dat_at<T>(s, lc);
 #if defined(OPS_1D)
     return *(s.md == 1 ? new (c.acc) ACC<T>(p) : new (c.acc) ACC<T>(s.md, s.xd, p));
 #elif defined(OPS_2D)
@@ -113,11 +130,16 @@ template <typename T> struct Param<ACC<T>> {
   }
 };
 
+// plain pointers: API v1 dats (the pointer at the current point, addressed with the
+// OPS_ACC<n> macros; ops_seq.h:129-296 of the reference), gbl/reduce data and ops_arg_idx
 template <typename T> struct Param<T *> {
-  struct St {};
-  static St prepare(const ops_arg &, const int *, int) { return St{}; }
-  static T *get(const St &, const ops_arg &a, Ctx &c, const int *lc, const int *start,
+  using St = DatSt;
+  static St prepare(const ops_arg &a, const int *start, int ndim) {
+    return dat_prepare(a, start, ndim);
+  }
+  static T *get(const St &s, const ops_arg &a, Ctx &c, const int *lc, const int *start,
                 int direct) {
+    if (a.argtype == OPS_ARG_DAT) return dat_at<T>(s, lc);
     if (a.argtype == OPS_ARG_IDX) {
       for (int d = 0; d < 3; d++) c.idx[d] = start[d] + lc[d];
       return (T *)c.idx;
@@ -150,6 +172,14 @@ void run(std::index_sequence<I...>, void (*kernel)(P...), ops_block block, int d
   ops_cpu_check_bounds(dim, range, args, N);
   ops_cpu_before_loop(args, N);
   auto st = std::make_tuple(Param<typename Strip<P>::type>::prepare(args[I], start, dim)...);
+  // sizes behind the OPS_ACC<n> macros of API v1 kernels (xdim<n> = size[0], ...)
+  for (int i = 0; i < N && i < OPS_V1_MAXARGS; i++)
+    if (args[i].argtype == OPS_ARG_DAT) {
+      ops_v1.xd[i] = args[i].dat->size[0];
+      ops_v1.yd[i] = args[i].dat->size[1];
+      ops_v1.zd[i] = args[i].dat->size[2];
+      ops_v1.md[i] = args[i].dat->dim;
+    }
   bool has_red = false;
   for (int i = 0; i < N; i++)
     if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) has_red = true;

@@ -28,6 +28,10 @@
 
 #define OPS_2D
 #include "ops_seq_v2.h"
+#include "ops_macros.h"
+
+// API v1 accessor sizes, filled by the loop driver (ops_seq_v2.h) before each loop
+struct ops_v1_shape ops_v1;
 
 struct OPS_instance {
   int soa = 0;

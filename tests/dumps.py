@@ -16,7 +16,7 @@ def read_dump(d):
         size = [int(x) for x in f[4:9]]
         xpad = int(f[24])
         dt = {8: np.float64, 4: np.float32}.get(tsize, np.float64)
-        if name in ("xx", "yy"):
+        if name in ("xx", "yy", "zz"):
             dt = np.int32
         raw = np.fromfile(d / f"{name}.bin", dtype=dt)
         shape = tuple(reversed(size[:nd]))

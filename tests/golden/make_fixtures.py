@@ -2,7 +2,7 @@
 builds (oracle/_ref, made by `make -f oracle/Makefile.ref`). Run in the build container,
 where /root/reference exists; the fixtures are committed, the binaries are not.
 
-    python tests/golden/make_fixtures.py [laplace] [clover] [tea]
+    python tests/golden/make_fixtures.py [laplace] [clover] [tea] [clover3d]
 
 Each fixture holds inputs (decks / parameters) and outputs: scalar results printed by the
 reference and, for every dat the run declared, its shape and the SHA-256 of its exact
@@ -170,6 +170,62 @@ def tea():
                    "iteration_counts_printed": iters, "tail": out.splitlines()[-4:]})
         (OUT / f"{name}.json").write_text(json.dumps(fx, indent=1))
         print("wrote", name, iters)
+
+
+CLOVER3D_DECKS = {
+    # apps/c/CloverLeaf_3D/clover.in with a 20x18x16 mesh and 10 steps
+    "clover3d_small": ("clover.in", {"x_cells=96": "x_cells=20", "y_cells=96": "y_cells=18",
+                                     "z_cells=96": "z_cells=16", "end_step=87": "end_step=10"}),
+    # four states: two cuboids (one moving), a point state; 24x20x22, 9 steps, dt rising
+    "clover3d_states": ("""*clover
+ state 1 density=0.2 energy=1.0
+ state 2 density=1.0 energy=2.5 geometry=cuboid xmin=0.0 xmax=5.0 ymin=0.0 ymax=2.0 zmin=0.0 zmax=4.0
+ state 3 density=0.5 energy=1.5 xvel=0.3 yvel=-0.2 zvel=0.1 geometry=cuboid xmin=6.0 xmax=9.0 ymin=1.0 ymax=4.0 zmin=2.0 zmax=8.0
+ state 4 density=2.0 energy=3.0 geometry=point xmin=5.0 ymin=5.0 zmin=5.0
+ x_cells=24
+ y_cells=20
+ z_cells=22
+ xmin=0.0
+ ymin=0.0
+ zmin=0.0
+ xmax=10.0
+ ymax=10.0
+ zmax=10.0
+ initial_timestep=0.04
+ max_timestep=0.04
+ timestep_rise=1.5
+ end_step=9
+ end_time=100.0
+ test_problem 0
+ checkpoint_frequency=10
+*endclover
+""", None),
+    # apps/c/CloverLeaf_3D/clover.in verbatim: 96^3, 87 steps (known answer ke 20.0546870878964)
+    "clover3d_bm": ("clover.in", {}),
+}
+
+
+def clover3d(which=None):
+    sys.path.insert(0, str(ROOT / "tests"))
+    from dumps import read_dump, read_reductions as rr
+    names = [which] if which else list(CLOVER3D_DECKS)
+    for name in names:
+        src, edits = CLOVER3D_DECKS[name]
+        if edits is None:
+            deck = src
+        else:
+            deck = Path("/root/reference/apps/c/CloverLeaf_3D", src).read_text()
+            for a, b in edits.items():
+                assert a in deck
+                deck = deck.replace(a, b)
+        with tempfile.TemporaryDirectory() as t:
+            out = run_ref("cloverleaf3d_dev_seq", t, deck=deck, deck_name="clover.in")
+            fx = _dump_fixture(read_dump(t), rr(t))
+            steps = (Path(t) / "clover.out").read_text().count(" Step ")
+        fx.update({"source": "apps/c/CloverLeaf_3D (reference dev_seq)", "deck": deck,
+                   "steps": steps, "tail": out.splitlines()[-6:]})
+        (OUT / f"{name}.json").write_text(json.dumps(fx, indent=1))
+        print("wrote", name, steps, "steps")
 
 
 if __name__ == "__main__":

@@ -1,6 +1,7 @@
 """Multi-component dats, AoS and SoA builds: a 2- and a 3-component dat set from their
 indices, a stencil reading neighbours of different components, a sum and a max over
-components, a periodic halo transfer of a 2-component dat; against numpy, and the device
+components, a periodic halo transfer of a 2-component dat, and the stencil again from an
+API v1 kernel (pointer arguments, OPS_ACC_MD<n> accessors); against numpy, and the device
 against the oracle bit for bit."""
 import numpy as np
 import pytest
@@ -14,7 +15,8 @@ APPS = ["multicomp", "multicomp_soa"]
 def run(app, kind, nx, ny):
     L = lib(app, kind)
     n = nx * ny
-    out = np.full(n * 2 + n * 3 + (nx - 2) * (ny - 2) + 2 + (nx + 2) * ny * 2, np.nan)
+    m = (nx - 2) * (ny - 2)
+    out = np.full(n * 2 + n * 3 + m + 2 + (nx + 2) * ny * 2 + m, np.nan)
     argc, av = argv()
     assert L.multicomp_run(argc, av, nx, ny, p(out)) == 0
     return out
@@ -36,7 +38,8 @@ def test_multicomp(app, kind, shape):
     assert np.array_equal(out[k:k + (nx - 2) * (ny - 2)], st.ravel()); k += (nx - 2) * (ny - 2)
     assert out[k] == v.sum() and out[k + 1] == w[..., 2].max(); k += 2
     wide = np.concatenate([v[:, -1:], v, v[:, :1]], axis=1)            # periodic along x
-    assert np.array_equal(out[k:], wide.ravel())
+    assert np.array_equal(out[k:k + wide.size], wide.ravel()); k += wide.size
+    assert np.array_equal(out[k:], st.ravel())                         # API v1 kernel
 
 
 @pytest.mark.gpu
