@@ -128,6 +128,16 @@ def roofline(stats):
                "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_ms * 1e3, 2)}
 
 
+def top_kernels(stats, k):
+    """The k kernels with the most device time: share of time, calls, GB/s (algorithmic)."""
+    tot = sum(e["ms"] for e in stats.values()) or 1.0
+    rows = sorted(stats.items(), key=lambda kv: -kv[1]["ms"])[:k]
+    return [{"kernel": n, "time_frac": round(e["ms"] / tot, 3), "calls": e["calls"],
+             "avg_us": round(e["ms"] * 1e3 / e["calls"], 2),
+             "GBps": round(e["bytes"] / (e["ms"] * 1e-3) / 1e9, 1) if e["ms"] else None}
+            for n, e in rows]
+
+
 def whole_step_bw(stats, seconds):
     tot_bytes = sum(e["bytes"] for e in stats.values())
     tot_ms = sum(e["ms"] for e in stats.values())
@@ -318,6 +328,81 @@ def adi_gpu(args):
             "algorithmic_GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def clover3d_gpu(args, R):
+    """CloverLeaf 3D (apps/c/CloverLeaf_3D, an API v1 application) on the reference's deck
+    clover.in (96^3, 87 steps, known answer ke 20.0546870878964) and on a larger mesh of
+    the same problem (--c3-cells^3, --c3-steps steps) that fills the GPU. A warm-up session
+    runs first (the autotuner picks workgroup shapes there); the timed session starts from
+    the generated state with every dat resident in HBM."""
+    import ops_amd
+    from ops_amd.apps.cloverleaf3d.decks import GOLD_KE_BM, bm_deck
+    L = ops_amd.app("cloverleaf3d")
+    out = {}
+    for tag, n, steps in (("deck_96", 96, 87), ("large", args.c3_cells, args.c3_steps)):
+        with tempfile.TemporaryDirectory() as t:
+            deck = Path(t) / "clover.in"
+            deck.write_text(bm_deck(n, steps))
+            argc, argv = ops_amd.argv()
+            log(f"[bench] cloverleaf3d {n}^3: warm-up")
+            assert L.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
+            assert L.cloverleaf3d_step(min(args.warmup, 12)) == 0
+            assert L.cloverleaf3d_finalize() == 0
+            assert L.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
+            R.ops_hip_device_sync()
+            R.ops_hip_reset_stats()
+            R.ops_hip_set_timing(1)
+            log(f"[bench] cloverleaf3d {n}^3: timed {steps} steps")
+            t0 = time.perf_counter()
+            rc = L.cloverleaf3d_step(steps)
+            R.ops_hip_device_sync()
+            t1 = time.perf_counter()
+            R.ops_hip_set_timing(0)
+            assert rc == 0
+            stats = ops_amd.kernel_stats()
+            s = (ctypes.c_double * 5)()
+            assert L.cloverleaf3d_summary(s) == 0
+            assert L.cloverleaf3d_finalize() == 0
+        secs = t1 - t0
+        e = {"workload": f"cloverleaf_3d clover.in {n}^3 cells, {steps} hydro steps",
+             "s": round(secs, 4), "ms_per_step": round(secs * 1e3 / steps, 3),
+             "cell_updates_per_s": n ** 3 * steps / secs,
+             "step_bandwidth": whole_step_bw(stats, secs),
+             "dominant_kernel": roofline(stats)[1], "ke": s[3],
+             "launches_per_step": round(sum(v["calls"] for v in stats.values()) / steps, 1),
+             "top_kernels": top_kernels(stats, 8)}
+        if n == 96 and steps == 87:
+            e["ke_golden"] = GOLD_KE_BM
+            e["ke_pct_diff"] = abs(100.0 * (s[3] / GOLD_KE_BM) - 100.0)
+            e["ke_pass_0.001pct"] = e["ke_pct_diff"] < 0.001
+        out[tag] = e
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        lib = ctypes.CDLL(str(ROOT / "oracle" / "_build" / "libcloverleaf3d_cpu.so"),
+                          mode=ctypes.RTLD_LOCAL)
+        lib.ops_cpu_set_threads.argtypes = [ctypes.c_int]
+        lib.ops_cpu_set_threads(threads)
+        lib.cloverleaf3d_init.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                          ctypes.c_char_p]
+        lib.cloverleaf3d_step.argtypes = [ctypes.c_int]
+        steps = 20
+        with tempfile.TemporaryDirectory() as t:
+            deck = Path(t) / "clover.in"
+            deck.write_text(bm_deck(96, 87))
+            argc, argv = ops_amd.argv()
+            assert lib.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
+            assert lib.cloverleaf3d_step(1) == 0
+            t0 = time.perf_counter()
+            assert lib.cloverleaf3d_step(steps) == 0
+            t1 = time.perf_counter()
+            assert lib.cloverleaf3d_finalize() == 0
+        out["cpu_port"] = {"value": 96 ** 3 * steps / (t1 - t0), "unit": "cell-updates/s",
+                           "cores": threads, "kind": "port",
+                           "sample": f"CloverLeaf 3D 96^3, steps 2..{steps + 1} of clover.in on "
+                                     f"the oracle host backend ({threads} threads), "
+                                     f"{t1 - t0:.2f} s"}
+    return out
+
+
 def clover_cpu(args, threads):
     """Oracle host backend, same application and deck, bounded sample of steps."""
     lib = ctypes.CDLL(str(ROOT / "oracle" / "_build" / "libcloverleaf_cpu.so"),
@@ -366,6 +451,9 @@ def main():
     ap.add_argument("--adi-steps", type=int, default=100)
     ap.add_argument("--adi-eager", action="store_true",
                     help="time eager launches instead of the captured step (for profilers)")
+    ap.add_argument("--no-clover3d", action="store_true")
+    ap.add_argument("--c3-cells", type=int, default=320)
+    ap.add_argument("--c3-steps", type=int, default=20)
     ap.add_argument("--tea-cells", type=int, default=4096)
     ap.add_argument("--tea-steps", type=int, default=10)
     ap.add_argument("--ingest-pmc", metavar="DIR", default=None,
@@ -419,6 +507,8 @@ def main():
         out["tealeaf_learned"] = tealeaf_learned_gpu(args)
     if not args.no_adi:
         out["adi"] = adi_gpu(args)
+    if not args.no_clover3d:
+        out["cloverleaf_3d"] = clover3d_gpu(args, R)
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] cpu baseline: {threads} threads, {args.cpu_steps} steps")
