@@ -164,22 +164,46 @@ def clover_gpu(args, R):
         assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
         # measured HBM bandwidth: a plain 1 GiB device copy (also the counter calibration)
         checks_copy = R.ops_hip_copy_probe(1 << 30, 10)
+        graph = not args.no_graph
+        if graph:
+            # capture the two step graphs before the timed region (dt stays on the GPU)
+            L.cloverleaf_set_graph(1)
+            assert L.cloverleaf_capture() == 0
         R.ops_hip_device_sync()
-        R.ops_hip_reset_stats()
-        R.ops_hip_set_timing(1)
-        log(f"[bench] timed: {args.steps} steps")
+        log(f"[bench] timed: {args.steps} steps ({'captured' if graph else 'eager'})")
+        if not graph:
+            R.ops_hip_reset_stats()
+            R.ops_hip_set_timing(1)
         t0 = time.perf_counter()
         rc = L.cloverleaf_step(args.steps)
         R.ops_hip_device_sync()
         t1 = time.perf_counter()
         R.ops_hip_set_timing(0)
+        L.cloverleaf_set_graph(0)
         assert rc == 0
-        stats = ops_amd.kernel_stats()
         s = (ctypes.c_double * 5)()
         assert L.cloverleaf_summary(s) == 0
         assert L.cloverleaf_finalize() == 0
+        eager_s = None
+        if graph:
+            # per-kernel durations (HIP events around every launch) from an eager pass of
+            # the same steps: the replayed graphs run the same kernels
+            log(f"[bench] eager pass with per-kernel timing: {args.steps} steps")
+            assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+            R.ops_hip_device_sync()
+            R.ops_hip_reset_stats()
+            R.ops_hip_set_timing(1)
+            e0 = time.perf_counter()
+            assert L.cloverleaf_step(args.steps) == 0
+            R.ops_hip_device_sync()
+            eager_s = time.perf_counter() - e0
+            R.ops_hip_set_timing(0)
+            assert L.cloverleaf_finalize() == 0
+        stats = ops_amd.kernel_stats()
     secs = t1 - t0
-    checks = {"ke": s[3], "hbm_copy_GBps": round(checks_copy, 1)}
+    checks = {"ke": s[3], "hbm_copy_GBps": round(checks_copy, 1), "captured_steps": graph}
+    if eager_s is not None:
+        checks["eager_ms_per_step_with_kernel_timing"] = round(eager_s * 1e3 / args.steps, 3)
     if args.steps == 87 and n == 3840:
         checks["ke_golden"] = GOLD_KE_BM16
         checks["ke_pct_diff"] = abs(100.0 * (s[3] / GOLD_KE_BM16) - 100.0)
@@ -347,21 +371,39 @@ def clover3d_gpu(args, R):
             assert L.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
             assert L.cloverleaf3d_step(min(args.warmup, 12)) == 0
             assert L.cloverleaf3d_finalize() == 0
+            graph = not args.no_graph
             assert L.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
+            if graph:
+                L.cloverleaf3d_set_graph(1)
+                assert L.cloverleaf3d_capture() == 0
             R.ops_hip_device_sync()
-            R.ops_hip_reset_stats()
-            R.ops_hip_set_timing(1)
+            if not graph:
+                R.ops_hip_reset_stats()
+                R.ops_hip_set_timing(1)
             log(f"[bench] cloverleaf3d {n}^3: timed {steps} steps")
             t0 = time.perf_counter()
             rc = L.cloverleaf3d_step(steps)
             R.ops_hip_device_sync()
             t1 = time.perf_counter()
             R.ops_hip_set_timing(0)
+            L.cloverleaf3d_set_graph(0)
             assert rc == 0
-            stats = ops_amd.kernel_stats()
             s = (ctypes.c_double * 5)()
             assert L.cloverleaf3d_summary(s) == 0
             assert L.cloverleaf3d_finalize() == 0
+            eager_s = None
+            if graph:
+                assert L.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
+                R.ops_hip_device_sync()
+                R.ops_hip_reset_stats()
+                R.ops_hip_set_timing(1)
+                e0 = time.perf_counter()
+                assert L.cloverleaf3d_step(steps) == 0
+                R.ops_hip_device_sync()
+                eager_s = time.perf_counter() - e0
+                R.ops_hip_set_timing(0)
+                assert L.cloverleaf3d_finalize() == 0
+            stats = ops_amd.kernel_stats()
         secs = t1 - t0
         e = {"workload": f"cloverleaf_3d clover.in {n}^3 cells, {steps} hydro steps",
              "s": round(secs, 4), "ms_per_step": round(secs * 1e3 / steps, 3),
@@ -369,7 +411,9 @@ def clover3d_gpu(args, R):
              "step_bandwidth": whole_step_bw(stats, secs),
              "dominant_kernel": roofline(stats)[1], "ke": s[3],
              "launches_per_step": round(sum(v["calls"] for v in stats.values()) / steps, 1),
-             "top_kernels": top_kernels(stats, 8)}
+             "top_kernels": top_kernels(stats, 8), "captured_steps": graph}
+        if eager_s is not None:
+            e["eager_ms_per_step_with_kernel_timing"] = round(eager_s * 1e3 / steps, 3)
         if n == 96 and steps == 87:
             e["ke_golden"] = GOLD_KE_BM
             e["ke_pct_diff"] = abs(100.0 * (s[3] / GOLD_KE_BM) - 100.0)
@@ -451,6 +495,8 @@ def main():
     ap.add_argument("--adi-steps", type=int, default=100)
     ap.add_argument("--adi-eager", action="store_true",
                     help="time eager launches instead of the captured step (for profilers)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time CloverLeaf steps as eager launches (default: replayed hipGraphs)")
     ap.add_argument("--no-clover3d", action="store_true")
     ap.add_argument("--c3-cells", type=int, default=320)
     ap.add_argument("--c3-steps", type=int, default=20)

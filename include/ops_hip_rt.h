@@ -104,6 +104,18 @@ int ops_hip_graph_begin(void);
 void *ops_hip_graph_end(void);
 int ops_hip_graph_replay(void *graph, int times);
 void ops_hip_graph_free(void *graph);
+/* Device-side chaining for captured steps (no reference counterpart; the reference reads
+   reductions and uploads constants on the host, ops_lib_core.cpp:1143-1149, and
+   cuda/master_kernel.cpp.j2:40-67):
+   - the device address of a fixed-size ops_decl_const symbol, which a kernel of the
+     application may overwrite (later launches read the new value);
+   - the device address of a reduction handle's running value (the finished result of the
+     last loop that reduced into it);
+   - discard: forget the handle's device value without reading it, as ops_reduction_result
+     would (the next reducing loop overwrites it instead of combining). */
+void *ops_hip_const_device_ptr(const char *name);
+void *ops_hip_reduction_device_ptr(ops_reduction h);
+void ops_hip_reduction_discard(ops_reduction h);
 /* Measured HBM bandwidth: a plain copy of `bytes` between two fresh device buffers,
    `reps` times on the runtime stream; returns (read + write bytes) / time in GB/s, or a
    negative value on error. Allocates and frees its buffers outside the timed region. */

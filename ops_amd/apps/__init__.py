@@ -14,6 +14,8 @@ SIGNATURES = {
     "cloverleaf": {
         "cloverleaf_init": (_i, [_i, _pp, ctypes.c_char_p]),
         "cloverleaf_step": (_i, [_i]),
+        "cloverleaf_set_graph": (_i, [_i]),
+        "cloverleaf_capture": (_i, []),
         "cloverleaf_run": (_i, [_i]),
         "cloverleaf_summary": (_i, [_p]),
         "cloverleaf_state": (_i, [_p]),
@@ -23,6 +25,8 @@ SIGNATURES = {
     "cloverleaf3d": {
         "cloverleaf3d_init": (_i, [_i, _pp, ctypes.c_char_p]),
         "cloverleaf3d_step": (_i, [_i]),
+        "cloverleaf3d_set_graph": (_i, [_i]),
+        "cloverleaf3d_capture": (_i, []),
         "cloverleaf3d_run": (_i, [_i]),
         "cloverleaf3d_summary": (_i, [_p]),
         "cloverleaf3d_state": (_i, [_p]),

@@ -16,6 +16,8 @@
 //   cloverleaf3d_init(argc, argv, deck)   parse the deck, build and generate the problem
 //   cloverleaf3d_step(n)                  advance n hydro steps
 //   cloverleaf3d_run(max_steps)           advance to the deck's end condition
+//   cloverleaf3d_set_graph(on)            replay cloverleaf3d_step's steps from captured
+//                                         hipGraphs (dt computed on the GPU)
 //   cloverleaf3d_summary(out[5])          field summary: volume, mass, ie, ke, pressure
 //   cloverleaf3d_state(out[4])            step, time, dt, end_step
 //   cloverleaf3d_fetch(name, out)         one dat, halos included
@@ -47,6 +49,23 @@ int g_cube = 1, g_sphe = 2, g_point = 3;
 
 #include "clover3d_kernels.h"
 
+// device-side time-step control for captured steps (apps/cloverleaf/native/clover_dt.hip)
+extern "C" {
+void *clover_dt_state_alloc(void);
+void clover_dt_state_free(void *st);
+void clover_dt_state_set(void *st, double dtold, double time, double dt);
+void clover_dt_state_get(void *st, double *out4);
+void clover_dt_launch(const void *local_dt, void *dt_const, void *st, double g_big,
+                      double dtrise, double dtmax, double dtmin);
+int ops_hip_graph_begin(void);
+void *ops_hip_graph_end(void);
+int ops_hip_graph_replay(void *graph, int times);
+void ops_hip_graph_free(void *graph);
+void *ops_hip_const_device_ptr(const char *name);
+void *ops_hip_reduction_device_ptr(ops_reduction h);
+void ops_hip_reduction_discard(ops_reduction h);
+}
+
 namespace {
 
 const int XDIR = 1, YDIR = 2, ZDIR = 3;
@@ -58,6 +77,10 @@ struct Sim {
   // run state
   int step = 0, advect_x = 1;
   double time = 0.0, dtold = 0.0;
+  // captured steps: one graph per sweep order (advect_x), the device dt state
+  int graph = 0;
+  void *step_graph[2] = {nullptr, nullptr};
+  void *dt_state = nullptr;
   // OPS objects
   ops_block blk;
   ops_dat density0, density1, energy0, energy1, pressure, viscosity, soundspeed, volume;
@@ -528,7 +551,7 @@ void viscosity_loop() {
                ops_arg_dat(S.zarea, 1, S.S000, "double", OPS_READ));
 }
 
-double calc_dt() {
+void calc_dt_loops() {
   int in[6];
   ops_par_loop(c3_calc_dt, "calc_dt_kernel", S.blk, 3, inner_range(in),
                ops_arg_dat(S.celldx, 1, S.SX_P, "double", OPS_READ),
@@ -548,13 +571,18 @@ double calc_dt() {
   ops_par_loop(c3_dt_min, "calc_dt_kernel_min", S.blk, 3, in,
                ops_arg_dat(S.work1, 1, S.S000, "double", OPS_READ),
                ops_arg_reduce(S.r_dt, 1, "double", OPS_MIN));
+}
+
+double calc_dt() {
+  calc_dt_loops();
   double local_dt;
   ops_reduction_result(S.r_dt, &local_dt);
   local_dt = MIN(local_dt, g_big);
   return local_dt;
 }
 
-void timestep() {
+// device: dt is computed on the GPU from the reduction's device value (captured steps)
+void timestep(bool device = false) {
   int f[NUM_FIELDS];
   ideal_gas(0);
   // zvel0 is not refreshed here, as in the reference (time_step.cpp:60-80)
@@ -563,6 +591,12 @@ void timestep() {
   viscosity_loop();
   set_fields(f, {FIELD_VISCOSITY});
   update_halo(f, 1);
+  if (device) {
+    calc_dt_loops();
+    clover_dt_launch(ops_hip_reduction_device_ptr(S.r_dt), ops_hip_const_device_ptr("dt"),
+                     S.dt_state, g_big, S.dtrise, S.dtmax, S.dtmin);
+    return;
+  }
   double dtlp = calc_dt();
   if (dtlp < S.dtmin) throw OPSException(OPS_RUNTIME_ERROR, "cloverleaf3d: timestep below dtmin");
   double dt_temp = g_big;
@@ -1037,12 +1071,78 @@ void hydro_step() {
   S.time = S.time + dt;
 }
 
+// the launches of one step with dt, dtold and time kept on the device
+void device_step() {
+  timestep(true);
+  pdv(1);
+  accelerate();
+  pdv(0);
+  flux_calc();
+  advection();
+  reset_field();
+}
+
+// n steps replayed from two captured graphs (one per sweep order); the results are those
+// of hydro_step: the same launches, and dt from the same operations on the same values
+void capture_steps() {
+  if (!S.dt_state) S.dt_state = clover_dt_state_alloc();
+  for (int k = 0; k < 2; k++) {
+    const int ax = S.advect_x ^ k;
+    if (S.step_graph[ax]) continue;
+    const int keep = S.advect_x;
+    S.advect_x = ax;
+    ops_hip_reduction_discard(S.r_dt);  // the captured MIN starts from its own result
+    if (ops_hip_graph_begin())
+      throw OPSException(OPS_RUNTIME_ERROR, "cloverleaf3d: graph capture unavailable");
+    try {
+      device_step();
+    } catch (...) {
+      ops_hip_graph_free(ops_hip_graph_end());  // leave capture mode before reporting
+      S.advect_x = keep;
+      throw;
+    }
+    S.step_graph[ax] = ops_hip_graph_end();
+    ops_hip_reduction_discard(S.r_dt);
+    S.advect_x = keep;
+    if (!S.step_graph[ax])
+      throw OPSException(OPS_RUNTIME_ERROR, "cloverleaf3d: graph capture failed");
+  }
+}
+
+void graph_steps(int n) {
+  capture_steps();
+  clover_dt_state_set(S.dt_state, S.dtold, S.time, dt);
+  for (int i = 0; i < n; i++) {
+    if (ops_hip_graph_replay(S.step_graph[S.advect_x], 1))
+      throw OPSException(OPS_RUNTIME_ERROR, "cloverleaf3d: graph replay failed");
+    S.advect_x = !S.advect_x;
+    S.step++;
+  }
+  double st[4];
+  clover_dt_state_get(S.dt_state, st);
+  S.dtold = st[0];
+  S.time = st[1];
+  dt = st[3];
+  if (st[2] != 0.0) throw OPSException(OPS_RUNTIME_ERROR, "cloverleaf3d: timestep below dtmin");
+}
+
+void free_graphs() {
+  for (void *&g : S.step_graph) {
+    ops_hip_graph_free(g);
+    g = nullptr;
+  }
+  clover_dt_state_free(S.dt_state);
+  S.dt_state = nullptr;
+}
+
 }  // namespace
 
 extern "C" int cloverleaf3d_init(int argc, const char **argv, const char *deck) {
   try {
     ops_init(argc, argv, 1);
+    const int graph = S.graph;
     S = Sim();
+    S.graph = graph;
     read_deck(deck);
     start();
     return 0;
@@ -1052,8 +1152,31 @@ extern "C" int cloverleaf3d_init(int argc, const char **argv, const char *deck) 
   }
 }
 
+// captured steps on (1) or off (0) for cloverleaf3d_step; returns the previous setting
+extern "C" int cloverleaf3d_set_graph(int on) {
+  const int old = S.graph;
+  S.graph = on;
+  return old;
+}
+
+// captures the two step graphs now (otherwise the first captured step does), so that a
+// timed region replays only; 0 on success
+extern "C" int cloverleaf3d_capture(void) {
+  try {
+    capture_steps();
+    return 0;
+  } catch (std::exception &e) {
+    fprintf(stderr, "cloverleaf3d_capture: %s\n", e.what());
+    return 1;
+  }
+}
+
 extern "C" int cloverleaf3d_step(int n) {
   try {
+    if (S.graph && n > 0) {
+      graph_steps(n);
+      return 0;
+    }
     for (int i = 0; i < n; i++) hydro_step();
     return 0;
   } catch (std::exception &e) {
@@ -1125,6 +1248,7 @@ extern "C" long cloverleaf3d_fetch(const char *name, double *out) {
 
 extern "C" int cloverleaf3d_finalize(void) {
   try {
+    free_graphs();
     ops_exit();
     free(states);
     states = NULL;

@@ -44,6 +44,10 @@ APP_SOURCES = {
     "multicomp": ["multicomp.cpp"],
     "cloverleaf3d": ["clover3d.cpp"],
 }
+# native sources one application shares with another: application -> [(app dir, file)]
+APP_SHARED_NATIVE = {
+    "cloverleaf3d": [("cloverleaf", "native/clover_dt.hip")],
+}
 # builds of an application's sources with extra defines: name -> (application, defines)
 APP_VARIANTS = {
     "multicomp_soa": ("multicomp", ["-DOPS_SOA"]),
@@ -117,8 +121,10 @@ def build_app(name: str, jobs: int = 8) -> Path:
             objs.append(ho)
             futs.append(ex.submit(_compile_cxx, h, ho, [f"-I{app_dir}", "-DOPS_APP_LIB", *defines]))
         extra = app_dir / "native"
-        if extra.exists():
-            for s in sorted(extra.glob("*.hip")):
+        natives = sorted(extra.glob("*.hip")) if extra.exists() else []
+        natives += [APPS / a / f for a, f in APP_SHARED_NATIVE.get(base, [])]
+        if natives:
+            for s in natives:
                 so = obj / (s.stem + ".o")
                 objs.append(so)
                 futs.append(ex.submit(_compile_hip, s, so, [f"-I{app_dir}", *defines]))

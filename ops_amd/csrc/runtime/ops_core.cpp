@@ -655,6 +655,29 @@ extern "C" void ops_decl_const_char(OPS_instance *instance, int dim, char const 
   }
 }
 
+// device address of a fixed-size ops_decl_const symbol, for kernels that update a constant
+// on the device (a captured time step cannot upload it from the host)
+extern "C" void *ops_hip_const_device_ptr(const char *name) {
+  auto &tab = ops_const_table();
+  auto it = tab.find(name);
+  if (it == tab.end() || it->second.nbytes < 0)
+    ops_throw(OPS_INVALID_ARGUMENT, std::string("ops_hip_const_device_ptr: no fixed-size "
+                                                "device constant named ") + name);
+  ops_const_entry &e = it->second;
+  if (e.dev == nullptr) e.dev = ops_dev_symbol_address(e.symbol);
+  return e.dev;
+}
+
+extern "C" void *ops_hip_reduction_device_ptr(ops_reduction h) { return h->data_d; }
+
+// drop a reduction handle's device value without reading it back: the next loop that
+// reduces into the handle starts from that loop's own result, as after
+// ops_reduction_result
+extern "C" void ops_hip_reduction_discard(ops_reduction h) {
+  h->dirty_d = 0;
+  h->initialized = 0;
+}
+
 extern "C" void ops_hip_register_const(const char *name, const void *symbol, long nbytes) {
   ops_const_entry &e = ops_const_table()[name];
   e.symbol = symbol;

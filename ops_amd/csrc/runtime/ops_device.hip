@@ -36,6 +36,9 @@ struct DevState {
   // reduction scratch: per-block partials (one region per loop argument) + counters
   char *red_scratch = nullptr;
   size_t red_scratch_cap = 0;
+  // outgrown reduction scratch buffers: a captured graph may still write to them, so they
+  // are only freed at exit
+  std::vector<char *> red_scratch_retired;
   // event timing
   struct Pending {
     int kid;
@@ -111,6 +114,8 @@ void ops_dev_exit(OPS_instance *inst) {
   for (hipEvent_t ev : g_dev.pool) (void)hipEventDestroy(ev);
   g_dev.pool.clear();
   if (g_dev.red_scratch) (void)hipFree(g_dev.red_scratch);
+  for (char *p : g_dev.red_scratch_retired) (void)hipFree(p);
+  g_dev.red_scratch_retired.clear();
   g_dev.red_scratch = nullptr;
   g_dev.red_scratch_cap = 0;
   (void)hipHostFree(g_dev.stage);
@@ -525,10 +530,11 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       need += ((size_t)lp->nblocks * (size_t)h->size + 255) & ~(size_t)255;
     }
   if (need > g_dev.red_scratch_cap) {
-    if (g_dev.red_scratch) {
-      HIPCHECK(hipStreamSynchronize(g_dev.stream));
-      HIPCHECK(hipFree(g_dev.red_scratch));
-    }
+    if (g_capturing)
+      ops_throw(OPS_RUNTIME_ERROR, std::string("ops_par_loop ") + name +
+                                       ": reduction scratch would grow during graph capture "
+                                       "(run the loop once before capturing it)");
+    if (g_dev.red_scratch) g_dev.red_scratch_retired.push_back(g_dev.red_scratch);
     size_t cap = need * 2;
     HIPCHECK(hipMalloc((void **)&g_dev.red_scratch, cap));
     g_dev.red_scratch_cap = cap;

@@ -793,3 +793,16 @@ extern "C" int ops_hip_graph_begin(void) { return 1; }
 extern "C" void *ops_hip_graph_end(void) { return nullptr; }
 extern "C" int ops_hip_graph_replay(void *, int) { return 1; }
 extern "C" void ops_hip_graph_free(void *) {}
+// device-side chaining has no host counterpart: the applications' captured-step mode is
+// GPU only (their host path reads reductions and uploads constants on the host)
+extern "C" void *ops_hip_const_device_ptr(const char *) {
+  fail(OPS_NOT_IMPLEMENTED, "ops_hip_const_device_ptr: not available on the host backend");
+  return nullptr;
+}
+extern "C" void *ops_hip_reduction_device_ptr(ops_reduction) {
+  fail(OPS_NOT_IMPLEMENTED, "ops_hip_reduction_device_ptr: not available on the host backend");
+  return nullptr;
+}
+extern "C" void ops_hip_reduction_discard(ops_reduction h) {
+  h->initialized = 0;
+}

@@ -142,3 +142,46 @@ def test_gpu_config2_known_answer(golden):
             meta = fx["dats"][k]
             arr = got[k].reshape(meta["shape"])
             assert sha(arr) == meta["sha256"], k
+
+
+@pytest.mark.gpu
+def test_gpu_captured_steps_bitwise_equal_reference(golden, tmp_path):
+    """Steps replayed from the two captured hipGraphs (dt computed on the GPU), in chunks
+    separated by eager work (field summaries, state reads, host synchronisations): every
+    dat bitwise equal to the reference after the deck's steps, time and dt equal to the
+    eager run's."""
+    import ops_amd
+    fx = golden("clover_small.json")
+    L = cu.lib("gpu")
+    L.cloverleaf_set_graph(1)
+    try:
+        deck = tmp_path / "clover.in"
+        deck.write_text(fx["deck"])
+        argc, argv = ops_amd.argv(f"OPS_DUMP_DIR={tmp_path}")
+        assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+        left, states = fx["steps"], []
+        for chunk in (3, 1, 5):
+            assert L.cloverleaf_step(chunk) == 0
+            left -= chunk
+            s = np.zeros(5)
+            assert L.cloverleaf_summary(s.ctypes.data_as(ctypes.c_void_p)) == 0
+            st = np.zeros(4)
+            L.cloverleaf_state(st.ctypes.data_as(ctypes.c_void_p))
+            states.append(st)
+        assert L.cloverleaf_step(left) == 0
+        s = np.zeros(5)
+        assert L.cloverleaf_summary(s.ctypes.data_as(ctypes.c_void_p)) == 0
+        assert L.cloverleaf_finalize() == 0
+    finally:
+        L.cloverleaf_set_graph(0)
+    assert check_dump(tmp_path, fx) == []
+    # the eager run's time and dt after the same step counts
+    L.cloverleaf_set_graph(0)
+    argc, argv = ops_amd.argv()
+    assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+    for chunk, st in zip((3, 1, 5), states):
+        assert L.cloverleaf_step(chunk) == 0
+        ref = np.zeros(4)
+        L.cloverleaf_state(ref.ctypes.data_as(ctypes.c_void_p))
+        assert np.array_equal(st, ref)
+    assert L.cloverleaf_finalize() == 0

@@ -162,3 +162,35 @@ def test_gpu_bitwise_equals_reference(golden, tmp_path, name):
     check_summary(s, got, fx, nx, ny, nz)
     if name == "clover3d_bm":
         assert abs(100.0 * (s[3] / GOLD_KE_BM) - 100.0) < 0.001
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["clover3d_small", "clover3d_states"])
+def test_gpu_captured_steps_bitwise_equal_reference(golden, tmp_path, name):
+    """Steps replayed from the two captured hipGraphs (dt computed on the GPU), in chunks
+    separated by eager work (field summaries, state reads, synchronisations): every dat
+    bitwise equal to the reference after the deck's steps."""
+    fx = golden(f"{name}.json")
+    L = lib("gpu")
+    L.cloverleaf3d_set_graph(1)
+    try:
+        deck = tmp_path / "clover.in"
+        deck.write_text(fx["deck"])
+        argc, argv = ops_amd.argv(f"OPS_DUMP_DIR={tmp_path}")
+        assert L.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
+        left = fx["steps"]
+        for chunk in (2, 1, 3):
+            assert L.cloverleaf3d_step(chunk) == 0
+            left -= chunk
+            s = np.zeros(5)
+            assert L.cloverleaf3d_summary(s.ctypes.data_as(ctypes.c_void_p)) == 0
+        assert L.cloverleaf3d_step(left) == 0
+        st = np.zeros(4)
+        L.cloverleaf3d_state(st.ctypes.data_as(ctypes.c_void_p))
+        assert st[0] == fx["steps"]
+        s = np.zeros(5)
+        assert L.cloverleaf3d_summary(s.ctypes.data_as(ctypes.c_void_p)) == 0
+        assert L.cloverleaf3d_finalize() == 0
+    finally:
+        L.cloverleaf3d_set_graph(0)
+    assert check_dump(tmp_path, fx) == []
