@@ -545,6 +545,7 @@ def main():
                    "autotune": not args.no_autotune},
         "roofline": roof,
         "step_bandwidth": whole_step_bw(stats, secs),
+        "top_kernels": top_kernels(stats, 10),
         "checks": checks,
     }
     if not args.no_tealeaf:

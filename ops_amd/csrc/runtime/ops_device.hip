@@ -449,8 +449,12 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   // segment) and the remaining threads stacked along y/z
   auto &kt = ops_kernel_table();
   const int vx = lp->vx >= 1 ? lp->vx : 1;
+  // a launch narrower than one wavefront along x (a halo strip a few columns wide) keeps
+  // the narrow heuristic shape: tuned shapes belong to a kernel's wide launches, and one
+  // kernel may be launched over rows and over columns (a halo kernel's faces)
+  const bool narrow = lp->count[0] < 64 * vx;
   unsigned bx, by, bz;
-  if (kernel_id >= 0 && kernel_id < (int)kt.size() && kt[kernel_id].bx > 0) {
+  if (kernel_id >= 0 && kernel_id < (int)kt.size() && kt[kernel_id].bx > 0 && !narrow) {
     bx = kt[kernel_id].bx;
     by = kt[kernel_id].by;
     bz = kt[kernel_id].bz;
@@ -484,8 +488,9 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   // autotuning: workgroup shapes tried on the kernel's first launches. Loops with a sum
   // reduction are left alone, since the shape sets the order of the partial sums.
   lp->tune_cand = -1;
-  if (g_autotune && kernel_id >= 0 && kernel_id < (int)kt.size() && kt[kernel_id].tune_state >= 0 &&
-      kt[kernel_id].bx == 0 && in->block_size_x <= 0 && in->block_size_y <= 0) {
+  if (g_autotune && !narrow && kernel_id >= 0 && kernel_id < (int)kt.size() &&
+      kt[kernel_id].tune_state >= 0 && kt[kernel_id].bx == 0 && in->block_size_x <= 0 &&
+      in->block_size_y <= 0) {
     bool sum_red = false;
     for (int i = 0; i < nargs; i++)
       if (args[i].argtype == OPS_ARG_GBL && args[i].acc == OPS_INC) sum_red = true;
