@@ -156,6 +156,42 @@ def add_v1_param(text: str, f: cparse.FuncDef) -> str:
     return head + sep + "ops_v1_shape ops_v1" + text[f.params_close:]
 
 
+def pointwise_params(text: str, f: cparse.FuncDef) -> set:
+    """Indices of the parameters that the kernel body touches only at offset 0: every
+    occurrence of the name is an accessor call with all-zero offsets, `a(0, 0)` (ACC) or
+    `a[OPS_ACC<n>(0, 0, 0)]` (API v1). Any other use (another offset, a component index, a
+    method, passing the object on, use through a macro of the application) keeps the
+    parameter off the list."""
+    body = cparse.mask(text[f.body_open:f.end])
+    out = set()
+    for i, ptxt in enumerate(f.params):
+        m = re.search(r"([A-Za-z_]\w*)\s*$", ptxt)
+        if not m:
+            continue
+        name = m.group(1)
+        occ = list(re.finditer(r"\b%s\b" % re.escape(name), body))
+        ok = bool(occ)
+        for mm in occ:
+            rest = body[mm.end():].lstrip()
+            if not (re.match(r"\(\s*0\s*(?:,\s*0\s*){0,2}\)", rest) or
+                    re.match(r"\[\s*OPS_ACC\d+\s*\(\s*0\s*(?:,\s*0\s*){0,2}\)\s*\]", rest)):
+                ok = False
+                break
+        if ok:
+            out.add(i)
+    return out
+
+
+def streamed(loop: Loop, i: int, pointwise: set) -> bool:
+    """A dat argument read once per point and only at the point itself (READ or RW,
+    one component, no prolong addressing, always present): the generated kernel loads it
+    with a non-temporal load into a register and hands the kernel that register (RW: stored
+    back with a plain store after the kernel)."""
+    a = loop.args[i]
+    return (a.kind == "dat" and i in pointwise and a.acc in ("OPS_READ", "OPS_RW") and
+            not a.opt and not a.prolong and cparse.int_literal(a.dim) == 1)
+
+
 # ------------------------------------------------------------------ code generation
 
 def const_dim(a: Arg, loop: Loop) -> str:
@@ -175,6 +211,13 @@ VX = int(os.environ.get("OPS_GEN_VX", "1"))
 RED_VX = int(os.environ.get("OPS_GEN_RED_VX", "4"))
 
 
+# non-temporal loads of point-wise arguments, opt-in (OPS_GEN_STREAM=1): on gfx950 they
+# lift a plain 5-read/2-write stream from 5.9 to 6.7 TB/s, but in the stencil kernels,
+# whose other arguments share L2 with them, the CloverLeaf steps gained under 1 % (2D)
+# or lost up to 8 % (3D at 96^3)
+STREAM = os.environ.get("OPS_GEN_STREAM", "0") == "1"
+
+
 def loop_vx(loop: Loop) -> int:
     reds = [a for a in loop.args if a.kind == "reduce"]
     if reds and all(a.acc in ("OPS_MIN", "OPS_MAX") for a in reds):
@@ -182,7 +225,8 @@ def loop_vx(loop: Loop) -> int:
     return VX
 
 
-def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False) -> str:
+def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
+               pointwise: Optional[set] = None) -> str:
     k = loop.stub
     L = []
     # kernel-argument struct
@@ -205,7 +249,9 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False) -> 
         # the same body twice: with every dat as a no-alias pointer parameter (the usual
         # launch: loads need not wait behind the stores of other arguments), and reading
         # the pointers from the struct (a dat passed to two arguments, one of them writing)
-        L.append(kernel_body(loop, params, ndim, k, True, v1))
+        # the no-alias variant also streams the point-wise arguments (non-temporal loads);
+        # the aliasing variant keeps every access on the dat's memory
+        L.append(kernel_body(loop, params, ndim, k, True, v1, pointwise or set()))
         L.append("")
         L.append(kernel_body(loop, params, ndim, k + "_alias", False, v1))
     else:
@@ -220,7 +266,7 @@ def noalias_variant(loop: Loop) -> bool:
 
 
 def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: bool,
-                v1: bool = False) -> str:
+                v1: bool = False, pointwise: set = frozenset()) -> str:
     k = loop.stub
     L = []
     red = [i for i, a in enumerate(loop.args) if a.kind == "reduce"]
@@ -254,6 +300,7 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
                 L.append(f"    v1s.xd[{i}] = p.xd{i}; v1s.yd[{i}] = p.yd{i}; "
                          f"v1s.zd[{i}] = p.zd{i}; v1s.md[{i}] = p.md{i};")
     call = []
+    after = []  # stores of streamed read-write arguments
     for i, a in enumerate(loop.args):
         pr = params[i]
         if a.kind == "dat":
@@ -269,6 +316,12 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
                 off = (f"((p.st0 + x) / p.mx{i}) * p.sx{i} + ((p.st1 + y) / p.my{i}) * p.sy{i}"
                        f" + ((p.st2 + z) / p.mz{i}) * p.sz{i}")
             ptr = f"{'q' if noalias else 'p.p'}{i} + ({off})"
+            if noalias and streamed(loop, i, pointwise):
+                L.append(f"    {t} *const e{i} = {ptr};")
+                L.append(f"    {t} v{i} = __builtin_nontemporal_load(e{i});")
+                if a.acc == "OPS_RW":
+                    after.append(f"    *e{i} = v{i};")
+                ptr = f"&v{i}"
             if not pr.is_acc:
                 # API v1: the pointer at this point; OPS_ACC<n> adds the stencil offset
                 call.append(f"({t} *)({ptr})")
@@ -302,6 +355,7 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
     if v1:
         call.append("v1s")
     L.append(f"    {loop.kernel}({', '.join(call)});")
+    L += after
     L.append("  }")
     if vx > 1:
         L.append("  }")
@@ -480,7 +534,9 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
         if len(params) != len(lp.args):
             raise TranslateError(f"{lp.file}:{lp.line}: kernel {lp.kernel} takes {len(params)} parameters "
                                  f"but the loop passes {len(lp.args)} arguments")
-        H.append(gen_kernel(lp, params, ndim, lp.kernel in v1_kernels))
+        htext = Path(headers[lp.kernel][0]).read_text()
+        pw = pointwise_params(htext, fdef) if STREAM else set()
+        H.append(gen_kernel(lp, params, ndim, lp.kernel in v1_kernels, pw))
         H.append("")
         stub_defs.append(gen_stub(lp, params, ns))
     H.append(f"}}  // namespace {ns}")

@@ -119,3 +119,36 @@ def test_output_compiles_for_gfx950(tmp_path):
             r["host"][0], "-o", str(tmp_path / "h.o")]
     p = subprocess.run(hcmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert p.returncode == 0, p.stdout[-3000:]
+
+
+def test_pointwise_arguments_and_streaming(tmp_path, monkeypatch):
+    from ops_amd.translator import hip_gfx950 as hg
+    from ops_amd.translator import cparse
+    text = KERNELS + """
+void k_pw(const ACC<double> &a, ACC<double> &b, const ACC<double> &c, ACC<double> &d) {
+  b(0, 0) = 2.0 * a(0, 0) + c(1, 0);
+  d(0, 0) = d(0, 0) + a(0, 0);
+}
+"""
+    f = {fd.name: fd for fd in cparse.find_functions(text)}
+    assert hg.pointwise_params(text, f["k_pw"]) == {0, 1, 3}     # c reads (1, 0)
+    assert hg.pointwise_params(text, f["k_v1"]) == {1}           # a reads (0, 1); n is n[0]
+    assert hg.pointwise_params(text, f["k_v1_md"]) == {1}        # v is a component access
+    main = MAIN.replace("}\n", """  ops_par_loop(k_pw, "k_pw", blk, 2, range,
+               ops_arg_dat(a, 1, s1, "double", OPS_READ),
+               ops_arg_dat(b, 1, s1, "double", OPS_WRITE),
+               ops_arg_dat(v, 1, s3, "double", OPS_READ),
+               ops_arg_dat(a, 1, s1, "double", OPS_RW));
+}
+""", 1)
+    src = make_app(tmp_path, kernels=text, main=main)
+    monkeypatch.setattr(hg, "STREAM", True)
+    hip = Path(translate_app("tapp", [src], str(tmp_path / "out"))["kernels"]).read_text()
+    body = hip[hip.index("void ops_kern_k_pw("):hip.index("void ops_kern_k_pw_alias(")]
+    # READ and RW point-wise arguments are loaded non-temporally and the RW one is stored
+    # back; the WRITE argument and the stencil argument are left on the dat's memory
+    assert "double v0 = __builtin_nontemporal_load(e0);" in body
+    assert "double v3 = __builtin_nontemporal_load(e3);" in body and "*e3 = v3;" in body
+    assert "v1 = " not in body and "v2 = " not in body
+    alias = hip[hip.index("void ops_kern_k_pw_alias("):]
+    assert "nontemporal" not in alias.split("void ops_par_loop_")[0]
