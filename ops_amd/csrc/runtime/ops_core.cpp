@@ -134,6 +134,9 @@ extern "C" void ops_init(const int argc, const char *const argv[], const int dia
   in->diags = diags_level;
   const char *env = getenv("OPS_DUMP_DIR");
   if (env) in->dump_dir = env;
+  // autotuning is per session: off unless this ops_init's environment or argv asks
+  // (shapes already decided stay decided for later sessions that tune)
+  g_autotune = 0;
   const char *aenv = getenv("OPS_AUTOTUNE");
   if (aenv) g_autotune = atoi(aenv);
   const char *tenv = getenv("OPS_HIP_TIMING");

@@ -44,6 +44,7 @@ struct DevState {
     int kid;
     hipEvent_t a, b;
     int cand;  // autotuning candidate measured by this launch, -1 for plain timing
+    double pts;  // loop points of the launch (candidates are compared per point)
   };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
@@ -201,20 +202,23 @@ extern "C" void ops_hip_set_timing(int on) { g_timing = on; }
 
 int ops_tune_next(int kid, int ncand, int reps) {
   auto &kt = ops_kernel_table();
-  if (!g_autotune || g_capturing || kid < 0 || kid >= (int)kt.size()) return -1;
+  if (!g_autotune || kid < 0 || kid >= (int)kt.size()) return -1;
   ops_kernel_entry &e = kt[kid];
   if (e.tune_state < 0) return -1;
+  // a decided shape also applies to captured launches; measuring needs events between
+  // launches, so a capture never starts or continues it
   if (e.tune_state == 2) return e.tune_choice;
+  if (g_capturing) return -1;
   if (ncand > 8) ncand = 8;
   e.tune_state = 1;
   return e.tune_calls % ncand;
   (void)reps;
 }
 
-void ops_tune_begin(int kid, int cand) {
+void ops_tune_begin(int kid, int cand, double pts) {
   hipEvent_t a = get_event(), b = get_event();
   HIPCHECK(hipEventRecord(a, g_dev.stream));
-  g_dev.pending.push_back({kid, a, b, cand});
+  g_dev.pending.push_back({kid, a, b, cand, pts});
 }
 
 void ops_tune_end(int kid, int ncand, int reps) {
@@ -226,7 +230,9 @@ void ops_tune_end(int kid, int ncand, int reps) {
   int best = 0;
   double best_ms = 1e300;
   for (int c = 0; c < ncand && c < 8; c++) {
-    const double avg = e.cand_n[c] ? e.cand_ms[c] / e.cand_n[c] : 1e300;
+    // time per loop point: one kernel's launches may differ in size (the blocks of a
+    // multi-block mesh, the faces of a halo update), so candidates are compared per point
+    const double avg = e.cand_pts[c] > 0 ? e.cand_ms[c] / e.cand_pts[c] : 1e300;
     if (avg < best_ms) {
       best_ms = avg;
       best = c;
@@ -247,6 +253,7 @@ void ops_dev_timing_flush() {
     if (p.cand >= 0 && p.cand < 8) {
       kt[p.kid].cand_ms[p.cand] += ms;
       kt[p.kid].cand_n[p.cand]++;
+      kt[p.kid].cand_pts[p.cand] += p.pts;
     }
     g_dev.pool.push_back(p.a);
     g_dev.pool.push_back(p.b);
@@ -554,12 +561,13 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       lp->red_init[i] = h->dirty_d ? 0 : 1;
     }
   if (lp->tune_cand >= 0) {
-    ops_tune_begin(kernel_id, lp->tune_cand);
+    ops_tune_begin(kernel_id, lp->tune_cand,
+                   (double)lp->count[0] * (double)lp->count[1] * (double)lp->count[2]);
     lp->timed = 1;
   } else if (g_timing) {
     hipEvent_t a = get_event(), b = get_event();
     HIPCHECK(hipEventRecord(a, g_dev.stream));
-    g_dev.pending.push_back({kernel_id, a, b, -1});
+    g_dev.pending.push_back({kernel_id, a, b, -1, 0.0});
     lp->timed = 1;
   }
   return 1;

@@ -23,6 +23,7 @@ struct ops_kernel_entry {
   int tune_choice = -1;
   double cand_ms[8] = {0};
   int cand_n[8] = {0};
+  double cand_pts[8] = {0};  // loop points measured per candidate
 };
 
 struct ops_const_entry {
@@ -80,9 +81,10 @@ void ops_trid_flush();
 // Autotuning seam. ops_tune_next returns the candidate (0..ncand-1) the next launch of
 // kernel kid should use while it is being measured, the chosen one afterwards, or -1 when
 // tuning is off (or the kernel is not tunable). A measured launch is bracketed by
-// ops_tune_begin/ops_tune_end, which decides once every candidate has `reps` samples.
+// ops_tune_begin/ops_tune_end, which decides once every candidate has `reps` samples;
+// candidates are compared by time per point (`pts`: the points the launch covers).
 int ops_tune_next(int kid, int ncand, int reps);
-void ops_tune_begin(int kid, int cand);
+void ops_tune_begin(int kid, int cand, double pts);
 void ops_tune_end(int kid, int ncand, int reps);
 extern int g_autotune;
 void ops_dev_halo_copy(ops_halo h);

@@ -300,14 +300,16 @@ void launch_chunk(const std::vector<Call> &calls, size_t first, size_t count) {
     B.sys[B.nsys++] = S;
   }
   if (B.nsys == 0) return;
+  double pts = 0.0;  // system points of the launch (autotuning compares time per point)
+  for (int k = 0; k < B.nsys; k++) pts += (double)B.sys[k].nlines * B.sys[k].n[sd];
   T *scr_p = (T *)scratch(sizeof(T) * (size_t)scr);
   hipStream_t st = (hipStream_t)ops_hip_get_stream_raw();
   if (sd == 0) {
     static const int kid = ops_hip_register_kernel("ops_tridiag_x");
     const int c = ops_tune_next(kid, 3, 2);
     const dim3 grid((unsigned)(line / XL));
-    if (c > 0 && ops_kernel_table()[kid].tune_state == 1) ops_tune_begin(kid, c);
-    else if (c == 0 && ops_kernel_table()[kid].tune_state == 1) ops_tune_begin(kid, 0);
+    if (c > 0 && ops_kernel_table()[kid].tune_state == 1) ops_tune_begin(kid, c, pts);
+    else if (c == 0 && ops_kernel_table()[kid].tune_state == 1) ops_tune_begin(kid, 0, pts);
     const bool measured = c >= 0 && ops_kernel_table()[kid].tune_state == 1;
     if (c == 1)
       hipLaunchKernelGGL((thomas_x<T, INC, 8>), grid, dim3(XL), 0, st, B, scr_p);

@@ -123,6 +123,19 @@ def test_gpu_uneven_blocks_equal_oracle():
 
 
 @pytest.mark.gpu
+def test_gpu_autotuned_uneven_blocks_equal_oracle(monkeypatch):
+    # the autotuner tries its workgroup shapes on the first launches (blocks of different
+    # sizes share one kernel); the fields stay bitwise the oracle's
+    monkeypatch.setenv("OPS_AUTOTUNE", "1")
+    sizes = [(33, 17, 9), (16, 40, 7), (70, 3, 20), (64, 24, 40)]
+    gpu = run("gpu", sizes, 12)
+    monkeypatch.delenv("OPS_AUTOTUNE")
+    cpu = run("cpu", sizes, 12)
+    for a, b in zip(cpu, gpu):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
 def test_gpu_48_block_configuration_first_block_equals_oracle():
     # the benchmark configuration: 48 blocks with extents in {64..256}, 100 steps; the
     # field is drawn block after block, so the oracle reproduces block 0 alone

@@ -36,14 +36,14 @@ def lib(kind):
     return _cpu
 
 
-def run(kind, deck_text, dump_dir=None, fetch=(), threads=1):
+def run(kind, deck_text, dump_dir=None, fetch=(), threads=1, tokens=()):
     L = lib(kind)
     if kind == "cpu":
         L.ops_cpu_set_threads(threads)
     with tempfile.TemporaryDirectory() as t:
         deck = Path(t) / "clover.in"
         deck.write_text(deck_text)
-        argc, argv = ops_amd.argv(*([f"OPS_DUMP_DIR={dump_dir}"] if dump_dir else []))
+        argc, argv = ops_amd.argv(*([f"OPS_DUMP_DIR={dump_dir}"] if dump_dir else []), *tokens)
         assert L.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
         n = L.cloverleaf3d_run(1 << 30)
         assert n >= 0
@@ -194,3 +194,16 @@ def test_gpu_captured_steps_bitwise_equal_reference(golden, tmp_path, name):
     finally:
         L.cloverleaf3d_set_graph(0)
     assert check_dump(tmp_path, fx) == []
+
+
+@pytest.mark.gpu
+def test_gpu_autotuned_bitwise_equals_reference(golden, tmp_path):
+    """The benchmark deck with the autotuner on: the loops try their candidate workgroup
+    shapes on their first launches, so the steps mix shapes; every dat is still bitwise
+    the reference's."""
+    fx = golden("clover3d_bm.json")
+    n, s, got = run("gpu", fx["deck"], dump_dir=str(tmp_path), fetch=SUMMARY_FIELDS,
+                    tokens=("OPS_AUTOTUNE=1",))
+    assert n == fx["steps"]
+    assert check_dump(tmp_path, fx) == []
+    check_summary(s, got, fx, 96, 96, 96)
