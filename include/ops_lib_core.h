@@ -284,6 +284,34 @@ void ops_fill_random_uniform(ops_dat dat);
 void ops_fill_random_normal(ops_dat dat);
 void ops_randomgen_exit(void);
 
+/* Checkpointing (ops/c/include/ops_checkpointing.h:61-181). The reference saves its
+   checkpoints as HDF5 files; built without HDF5 (ops/c/src/externlib/ops_checkpointing.cpp:
+   1734-1777) every entry point is inert: init reports "not restarting", every loop runs,
+   the fast-forward calls report "no checkpoint taken". No HDF5 library is available to this
+   backend either, so it exports exactly that behaviour: applications that call these
+   (apps/c/CloverLeaf/build_field.cpp:236) link and run unchanged. */
+typedef enum {
+  OPS_CHECKPOINT_INITPHASE = 1,
+  OPS_CHECKPOINT_MANUAL_DATLIST = 2,
+  OPS_CHECKPOINT_FASTFW = 4,
+  OPS_CHECKPOINT_MANUAL = 8
+} ops_checkpoint_options;
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+bool ops_checkpointing_init(const char *filename, double interval, int options);
+void ops_checkpointing_initphase_done(void);
+bool ops_checkpointing_before(ops_arg *args, int nargs, int *range, int loop_id);
+bool ops_checkpointing_name_before(ops_arg *args, int nargs, int *range, const char *s);
+void ops_checkpointing_exit(OPS_instance *instance);
+void ops_checkpointing_reduction(ops_reduction red);
+void ops_checkpointing_manual_datlist(int ndats, ops_dat *datlist);
+bool ops_checkpointing_fastfw(int nbytes, char *payload);
+bool ops_checkpointing_manual_datlist_fastfw(int ndats, ops_dat *datlist, int nbytes,
+                                             char *payload);
+bool ops_checkpointing_manual_datlist_fastfw_trigger(int ndats, ops_dat *datlist, int nbytes,
+                                                     char *payload);
+
 /* backend-specific: last error text for C/ctypes callers that cannot catch C++ */
 const char *ops_get_last_error(void);
 

@@ -552,6 +552,34 @@ extern "C" void ops_reduction_result_char(ops_reduction handle, int type_size, c
   handle->initialized = 0;
 }
 
+// ---------------------------------------------------------------- checkpointing
+// The reference's behaviour when built without HDF5 (ops/c/src/externlib/
+// ops_checkpointing.cpp:1734-1777): nothing is saved or restored, every loop runs.
+
+extern "C" bool ops_checkpointing_init(const char *, double, int) { return false; }
+extern "C" void ops_checkpointing_initphase_done(void) {}
+extern "C" bool ops_checkpointing_before(ops_arg *, int, int *, int) { return true; }
+extern "C" bool ops_checkpointing_name_before(ops_arg *, int, int *, const char *) {
+  return true;
+}
+extern "C" void ops_checkpointing_exit(OPS_instance *) {}
+// the reference stub completes the reduction (ops_execute_reduction); here a reduction's
+// value is on the device until it is read, so this brings it to the host handle
+extern "C" void ops_checkpointing_reduction(ops_reduction red) {
+  if (red && red->dirty_d) {
+    ops_dev_d2h(red->data, red->data_d, (size_t)red->size);
+    red->dirty_d = 0;
+  }
+}
+extern "C" void ops_checkpointing_manual_datlist(int, ops_dat *) {}
+extern "C" bool ops_checkpointing_fastfw(int, char *) { return false; }
+extern "C" bool ops_checkpointing_manual_datlist_fastfw(int, ops_dat *, int, char *) {
+  return false;
+}
+extern "C" bool ops_checkpointing_manual_datlist_fastfw_trigger(int, ops_dat *, int, char *) {
+  return false;
+}
+
 // ---------------------------------------------------------------- halos
 
 // Every index a halo reads or writes must lie inside the dat's allocation: the window of
