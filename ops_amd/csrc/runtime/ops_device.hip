@@ -436,7 +436,14 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       for (int d = 0; d < 3; d++) {
         int s = (d < dim && d < a.stencil->dims) ? sst[d] : 0;
         if (stype == 2 && d < dim) s *= mgs(d);
-        lp->step[i][d] = (int)(s * cumd * comp);
+        const long long step = (long long)s * cumd * comp;
+        if (step < 0 || step > INT_MAX)
+          ops_throw(OPS_INVALID_ARGUMENT,
+                    std::string("ops_par_loop ") + (name ? name : "") + ": argument " +
+                        std::to_string(i) + " steps " + std::to_string(step) +
+                        " elements along dimension " + std::to_string(d) +
+                        ", beyond the 32-bit step of the generated kernels");
+        lp->step[i][d] = (int)step;
         lp->mg[i][d] = (stype == 1 && d < dim) ? mgs(d) : 1;
         cumd *= dat->size[d];
       }
@@ -526,6 +533,20 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
     ops_throw(OPS_INVALID_ARGUMENT,
               "ops_par_loop: workgroup shape must be a whole number of 64-lane wavefronts "
               "and at most 256 threads");
+  // The generated kernels address a dat argument from the workgroup's origin (64-bit,
+  // wave-uniform) plus the lane's offset inside the workgroup, tx*sx + ty*sy + tz*sz,
+  // taken as 24-bit products along x and y (full-rate multiplies): the x and y steps must
+  // be below 2^24 elements and the lane offset must fit 32 bits.
+  for (int i = 0; i < nargs; i++) {
+    if (args[i].argtype != OPS_ARG_DAT || !args[i].opt) continue;
+    const long long lane_max = (long long)(bx - 1) * lp->step[i][0] +
+                               (long long)(by - 1) * lp->step[i][1] +
+                               (long long)(bz - 1) * lp->step[i][2];
+    if (lp->step[i][0] >= (1 << 24) || lp->step[i][1] >= (1 << 24) || lane_max > UINT_MAX)
+      ops_throw(OPS_INVALID_ARGUMENT,
+                std::string("ops_par_loop ") + (name ? name : "") + ": argument " +
+                    std::to_string(i) + " has an x or y step of 2^24 elements or more");
+  }
   lp->block[0] = bx;
   lp->block[1] = by;
   lp->block[2] = bz;

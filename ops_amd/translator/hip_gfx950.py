@@ -265,6 +265,21 @@ def noalias_variant(loop: Loop) -> bool:
     return len(dats) >= 2 and any(a.acc != "OPS_READ" for a in dats)
 
 
+def dat_offset(i: int, ndim: int) -> str:
+    """Element offset of dat argument i at point (x, y, z): the workgroup origin's part in
+    64 bits, then the lane's part (unsigned, 24-bit products along x and y). Dimensions
+    beyond the application's are 0 inside the kernel's bounds test, so they are left out."""
+    uni = [f"(long long)x0 * p.sx{i}"]
+    lane = [f"(unsigned)__umul24(tx, (unsigned)p.sx{i})"]
+    if ndim >= 2:
+        uni.append(f"(long long)y0 * p.sy{i}")
+        lane.append(f"(unsigned)__umul24(ty, (unsigned)p.sy{i})")
+    if ndim >= 3:
+        uni.append(f"(long long)z0 * p.sz{i}")
+        lane.append(f"tz * (unsigned)p.sz{i}")
+    return f"({' + '.join(uni)}) + ({' + '.join(lane)})"
+
+
 def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: bool,
                 v1: bool = False, pointwise: set = frozenset()) -> str:
     k = loop.stub
@@ -281,16 +296,23 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
         L.append(f"  for (int d = 0; d < {d}; d++) r{i}[d] = {RED_OP[a.acc]}<{t}>::identity();")
     L.append("  unsigned bx, by, bz;")
     L.append("  ops_block_coords(bx, by, bz);")
-    L.append("  const int y = (int)(by * blockDim.y + threadIdx.y);")
-    L.append("  const int z = (int)(bz * blockDim.z + threadIdx.z);")
+    # the workgroup's origin (x0, y0, z0) is wave-uniform, so its part of every address is
+    # scalar arithmetic; the lane's part uses 24-bit multiplies (the runtime checks that the
+    # x and y steps fit), where a full 32-bit multiply per argument and dimension would be
+    # a quarter-rate vector instruction
+    L.append("  const unsigned tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;")
+    L.append("  const int y0 = (int)(by * blockDim.y), z0 = (int)(bz * blockDim.z);")
+    L.append("  const int y = y0 + (int)ty;")
+    L.append("  const int z = z0 + (int)tz;")
     vx = loop_vx(loop)
     if vx > 1:
         # vx points per lane, blockDim.x apart: every pass of the wavefront stays contiguous
         L.append("#pragma unroll")
         L.append(f"  for (int v = 0; v < {vx}; v++) {{")
-        L.append(f"  const int x = (int)((bx * {vx} + v) * blockDim.x + threadIdx.x);")
+        L.append(f"  const int x0 = (int)((bx * {vx} + v) * blockDim.x);")
     else:
-        L.append("  const int x = (int)(bx * blockDim.x + threadIdx.x);")
+        L.append("  const int x0 = (int)(bx * blockDim.x);")
+    L.append("  const int x = x0 + (int)tx;")
     L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
     if v1:
         # sizes behind OPS_ACC<n>(...) for the dat arguments (xdim<n> = size[0], ...)
@@ -309,13 +331,15 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
                                      "must be ACC<T>& (or a pointer addressed with OPS_ACC<n>) "
                                      "for an ops_arg_dat argument")
             t = pr.ctype
-            off = f"x * p.sx{i} + y * p.sy{i} + z * p.sz{i}"
+            off = dat_offset(i, ndim)
             if a.prolong:
                 # coarse index = global loop index / mgrid stride (C division, as the
                 # reference's generated loops: templates/cpp/mpi_openmp/loop_host.cpp.j2:234-237)
                 off = (f"((p.st0 + x) / p.mx{i}) * p.sx{i} + ((p.st1 + y) / p.my{i}) * p.sy{i}"
                        f" + ((p.st2 + z) / p.mz{i}) * p.sz{i}")
-            ptr = f"{'q' if noalias else 'p.p'}{i} + ({off})"
+            # (pointer + uniform part) + lane part: the first sum stays scalar
+            ptr = f"{'q' if noalias else 'p.p'}{i} + {off}" if not a.prolong else \
+                f"{'q' if noalias else 'p.p'}{i} + ({off})"
             if noalias and streamed(loop, i, pointwise):
                 L.append(f"    {t} *const e{i} = {ptr};")
                 L.append(f"    {t} v{i} = __builtin_nontemporal_load(e{i});")
