@@ -182,6 +182,49 @@ def pointwise_params(text: str, f: cparse.FuncDef) -> set:
     return out
 
 
+def written_once_params(text: str, f: cparse.FuncDef) -> set:
+    """Indices of the parameters that the kernel body writes exactly once at the point
+    itself and never reads: the name occurs once, as the left side of `a(0, 0) = ...` (ACC)
+    or `a[OPS_ACC<n>(0, 0)] = ...` (API v1), and that assignment starts a statement at the
+    top level of the body (not under if/else/for/while/switch), in a body without return
+    or goto."""
+    body = cparse.mask(text[f.body_open:f.end])
+    if re.search(r"\b(return|goto)\b", body):
+        return set()
+    out = set()
+    for i, ptxt in enumerate(f.params):
+        m = re.search(r"([A-Za-z_]\w*)\s*$", ptxt)
+        if not m:
+            continue
+        name = m.group(1)
+        occ = list(re.finditer(r"\b%s\b" % re.escape(name), body))
+        if len(occ) != 1:
+            continue
+        o = occ[0]
+        rest = body[o.end():]
+        if not (re.match(r"\s*\(\s*0\s*(?:,\s*0\s*){0,2}\)\s*=(?!=)", rest) or
+                re.match(r"\s*\[\s*OPS_ACC\d+\s*\(\s*0\s*(?:,\s*0\s*){0,2}\)\s*\]\s*=(?!=)", rest)):
+            continue
+        before = body[:o.start()]
+        if before.count("{") - before.count("}") != 1:
+            continue  # inside a nested block
+        # the statement begins right here: only whitespace since the last ; { or }
+        last = max(before.rfind(";"), before.rfind("{"), before.rfind("}"))
+        if before[last + 1:].strip():
+            continue
+        out.add(i)
+    return out
+
+
+def nt_stored(loop: Loop, i: int, written_once: set) -> bool:
+    """A WRITE dat argument that the kernel assigns once at its own point and never reads
+    (one component, no prolong addressing, always present): the generated kernel hands the
+    kernel a register and writes it with a non-temporal store afterwards."""
+    a = loop.args[i]
+    return (a.kind == "dat" and i in written_once and a.acc == "OPS_WRITE" and
+            not a.opt and not a.prolong and cparse.int_literal(a.dim) == 1)
+
+
 def streamed(loop: Loop, i: int, pointwise: set) -> bool:
     """A dat argument read once per point and only at the point itself (READ or RW,
     one component, no prolong addressing, always present): the generated kernel loads it
@@ -216,6 +259,9 @@ RED_VX = int(os.environ.get("OPS_GEN_RED_VX", "4"))
 # whose other arguments share L2 with them, the CloverLeaf steps gained under 1 % (2D)
 # or lost up to 8 % (3D at 96^3)
 STREAM = os.environ.get("OPS_GEN_STREAM", "0") == "1"
+# non-temporal stores of the arguments a kernel writes once at its own point and never
+# reads (OPS_GEN_NTSTORE=0 turns them off)
+NTSTORE = os.environ.get("OPS_GEN_NTSTORE", "1") == "1"
 
 
 def loop_vx(loop: Loop) -> int:
@@ -226,7 +272,7 @@ def loop_vx(loop: Loop) -> int:
 
 
 def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
-               pointwise: Optional[set] = None) -> str:
+               pointwise: Optional[set] = None, written_once: Optional[set] = None) -> str:
     k = loop.stub
     L = []
     # kernel-argument struct
@@ -251,7 +297,8 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
         # the pointers from the struct (a dat passed to two arguments, one of them writing)
         # the no-alias variant also streams the point-wise arguments (non-temporal loads);
         # the aliasing variant keeps every access on the dat's memory
-        L.append(kernel_body(loop, params, ndim, k, True, v1, pointwise or set()))
+        L.append(kernel_body(loop, params, ndim, k, True, v1, pointwise or set(),
+                             written_once or set()))
         L.append("")
         L.append(kernel_body(loop, params, ndim, k + "_alias", False, v1))
     else:
@@ -281,7 +328,8 @@ def dat_offset(i: int, ndim: int) -> str:
 
 
 def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: bool,
-                v1: bool = False, pointwise: set = frozenset()) -> str:
+                v1: bool = False, pointwise: set = frozenset(),
+                written_once: set = frozenset()) -> str:
     k = loop.stub
     L = []
     red = [i for i, a in enumerate(loop.args) if a.kind == "reduce"]
@@ -345,6 +393,11 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
                 L.append(f"    {t} v{i} = __builtin_nontemporal_load(e{i});")
                 if a.acc == "OPS_RW":
                     after.append(f"    *e{i} = v{i};")
+                ptr = f"&v{i}"
+            elif noalias and nt_stored(loop, i, written_once):
+                L.append(f"    {t} *const e{i} = {ptr};")
+                L.append(f"    {t} v{i};")
+                after.append(f"    __builtin_nontemporal_store(v{i}, e{i});")
                 ptr = f"&v{i}"
             if not pr.is_acc:
                 # API v1: the pointer at this point; OPS_ACC<n> adds the stencil offset
@@ -449,8 +502,10 @@ def const_decl(c: Const) -> Tuple[str, str]:
 
 
 def translate_app(app_name: str, sources: List[str], out_dir: str,
-                  app_dir: Optional[str] = None) -> Dict[str, object]:
-    """Translate the application made of `sources`; returns the generated file list."""
+                  app_dir: Optional[str] = None, ntstore: Optional[bool] = None) -> Dict[str, object]:
+    """Translate the application made of `sources`; returns the generated file list.
+    `ntstore` overrides OPS_GEN_NTSTORE (non-temporal stores of write-once arguments)."""
+    use_nt = NTSTORE if ntstore is None else ntstore
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     app_dir = app_dir or str(Path(sources[0]).parent)
@@ -560,7 +615,8 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
                                  f"but the loop passes {len(lp.args)} arguments")
         htext = Path(headers[lp.kernel][0]).read_text()
         pw = pointwise_params(htext, fdef) if STREAM else set()
-        H.append(gen_kernel(lp, params, ndim, lp.kernel in v1_kernels, pw))
+        wo = written_once_params(htext, fdef) if use_nt else set()
+        H.append(gen_kernel(lp, params, ndim, lp.kernel in v1_kernels, pw, wo))
         H.append("")
         stub_defs.append(gen_stub(lp, params, ns))
     H.append(f"}}  // namespace {ns}")

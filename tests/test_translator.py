@@ -152,3 +152,46 @@ void k_pw(const ACC<double> &a, ACC<double> &b, const ACC<double> &c, ACC<double
     assert "v1 = " not in body and "v2 = " not in body
     alias = hip[hip.index("void ops_kern_k_pw_alias("):]
     assert "nontemporal" not in alias.split("void ops_par_loop_")[0]
+
+
+def test_write_once_arguments_get_non_temporal_stores(tmp_path):
+    from ops_amd.translator import hip_gfx950 as hg
+    from ops_amd.translator import cparse
+    text = KERNELS + """
+void k_w(const ACC<double> &a, ACC<double> &b, ACC<double> &c, ACC<double> &d, ACC<double> &e) {
+  double t = a(0, 0);
+  b(0, 0) = 2.0 * t;
+  if (t > 0.0) c(0, 0) = t;
+  d(0, 0) = t;
+  e(0, 0) = d(0, 0) + 1.0;
+}
+void k_ret(const ACC<double> &a, ACC<double> &b) {
+  if (a(0, 0) < 0.0) return;
+  b(0, 0) = a(0, 0);
+}
+"""
+    f = {fd.name: fd for fd in cparse.find_functions(text)}
+    # b: one top-level write; c: conditional; d: read after the write; e: stencil-free
+    # single write whose right side reads another argument
+    assert hg.written_once_params(text, f["k_w"]) == {1, 4}
+    assert hg.written_once_params(text, f["k_ret"]) == set()
+    assert hg.written_once_params(text, f["k_v1"]) == {1}        # b[OPS_ACC1(0, 0)] = ...
+    main = MAIN.replace("}\n", """  ops_par_loop(k_w, "k_w", blk, 2, range,
+               ops_arg_dat(a, 1, s1, "double", OPS_READ),
+               ops_arg_dat(b, 1, s1, "double", OPS_WRITE),
+               ops_arg_dat(v, 1, s1, "double", OPS_WRITE),
+               ops_arg_dat(a, 1, s1, "double", OPS_RW),
+               ops_arg_dat(v, 1, s1, "double", OPS_RW));
+}
+""", 1)
+    src = make_app(tmp_path, kernels=text, main=main)
+    hip = Path(translate_app("tapp", [src], str(tmp_path / "out"), ntstore=True)["kernels"]).read_text()
+    body = hip[hip.index("void ops_kern_k_w("):hip.index("void ops_kern_k_w_alias(")]
+    # only b (WRITE, written once) goes through a register and a non-temporal store; e is
+    # written once but its access is RW, so it stays on the dat's memory
+    assert "double v1;" in body and "__builtin_nontemporal_store(v1, e1);" in body
+    assert body.count("__builtin_nontemporal_store") == 1
+    alias = hip[hip.index("void ops_kern_k_w_alias("):].split("void ops_par_loop_")[0]
+    assert "nontemporal" not in alias
+    off = Path(translate_app("tapp", [src], str(tmp_path / "o2"), ntstore=False)["kernels"]).read_text()
+    assert "nontemporal_store" not in off
