@@ -240,6 +240,14 @@ void ops_tune_end(int kid, int ncand, int reps) {
   }
   e.tune_choice = best;
   e.tune_state = 2;
+  // OPS_TUNE_REPORT=1: one line per decision on stderr (time per point of each candidate)
+  static const bool report = getenv("OPS_TUNE_REPORT") && atoi(getenv("OPS_TUNE_REPORT")) > 0;
+  if (report) {
+    fprintf(stderr, "ops_autotune %s:", e.name.c_str());
+    for (int c = 0; c < ncand && c < 8; c++)
+      fprintf(stderr, " %d:%.4g", c, e.cand_pts[c] > 0 ? 1e6 * e.cand_ms[c] / e.cand_pts[c] : -1.0);
+    fprintf(stderr, " ns/point -> %d\n", best);
+  }
 }
 
 void ops_dev_timing_flush() {
@@ -250,7 +258,11 @@ void ops_dev_timing_flush() {
     float ms = 0.f;
     HIPCHECK(hipEventElapsedTime(&ms, p.a, p.b));
     kt[p.kid].total_ms += ms;
-    if (p.cand >= 0 && p.cand < 8) {
+    // a kernel's first measured launch is not counted: it pays for first use (cold caches
+    // and TLBs over newly allocated dats) and would always fall on candidate 0
+    if (p.cand >= 0 && p.cand < 8 && !kt[p.kid].tune_warm) {
+      kt[p.kid].tune_warm = 1;
+    } else if (p.cand >= 0 && p.cand < 8) {
       kt[p.kid].cand_ms[p.cand] += ms;
       kt[p.kid].cand_n[p.cand]++;
       kt[p.kid].cand_pts[p.cand] += p.pts;

@@ -24,6 +24,7 @@ struct ops_kernel_entry {
   double cand_ms[8] = {0};
   int cand_n[8] = {0};
   double cand_pts[8] = {0};  // loop points measured per candidate
+  int tune_warm = 0;         // first measured launch seen (it is left out of the averages)
 };
 
 struct ops_const_entry {
