@@ -9,7 +9,7 @@
 //   api_tridiag    ops_tridMultiDimBatch[_Inc] on caller-provided coefficients
 //   api_reduce     INC/MIN/MAX reductions on double and int, a 2-component reduction,
 //                  accumulation across two loops before ops_reduction_result
-//   api_random     ops_randomgen_init + ops_fill_random_uniform on a dat, read back with
+//   api_random[_normal]  ops_randomgen_init + ops_fill_random_uniform / _normal on a dat, read back with
 //                  its halos
 #include <stdio.h>
 #include <stdlib.h>
@@ -304,9 +304,11 @@ extern "C" int api_reduce(int argc, const char **argv, int nx, int ny, const dou
   }
 }
 
-// ops_fill_random_uniform on an nx x ny double dat with one halo layer; out receives the
-// whole allocation ((nx+2)*(ny+2), x fastest)
-extern "C" int api_random(int argc, const char **argv, unsigned seed, int nx, int ny, double *out) {
+// ops_fill_random_uniform (normal = 0) or ops_fill_random_normal (normal = 1) on an
+// nx x ny double dat with one halo layer; out receives the whole allocation
+// ((nx+2)*(ny+2), x fastest)
+static int random_fill(int argc, const char **argv, unsigned seed, int nx, int ny, double *out,
+                       int normal) {
   try {
     ops_init(argc, argv, 1);
     ops_block blk = ops_decl_block(2, "rng");
@@ -314,12 +316,22 @@ extern "C" int api_random(int argc, const char **argv, unsigned seed, int nx, in
     double *none = nullptr;
     ops_dat V = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "v");
     ops_randomgen_init(seed, 0);
-    ops_fill_random_uniform(V);
+    if (normal) ops_fill_random_normal(V);
+    else ops_fill_random_uniform(V);
     fetch_all(V, out);
     ops_randomgen_exit();
     ops_exit();
     return 0;
   } catch (std::exception &e) {
-    return fail("api_random", e);
+    return fail(normal ? "api_random_normal" : "api_random", e);
   }
+}
+
+extern "C" int api_random(int argc, const char **argv, unsigned seed, int nx, int ny, double *out) {
+  return random_fill(argc, argv, seed, nx, ny, out, 0);
+}
+
+extern "C" int api_random_normal(int argc, const char **argv, unsigned seed, int nx, int ny,
+                                 double *out) {
+  return random_fill(argc, argv, seed, nx, ny, out, 1);
 }

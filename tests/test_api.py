@@ -301,3 +301,47 @@ def test_random_uniform_is_mt19937_canonical(kind):
     assert L.api_random(argc, av, 1234, nx, ny, p(out)) == 0
     # the whole allocation, halos included, in memory order (ops_lib_core.cpp:2404-2416)
     assert np.array_equal(out, mt_uniform(1234, out.size))
+
+
+def mt_normal(seed, n):
+    """std::normal_distribution<double>(0, 1) over std::mt19937(seed), as libstdc++ forms it
+    (bits/random.tcc, the Marsaglia polar method): draw x = 2u - 1 and y = 2v - 1 from the
+    canonical uniforms until 0 < r2 = x*x + y*y <= 1, m = sqrt(-2 log(r2) / r2), return y*m
+    and keep x*m for the next call. One distribution object per fill, so a fill of odd
+    length drops its last saved value. math.log / math.sqrt are the C library's."""
+    u = iter(mt_uniform(seed, 4 * n + 64))
+    out, saved = [], None
+    while len(out) < n:
+        if saved is not None:
+            out.append(saved)
+            saved = None
+            continue
+        while True:
+            x = 2.0 * next(u) - 1.0
+            y = 2.0 * next(u) - 1.0
+            r2 = x * x + y * y
+            if 0.0 < r2 <= 1.0:
+                break
+        m = math.sqrt(-2.0 * math.log(r2) / r2)
+        saved = x * m
+        out.append(y * m)
+    return np.array(out)
+
+
+def test_mt_normal_statistics():
+    # the restatement itself: standard normal moments over 20000 draws, and the pair
+    # structure (the saved value follows its partner)
+    z = mt_normal(99, 20000)
+    assert abs(z.mean()) < 0.03 and abs(z.std() - 1.0) < 0.03
+    assert abs(np.mean(z ** 4) - 3.0) < 0.15
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_random_normal_is_libstdcxx_polar(kind):
+    L = lib("apitest", kind)
+    nx, ny = 13, 7
+    out = np.zeros((nx + 2) * (ny + 2))
+    argc, av = argv()
+    assert L.api_random_normal(argc, av, 4321, nx, ny, p(out)) == 0
+    # the whole allocation, halos included, in memory order (ops_lib_core.cpp:2452-2474)
+    assert np.array_equal(out, mt_normal(4321, out.size))
