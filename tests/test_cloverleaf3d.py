@@ -207,3 +207,71 @@ def test_gpu_autotuned_bitwise_equals_reference(golden, tmp_path):
     assert n == fx["steps"]
     assert check_dump(tmp_path, fx) == []
     check_summary(s, got, fx, 96, 96, 96)
+
+
+SPHERE_DECK = """*clover
+ state 1 density=0.2 energy=1.0
+ state 2 density=1.0 energy=2.5 xvel=0.25 geometry=sphere xmin=3.7 ymin=5.2 zmin=4.4 radius=2.3
+ x_cells=40
+ y_cells=36
+ z_cells=44
+ xmin=0.0
+ ymin=0.0
+ zmin=0.0
+ xmax=10.0
+ ymax=10.0
+ zmax=10.0
+ initial_timestep=0.04
+ max_timestep=0.04
+ end_step=1
+ end_time=100.0
+ test_problem 0
+ checkpoint_frequency=10
+*endclover
+"""
+
+
+@pytest.mark.parametrize("kind", ["cpu", pytest.param("gpu", marks=pytest.mark.gpu)])
+def test_sphere_state_covers_the_cells_whose_centre_is_inside(kind):
+    """The sphere branch of the generate kernel, which the reference's deck reader cannot
+    reach (it never reads a state's radius): after set-up, a cell holds the sphere state's
+    density and energy exactly when its centre lies within the radius of the (off-centre)
+    sphere centre, the node at its lower corner carries the state's velocity, and the
+    flagged cells' volume matches 4/3 pi r^3 to the discretisation error."""
+    L = lib(kind)
+    if kind == "cpu":
+        L.ops_cpu_set_threads(4)
+    with tempfile.TemporaryDirectory() as t:
+        deck = Path(t) / "clover.in"
+        deck.write_text(SPHERE_DECK)
+        argc, argv = ops_amd.argv()
+        assert L.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
+        got = {}
+        for name in ("density0", "energy0", "xvel0", "cellx", "celly", "cellz"):
+            cnt = L.cloverleaf3d_fetch(name.encode(), None)
+            buf = np.zeros(cnt)
+            assert L.cloverleaf3d_fetch(name.encode(), buf.ctypes.data_as(ctypes.c_void_p)) == cnt
+            got[name] = buf
+        assert L.cloverleaf3d_finalize() == 0
+    cx, cy, cz = got["cellx"], got["celly"], got["cellz"]
+    nx, ny, nz = cx.size, cy.size, cz.size            # cell allocation extents
+    rho = got["density0"].reshape(nz, ny, nx)
+    e = got["energy0"].reshape(nz, ny, nx)
+    vel = got["xvel0"].reshape(nz + 1, ny + 1, nx + 1)[:nz, :ny, :nx]
+    # the deck reader moves every state's min corner up by 1/100 of a cell, computed in
+    # float (apps/c/CloverLeaf_3D/read_input.cpp:304-315): for a sphere that is its centre
+    cen = [c + float(np.float32(10.0 / n)) / 100.0 for c, n in ((3.7, 40), (5.2, 36), (4.4, 44))]
+    d2 = ((cx[None, None, :] - cen[0]) ** 2 + (cy[None, :, None] - cen[1]) ** 2 +
+          (cz[:, None, None] - cen[2]) ** 2)
+    inside = np.sqrt(d2) <= 2.3
+    written = rho != 0.0                               # the generate loop's range
+    assert written.sum() >= 40 * 36 * 44
+    assert np.array_equal(rho[written], np.where(inside, 1.0, 0.2)[written])
+    assert np.array_equal(e[written], np.where(inside, 2.5, 1.0)[written])
+    assert np.array_equal(vel[written], np.where(inside, 0.25, 0.0)[written])
+    # interior cells flagged x cell volume against the sphere's volume (the sphere lies
+    # inside the domain: 3.7 - 2.3 > 0 and 4.4 + 2.3 < 10)
+    interior = (cx[None, None, :] > 0) & (cx[None, None, :] < 10) & (cy[None, :, None] > 0) & \
+        (cy[None, :, None] < 10) & (cz[:, None, None] > 0) & (cz[:, None, None] < 10)
+    vol = (inside & interior).sum() * (10 / 40) * (10 / 36) * (10 / 44)
+    assert abs(vol / (4.0 / 3.0 * math.pi * 2.3 ** 3) - 1.0) < 0.03
