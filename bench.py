@@ -61,16 +61,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profile(kernel):
+def traffic_from_profile(kernel, algorithmic=None):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    committed under profiles/ (null when not measured for this kernel)."""
+    committed under profiles/ (null when not measured for this kernel). The counters are
+    the median over a profiled run's dispatches, so a kernel launched on meshes of several
+    sizes reports its most frequent size: when that value is not within 0.5-4x of this
+    launch's algorithmic bytes it was measured at another size, and null is returned."""
     p = ROOT / "profiles" / "pmc_traffic.json"
     if not p.exists():
         return None
     try:
         d = json.loads(p.read_text())
         e = d.get("kernels", {}).get(kernel)
-        return None if e is None else float(e["bytes_per_launch"])
+        if e is None:
+            return None
+        t = float(e["bytes_per_launch"])
+        if algorithmic and not 0.5 <= t / algorithmic <= 4.0:
+            return None
+        return t
     except Exception:
         return None
 
@@ -124,7 +132,7 @@ def roofline(stats):
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
     return k, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-               "traffic": traffic_from_profile(k), "kernel": k,
+               "traffic": traffic_from_profile(k, per_launch_bytes), "kernel": k,
                "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_ms * 1e3, 2)}
 
 
