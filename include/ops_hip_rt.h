@@ -100,6 +100,19 @@ void ops_hip_reset_stats(void);
    begin and end only launches may be issued on the runtime stream: no host
    synchronisation, no allocation, no ops_reduction_result, no host<->device copies.
    begin returns 0 on success; end returns an executable graph (NULL on failure). */
+/* Loop batches (no reference counterpart; the reference launches every ops_par_loop on
+   its own). Between ops_hip_batch_begin and ops_hip_batch_end the caller promises that the
+   loops it issues are independent: no dat written by one is read or written by another.
+   Consecutive launches of one generated kernel (loops without reductions) are then
+   collected and launched together, one launch per batch, each workgroup finding its loop
+   from the batch's prefix sums of workgroups. Any other runtime operation, another
+   kernel's launch and ops_hip_batch_end launch the pending batch first. Results are those
+   of the separate launches. */
+void ops_hip_batch_begin(void);
+void ops_hip_batch_end(void);
+/* for generated stubs: 1 if this launch joins a batch (batching on); `flush` launches the
+   stub's pending batch, and is called by the runtime before any other operation */
+int ops_hip_batch_join(int kernel_id, void (*flush)(void));
 int ops_hip_graph_begin(void);
 void *ops_hip_graph_end(void);
 int ops_hip_graph_replay(void *graph, int times);

@@ -64,6 +64,8 @@ void ops_hip_graph_free(void *graph);
 void *ops_hip_const_device_ptr(const char *name);
 void *ops_hip_reduction_device_ptr(ops_reduction h);
 void ops_hip_reduction_discard(ops_reduction h);
+void ops_hip_batch_begin(void);
+void ops_hip_batch_end(void);
 }
 
 namespace {
@@ -456,6 +458,9 @@ void update_halo(int *fl, int depth) {
         {{2, z0, 1}, {x0 - d, x1 + d, y0 - d, y1 + d, z0 - d, z0}},
         {{2, z1, 1}, {x0 - d, x1 + d, y0 - d, y1 + d, z1, z1 + d}}};
     for (int s = 0; s < 6; s++) {
+      // the two faces of an axis are independent (each writes its own halo layers and
+      // reads interior layers): one launch per axis, axes in the reference's order
+      if (s % 2 == 0) ops_hip_batch_begin();
       ops_stencil st = S.H[s];
       ops_par_loop(c3_halo_cells, "update_halo_kernel1", S.blk, 3, faces[s].r,
                    ops_arg_dat_opt(S.density0, 1, st, "double", OPS_RW, fl[FIELD_DENSITY0]),
@@ -467,6 +472,7 @@ void update_halo(int *fl, int depth) {
                    ops_arg_dat_opt(S.soundspeed, 1, st, "double", OPS_RW, fl[FIELD_SOUNDSPEED]),
                    ops_arg_gbl(fl, NUM_FIELDS, "int", OPS_READ),
                    ops_arg_gbl(faces[s].m, 6, "int", OPS_READ), ops_arg_idx());
+      if (s % 2 == 1) ops_hip_batch_end();
     }
   }
   if (any(fl, {FIELD_XVEL0, FIELD_XVEL1, FIELD_YVEL0, FIELD_YVEL1, FIELD_ZVEL0, FIELD_ZVEL1})) {
@@ -479,6 +485,7 @@ void update_halo(int *fl, int depth) {
         {{2, z0, 0, 1, 1, -1}, {x0 - d, x1 + 1 + d, y0 - d, y1 + 1 + d, z0 - d, z0}},
         {{2, z1, 0, 1, 1, -1}, {x0 - d, x1 + 1 + d, y0 - d, y1 + 1 + d, z1 + 1, z1 + 1 + d}}};
     for (int s = 0; s < 6; s++) {
+      if (s % 2 == 0) ops_hip_batch_begin();
       ops_stencil st = S.H[s];
       ops_par_loop(c3_halo_vel, "update_halo_kernel2_vel", S.blk, 3, faces[s].r,
                    ops_arg_dat_opt(S.xvel0, 1, st, "double", OPS_RW, fl[FIELD_XVEL0]),
@@ -489,6 +496,7 @@ void update_halo(int *fl, int depth) {
                    ops_arg_dat_opt(S.zvel1, 1, st, "double", OPS_RW, fl[FIELD_ZVEL1]),
                    ops_arg_gbl(fl, NUM_FIELDS, "int", OPS_READ),
                    ops_arg_gbl(faces[s].m, 6, "int", OPS_READ), ops_arg_idx());
+      if (s % 2 == 1) ops_hip_batch_end();
     }
   }
   // face fluxes: layer h mirrors 2*pivot - h; the flux normal to the face changes sign
@@ -521,16 +529,22 @@ void update_halo(int *fl, int depth) {
         {{0, x1 - 1, 0, 1, VZ, MZ}, {x1, x1 + d, y0 - d, y1 + d, z0 - d, z1 + 1 + d}},
         {{2, z0, 0, -1, VZ, MZ}, {x0 - d, x1 + d, y0 - d, y1 + d, z0 - d, z0}},
         {{2, z1, 0, -1, VZ, MZ}, {x0 - d, x1 + d, y0 - d, y1 + d, z1 + 1, z1 + 1 + d}}}}};
-  for (Group &g : groups) {
-    if (!fl[g.vf] && !fl[g.mf]) continue;
-    for (int s = 0; s < 6; s++) {
-      ops_stencil st = S.H[s];
-      ops_par_loop(c3_halo_flux, "update_halo_kernel_flux", S.blk, 3, g.faces[s].r,
-                   ops_arg_dat_opt(g.v, 1, st, "double", OPS_RW, fl[g.vf]),
-                   ops_arg_dat_opt(g.m, 1, st, "double", OPS_RW, fl[g.mf]),
-                   ops_arg_gbl(fl, NUM_FIELDS, "int", OPS_READ),
-                   ops_arg_gbl(g.faces[s].m, 6, "int", OPS_READ), ops_arg_idx());
+  // the groups touch different dats, so they go axis by axis: per axis one launch covers
+  // both faces of every group; each group still sees its faces in the reference's order
+  for (int s0 = 0; s0 < 6; s0 += 2) {
+    ops_hip_batch_begin();
+    for (Group &g : groups) {
+      if (!fl[g.vf] && !fl[g.mf]) continue;
+      for (int s = s0; s < s0 + 2; s++) {
+        ops_stencil st = S.H[s];
+        ops_par_loop(c3_halo_flux, "update_halo_kernel_flux", S.blk, 3, g.faces[s].r,
+                     ops_arg_dat_opt(g.v, 1, st, "double", OPS_RW, fl[g.vf]),
+                     ops_arg_dat_opt(g.m, 1, st, "double", OPS_RW, fl[g.mf]),
+                     ops_arg_gbl(fl, NUM_FIELDS, "int", OPS_READ),
+                     ops_arg_gbl(g.faces[s].m, 6, "int", OPS_READ), ops_arg_idx());
+      }
     }
+    ops_hip_batch_end();
   }
 }
 

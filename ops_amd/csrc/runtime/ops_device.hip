@@ -196,6 +196,41 @@ extern "C" void *ops_hip_get_stream(void) {
 void *ops_hip_get_stream_raw() { return (void *)g_dev.stream; }
 extern "C" void ops_hip_device_sync(void) { ops_dev_sync(); }
 
+// ------------------------------------------------------------------ loop batches
+
+namespace {
+bool g_batch_on = false;
+int g_batch_kid = -1;             // kernel whose batch is pending
+void (*g_batch_fn)(void) = nullptr;  // the pending batch's launcher
+bool g_batch_keep = false;        // loop_begin of the pending kernel: do not flush it
+}  // namespace
+
+void ops_loop_batch_flush() {
+  if (!g_batch_fn || g_batch_keep) return;
+  void (*f)(void) = g_batch_fn;
+  g_batch_fn = nullptr;
+  g_batch_kid = -1;
+  f();
+}
+
+extern "C" void ops_hip_batch_begin(void) {
+  ops_trid_flush();
+  g_batch_on = true;
+}
+
+extern "C" void ops_hip_batch_end(void) {
+  g_batch_on = false;
+  ops_loop_batch_flush();
+}
+
+extern "C" int ops_hip_batch_join(int kernel_id, void (*flush)(void)) {
+  if (!g_batch_on || kernel_id < 0) return 0;
+  if (g_batch_fn && g_batch_kid != kernel_id) ops_loop_batch_flush();
+  g_batch_kid = kernel_id;
+  g_batch_fn = flush;
+  return 1;
+}
+
 // ------------------------------------------------------------------ timing
 
 extern "C" void ops_hip_set_timing(int on) { g_timing = on; }
@@ -378,7 +413,11 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
                                   ops_block block, int dim, const int *range, ops_arg *args,
                                   int nargs) {
   OPS_instance *in = ops_instance_get();
+  // another launch of the kernel whose batch is pending may join that batch: everything
+  // else pending runs first
+  g_batch_keep = g_batch_on && kernel_id >= 0 && kernel_id == g_batch_kid;
   ops_trid_flush();
+  g_batch_keep = false;
   if (nargs > OPS_HIP_MAX_ARGS) ops_throw(OPS_NOT_IMPLEMENTED, "too many loop arguments");
   lp->kernel_id = kernel_id;
   lp->ndim = dim;

@@ -79,6 +79,8 @@ void *ops_hip_get_stream_raw();
 // launches the deferred tridiagonal batch, if any (every launch, copy and synchronisation
 // of the device layer calls it first)
 void ops_trid_flush();
+// launches the pending loop batch, if any (ops_trid_flush calls it first)
+void ops_loop_batch_flush();
 // Autotuning seam. ops_tune_next returns the candidate (0..ncand-1) the next launch of
 // kernel kid should use while it is being measured, the chosen one afterwards, or -1 when
 // tuning is off (or the kernel is not tunable). A measured launch is bracketed by

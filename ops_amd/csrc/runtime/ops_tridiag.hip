@@ -390,6 +390,9 @@ void enqueue(int ndim, int solvedim, const int *range, ops_dat a, ops_dat b, ops
 }  // namespace
 
 void ops_trid_flush() {
+  // a pending loop batch was issued before any pending tridiagonal call (a loop's launch
+  // flushes the tridiagonal queue first), so it runs first
+  ops_loop_batch_flush();
   if (g_pending.calls.empty() || g_pending.flushing) return;
   g_pending.flushing = true;
   std::vector<Call> calls;

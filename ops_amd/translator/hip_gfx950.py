@@ -291,6 +291,17 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
             L.append(f"  {t} *rp{i};")
     L.append("};")
     L.append("")
+    if batchable(loop):
+        # a batch of up to 8 launches travels by value as the kernel argument (kept under
+        # 4 KB: fewer loops per batch for large argument structs)
+        L.append(f"constexpr int ops_nb_{k} = (int)(3000 / sizeof(ops_kp_{k})) < 8 ? "
+                 f"(int)(3000 / sizeof(ops_kp_{k})) : 8;")
+        L.append(f"struct ops_kb_{k} {{")
+        L.append(f"  ops_kp_{k} p[ops_nb_{k} > 0 ? ops_nb_{k} : 1];")
+        L.append("  unsigned first[9], g0[8], g1[8], b0[8], b1[8], b2[8];")
+        L.append("  int n;")
+        L.append("};")
+        L.append("")
     if noalias_variant(loop):
         # the same body twice: with every dat as a no-alias pointer parameter (the usual
         # launch: loads need not wait behind the stores of other arguments), and reading
@@ -303,7 +314,46 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
         L.append(kernel_body(loop, params, ndim, k + "_alias", False, v1))
     else:
         L.append(kernel_body(loop, params, ndim, k, False, v1))
+    if batchable(loop):
+        L.append("")
+        L.append(batch_kernel(loop, params, ndim, v1))
     return "\n".join(L)
+
+
+def batchable(loop: Loop) -> bool:
+    """Loops that may join a launch batch: no reductions (their per-workgroup partials and
+    finisher belong to one launch), no prolong addressing, one point per lane."""
+    return (not any(a.kind == "reduce" for a in loop.args) and
+            not any(a.kind == "dat" and a.prolong for a in loop.args) and loop_vx(loop) == 1)
+
+
+def batch_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool) -> str:
+    """The batch variant: the aliasing-safe body (pointers from the argument struct) behind
+    a prologue that finds the workgroup's loop in the batch (prefix sums of workgroups),
+    then its tile (row-major over that loop's grid, no XCD remap: batches are small) and the
+    lane's position in that loop's workgroup shape (threads beyond it return; the body has
+    no barriers)."""
+    k = loop.stub
+    body = kernel_body(loop, params, ndim, k + "_batch", False, v1)
+    sig = f"void ops_kern_{k}_batch(const ops_kp_{k} p) {{"
+    assert sig in body
+    body = body.replace(sig, f"void ops_kern_{k}_batch(const ops_kb_{k} B) {{")
+    pro = ("  unsigned bx, by, bz;\n  ops_block_coords(bx, by, bz);\n"
+           "  const unsigned tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;")
+    assert pro in body
+    body = body.replace(pro, "\n".join([
+        "  int j = 0;",
+        "  while (j + 1 < B.n && blockIdx.x >= B.first[j + 1]) j++;",
+        f"  const ops_kp_{k} &p = B.p[j];",
+        "  const unsigned bdx = B.b0[j], bdy = B.b1[j], bdz = B.b2[j];",
+        "  const unsigned lb = blockIdx.x - B.first[j];",
+        "  const unsigned bx = lb % B.g0[j], by = (lb / B.g0[j]) % B.g1[j], bz = lb / (B.g0[j] * B.g1[j]);",
+        "  const unsigned tx = threadIdx.x % bdx, ty = (threadIdx.x / bdx) % bdy, tz = threadIdx.x / (bdx * bdy);",
+        "  if (tz >= bdz) return;"]))
+    for d in "xyz":
+        body = body.replace(f"blockDim.{d}", f"bd{d}")
+    assert "blockDim" not in body and "threadIdx.y" not in body
+    return body
 
 
 def noalias_variant(loop: Loop) -> bool:
@@ -449,8 +499,24 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     k = loop.stub
     n = len(loop.args)
     sig = ", ".join(f"ops_arg arg{i}" for i in range(n))
-    L = [f"void ops_par_loop_{k}(char const *name, ops_block block, int dim, int *range"
-         + (f", {sig}" if n else "") + ") {"]
+    L = []
+    bat = batchable(loop)
+    if bat:
+        # the pending batch of this loop's kernel and its launcher
+        L += [f"static {ns}::ops_kb_{k} ops_batch_{k};",
+              f"static int ops_batch_n_{k} = 0;",
+              f"static void *ops_batch_stream_{k} = nullptr;",
+              f"static void ops_batch_flush_{k}(void) {{",
+              f"  const int n = ops_batch_n_{k};",
+              "  if (n == 0) return;",
+              f"  ops_batch_n_{k} = 0;",
+              f"  ops_batch_{k}.n = n;",
+              f"  hipLaunchKernelGGL({ns}::ops_kern_{k}_batch, dim3(ops_batch_{k}.first[n]), dim3(256), 0, "
+              f"(hipStream_t)ops_batch_stream_{k}, ops_batch_{k});",
+              f"  ops_hip_check_launch(\"{k} (batch)\");",
+              "}", ""]
+    L += [f"void ops_par_loop_{k}(char const *name, ops_block block, int dim, int *range"
+          + (f", {sig}" if n else "") + ") {"]
     L.append(f"  static int kid = ops_hip_register_kernel(\"{loop.kernel}\");")
     L.append(f"  ops_arg args[{max(n, 1)}] = {{{', '.join(f'arg{i}' for i in range(n)) or 'ops_arg_idx()'}}};")
     L.append("  ops_hip_loop lp;")
@@ -476,6 +542,24 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
             L.append(f"    p.rp{i} = ({t} *)lp.red_part[{i}];")
     launch = ("hipLaunchKernelGGL({}, dim3(lp.grid[0], lp.grid[1], lp.grid[2]), "
               "dim3(lp.block[0], lp.block[1], lp.block[2]), 0, (hipStream_t)lp.stream, p{});")
+    if bat:
+        # join the pending batch (an untimed launch while batching is on), else launch alone
+        # after whatever this kernel has pending
+        L.append(f"    if ({ns}::ops_nb_{k} >= 2 && !lp.timed && "
+                 f"ops_hip_batch_join(kid, ops_batch_flush_{k})) {{")
+        L.append(f"      if (ops_batch_n_{k} == {ns}::ops_nb_{k}) ops_batch_flush_{k}();")
+        L.append(f"      const int j = ops_batch_n_{k}++;")
+        L.append(f"      auto &B = ops_batch_{k};")
+        L.append("      if (j == 0) B.first[0] = 0;")
+        L.append("      B.p[j] = p;")
+        L.append("      B.g0[j] = lp.grid[0]; B.g1[j] = lp.grid[1];")
+        L.append("      B.b0[j] = lp.block[0]; B.b1[j] = lp.block[1]; B.b2[j] = lp.block[2];")
+        L.append("      B.first[j + 1] = B.first[j] + (unsigned)lp.nblocks;")
+        L.append(f"      ops_batch_stream_{k} = lp.stream;")
+        L.append("      ops_hip_loop_end(&lp, args, " + str(n) + ");")
+        L.append("      return;")
+        L.append("    }")
+        L.append(f"    ops_batch_flush_{k}();")
     if noalias_variant(loop):
         ptrs = "".join(f", p.p{i}" for i, a in enumerate(loop.args) if a.kind == "dat")
         L.append("    if (lp.alias)")

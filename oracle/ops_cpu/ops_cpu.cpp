@@ -789,6 +789,10 @@ extern "C" void ops_cpu_set_threads(int n) {
 
 // graph capture is a device feature: the host backend reports it unavailable, so callers
 // run their launches eagerly
+// loop batches only change how launches are grouped: the host backend runs every loop as
+// it comes
+extern "C" void ops_hip_batch_begin(void) {}
+extern "C" void ops_hip_batch_end(void) {}
 extern "C" int ops_hip_graph_begin(void) { return 1; }
 extern "C" void *ops_hip_graph_end(void) { return nullptr; }
 extern "C" int ops_hip_graph_replay(void *, int) { return 1; }
