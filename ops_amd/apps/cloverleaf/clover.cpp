@@ -59,6 +59,8 @@ void ops_hip_graph_free(void *graph);
 void *ops_hip_const_device_ptr(const char *name);
 void *ops_hip_reduction_device_ptr(ops_reduction h);
 void ops_hip_reduction_discard(ops_reduction h);
+void ops_hip_batch_begin(void);
+void ops_hip_batch_end(void);
 }
 
 namespace {
@@ -356,6 +358,10 @@ bool any(const int *f, std::initializer_list<int> ids) {
 // reflective boundary update of the flagged fields, `depth` halo layers
 void update_halo(int *fl, int depth) {
   const int x0 = field.x_min, x1 = field.x_max, y0 = field.y_min, y1 = field.y_max;
+  // the two faces of an axis are independent when the mesh has more than 2*depth+1 cells
+  // along it (a face's mirror source then stays clear of the other face's halo layers);
+  // on thinner meshes they run one after the other
+  const bool batch_axis[2] = {y1 - y0 > 2 * depth + 1, x1 - x0 > 2 * depth + 1};
   if (any(fl, {FIELD_DENSITY0, FIELD_DENSITY1, FIELD_ENERGY0, FIELD_ENERGY1, FIELD_PRESSURE,
                FIELD_VISCOSITY, FIELD_SOUNDSPEED})) {
     // cell centres: layer h mirrors h' = 2*pivot - 1 - h
@@ -368,6 +374,8 @@ void update_halo(int *fl, int depth) {
     int *ms[4] = {mb, mt, ml, mr};
     int *rs[4] = {rb, rt, rl, rr};
     for (int s = 0; s < 4; s++) {
+      // one launch per axis (faces 0-1: y, 2-3: x), axes in the reference's order
+      if (s % 2 == 0 && batch_axis[s / 2]) ops_hip_batch_begin();
       ops_stencil st = S.H[s];
       ops_par_loop(cl_halo_cells, "update_halo_kernel1", S.blk, 2, rs[s],
                    ops_arg_dat_opt(S.density0, 1, st, "double", OPS_RW, fl[FIELD_DENSITY0]),
@@ -379,6 +387,7 @@ void update_halo(int *fl, int depth) {
                    ops_arg_dat_opt(S.soundspeed, 1, st, "double", OPS_RW, fl[FIELD_SOUNDSPEED]),
                    ops_arg_gbl(fl, NUM_FIELDS, "int", OPS_READ),
                    ops_arg_gbl(ms[s], 5, "int", OPS_READ), ops_arg_idx());
+      if (s % 2 == 1) ops_hip_batch_end();
     }
   }
   if (any(fl, {FIELD_XVEL0, FIELD_XVEL1, FIELD_YVEL0, FIELD_YVEL1})) {
@@ -392,6 +401,8 @@ void update_halo(int *fl, int depth) {
     int *ms[4] = {mb, mt, ml, mr};
     int *rs[4] = {rb, rt, rl, rr};
     for (int s = 0; s < 4; s++) {
+      // one launch per axis (faces 0-1: y, 2-3: x), axes in the reference's order
+      if (s % 2 == 0 && batch_axis[s / 2]) ops_hip_batch_begin();
       ops_stencil st = S.H[s];
       ops_par_loop(cl_halo_vel, "update_halo_kernel2_vel", S.blk, 2, rs[s],
                    ops_arg_dat_opt(S.xvel0, 1, st, "double", OPS_RW, fl[FIELD_XVEL0]),
@@ -400,6 +411,7 @@ void update_halo(int *fl, int depth) {
                    ops_arg_dat_opt(S.yvel1, 1, st, "double", OPS_RW, fl[FIELD_YVEL1]),
                    ops_arg_gbl(fl, NUM_FIELDS, "int", OPS_READ),
                    ops_arg_gbl(ms[s], 5, "int", OPS_READ), ops_arg_idx());
+      if (s % 2 == 1) ops_hip_batch_end();
     }
   }
   if (any(fl, {FIELD_VOL_FLUX_X, FIELD_MASS_FLUX_X})) {
@@ -414,12 +426,15 @@ void update_halo(int *fl, int depth) {
     int *ms[4] = {mb, mt, ml, mr};
     int *rs[4] = {rb, rt, rl, rr};
     for (int s = 0; s < 4; s++) {
+      // one launch per axis (faces 0-1: y, 2-3: x), axes in the reference's order
+      if (s % 2 == 0 && batch_axis[s / 2]) ops_hip_batch_begin();
       ops_stencil st = S.H[s];
       ops_par_loop(cl_halo_flux, "update_halo_kernel3_flux_x", S.blk, 2, rs[s],
                    ops_arg_dat_opt(S.vol_flux_x, 1, st, "double", OPS_RW, fl[FIELD_VOL_FLUX_X]),
                    ops_arg_dat_opt(S.mass_flux_x, 1, st, "double", OPS_RW, fl[FIELD_MASS_FLUX_X]),
                    ops_arg_gbl(fl, NUM_FIELDS, "int", OPS_READ),
                    ops_arg_gbl(ms[s], 6, "int", OPS_READ), ops_arg_idx());
+      if (s % 2 == 1) ops_hip_batch_end();
     }
   }
   if (any(fl, {FIELD_VOL_FLUX_Y, FIELD_MASS_FLUX_Y})) {
@@ -434,12 +449,15 @@ void update_halo(int *fl, int depth) {
     int *ms[4] = {mb, mt, ml, mr};
     int *rs[4] = {rb, rt, rl, rr};
     for (int s = 0; s < 4; s++) {
+      // one launch per axis (faces 0-1: y, 2-3: x), axes in the reference's order
+      if (s % 2 == 0 && batch_axis[s / 2]) ops_hip_batch_begin();
       ops_stencil st = S.H[s];
       ops_par_loop(cl_halo_flux, "update_halo_kernel4_flux_y", S.blk, 2, rs[s],
                    ops_arg_dat_opt(S.vol_flux_y, 1, st, "double", OPS_RW, fl[FIELD_VOL_FLUX_Y]),
                    ops_arg_dat_opt(S.mass_flux_y, 1, st, "double", OPS_RW, fl[FIELD_MASS_FLUX_Y]),
                    ops_arg_gbl(fl, NUM_FIELDS, "int", OPS_READ),
                    ops_arg_gbl(ms[s], 6, "int", OPS_READ), ops_arg_idx());
+      if (s % 2 == 1) ops_hip_batch_end();
     }
   }
 }

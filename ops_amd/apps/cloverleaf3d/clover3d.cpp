@@ -447,6 +447,10 @@ void update_halo(int *fl, int depth) {
   const int d = depth;
   const int x0 = field.x_min, x1 = field.x_max, y0 = field.y_min, y1 = field.y_max;
   const int z0 = field.z_min, z1 = field.z_max;
+  // the two faces of an axis are independent when the mesh has more than 2*d+1 cells
+  // along it (a face's mirror source then stays clear of the other face's halo layers);
+  // on thinner meshes they run one after the other. Faces 0-1: y, 2-3: x, 4-5: z.
+  const bool batch_axis[3] = {y1 - y0 > 2 * d + 1, x1 - x0 > 2 * d + 1, z1 - z0 > 2 * d + 1};
   if (any(fl, {FIELD_DENSITY0, FIELD_DENSITY1, FIELD_ENERGY0, FIELD_ENERGY1, FIELD_PRESSURE,
                FIELD_VISCOSITY, FIELD_SOUNDSPEED})) {
     // cell centres: layer h mirrors h' = 2*pivot - 1 - h
@@ -458,9 +462,8 @@ void update_halo(int *fl, int depth) {
         {{2, z0, 1}, {x0 - d, x1 + d, y0 - d, y1 + d, z0 - d, z0}},
         {{2, z1, 1}, {x0 - d, x1 + d, y0 - d, y1 + d, z1, z1 + d}}};
     for (int s = 0; s < 6; s++) {
-      // the two faces of an axis are independent (each writes its own halo layers and
-      // reads interior layers): one launch per axis, axes in the reference's order
-      if (s % 2 == 0) ops_hip_batch_begin();
+      // one launch per axis, axes in the reference's order
+      if (s % 2 == 0 && batch_axis[s / 2]) ops_hip_batch_begin();
       ops_stencil st = S.H[s];
       ops_par_loop(c3_halo_cells, "update_halo_kernel1", S.blk, 3, faces[s].r,
                    ops_arg_dat_opt(S.density0, 1, st, "double", OPS_RW, fl[FIELD_DENSITY0]),
@@ -485,7 +488,7 @@ void update_halo(int *fl, int depth) {
         {{2, z0, 0, 1, 1, -1}, {x0 - d, x1 + 1 + d, y0 - d, y1 + 1 + d, z0 - d, z0}},
         {{2, z1, 0, 1, 1, -1}, {x0 - d, x1 + 1 + d, y0 - d, y1 + 1 + d, z1 + 1, z1 + 1 + d}}};
     for (int s = 0; s < 6; s++) {
-      if (s % 2 == 0) ops_hip_batch_begin();
+      if (s % 2 == 0 && batch_axis[s / 2]) ops_hip_batch_begin();
       ops_stencil st = S.H[s];
       ops_par_loop(c3_halo_vel, "update_halo_kernel2_vel", S.blk, 3, faces[s].r,
                    ops_arg_dat_opt(S.xvel0, 1, st, "double", OPS_RW, fl[FIELD_XVEL0]),
@@ -532,7 +535,7 @@ void update_halo(int *fl, int depth) {
   // the groups touch different dats, so they go axis by axis: per axis one launch covers
   // both faces of every group; each group still sees its faces in the reference's order
   for (int s0 = 0; s0 < 6; s0 += 2) {
-    ops_hip_batch_begin();
+    if (batch_axis[s0 / 2]) ops_hip_batch_begin();
     for (Group &g : groups) {
       if (!fl[g.vf] && !fl[g.mf]) continue;
       for (int s = s0; s < s0 + 2; s++) {
