@@ -43,6 +43,17 @@ __device__ __forceinline__ void ops_block_coords(unsigned &bx, unsigned &by, uns
   bz = t / gy;
 }
 
+/* Tile coordinates of tile vt of a loop walked grid-stride (loops with reductions run on a
+   capped 1-D grid): the same XCD remap over all n tiles of the tile grid gx x gy x ... */
+__device__ __forceinline__ void ops_tile_coords(unsigned vt, unsigned n, unsigned gx, unsigned gy,
+                                                unsigned &bx, unsigned &by, unsigned &bz) {
+  unsigned t = ops_xcd_remap(vt, n);
+  bx = t % gx;
+  t /= gx;
+  by = t % gy;
+  bz = t / gy;
+}
+
 /* ---- reduction operators ---- */
 template <typename T> struct ops_red_inc {
   static __device__ __forceinline__ T identity() { return T(0); }

@@ -10,7 +10,7 @@
  *
  * A generated stub calls ops_hip_loop_begin(), launches its kernel with the geometry and
  * device addresses the descriptor holds, then calls ops_hip_loop_end(). Reductions are
- * finished on the device (per-block partials, the last block folds them into the
+ * finished on the device (per-workgroup partials, folded by a finishing kernel into the
  * handle's device value); only ops_reduction_result moves the value to the host.
  */
 #ifndef OPS_AMD_HIP_RT_H
@@ -58,6 +58,12 @@ typedef struct ops_hip_loop {
                     generated stub then launches the kernel variant without no-alias
                     pointer parameters */
   double bytes;  /* ops_compute_transfer bytes of this launch */
+  void (*batch)(void); /* set by the stub: the launcher of its pending loop batch (NULL
+                          when the loop cannot join one) */
+  /* loops with reductions: the launch is a capped 1-D grid of nblocks workgroups that walk
+     the `tiles` tiles of the tile grid tgrid[0] x tgrid[1] x tgrid[2] grid-stride */
+  unsigned tiles;
+  unsigned tgrid[3];
 } ops_hip_loop;
 
 /* Registers a generated loop name once; returns its id for timing and tuning tables. */
@@ -74,10 +80,18 @@ int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *name, ops_bl
 /* Completes bookkeeping after the launch: dirty bits, reduction state, timing. */
 void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs);
 
-/* Registers a device symbol (its host handle, HIP_SYMBOL(x)) as the home of an
-   ops_decl_const name; called from the generated translation unit's static initialiser.
-   nbytes < 0: runtime-sized, the symbol holds a device pointer the runtime allocates. */
-void ops_hip_register_const(const char *name, const void *symbol, long nbytes);
+/* Constants. A translated application defines ops_decl_const_char and
+   ops_hip_const_device_ptr itself, over its own table of device symbols (the reference's
+   generated master file defines ops_decl_const_char per application,
+   templates/cpp/cuda/master_kernel.cpp.j2:40-67), and calls these with the symbol's host
+   handle (HIP_SYMBOL(x)) and its declared size; nbytes < 0: runtime-sized, the symbol
+   holds a device pointer the runtime allocates and owns until ops_exit.
+   ops_hip_const_set uploads `bytes` bytes of `data` in stream order;
+   ops_hip_const_symbol_ptr returns the device address of a fixed-size symbol (throws
+   OPS_INVALID_ARGUMENT for a null or runtime-sized one). */
+void ops_hip_const_set(const void *symbol, long nbytes, const char *name, const char *data,
+                       long bytes);
+void *ops_hip_const_symbol_ptr(const void *symbol, long nbytes, const char *name);
 
 /* Checks a HIP status from generated code; throws OPSException(OPS_RUNTIME_ERROR). */
 void ops_hip_check(int status, const char *what, const char *file, int line);
@@ -111,7 +125,8 @@ void ops_hip_reset_stats(void);
 void ops_hip_batch_begin(void);
 void ops_hip_batch_end(void);
 /* for generated stubs: 1 if this launch joins a batch (batching on); `flush` launches the
-   stub's pending batch, and is called by the runtime before any other operation */
+   stub's pending batch (it also identifies the batch: a launch of another stub launches
+   the pending one first), and is called by the runtime before any other operation */
 int ops_hip_batch_join(int kernel_id, void (*flush)(void));
 int ops_hip_graph_begin(void);
 void *ops_hip_graph_end(void);

@@ -64,6 +64,7 @@ SIGNATURES = {
         "api_alias": (_i, [_i, _pp, _i, _i, _p, _p]),
         "api_random": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
         "api_random_normal": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
+        "api_batch": (_i, [_i, _pp, _i, _i, _p]),
     },
     "multicomp": {
         "multicomp_run": (_i, [_i, _pp, _i, _i, _p]),

@@ -11,6 +11,9 @@
 //                  accumulation across two loops before ops_reduction_result
 //   api_random[_normal]  ops_randomgen_init + ops_fill_random_uniform / _normal on a dat, read back with
 //                  its halos
+//   api_batch      loop batches (ops_hip_batch_begin/end): many independent launches of one
+//                  kernel of mixed extents, two stubs of that kernel interleaved, a
+//                  tridiagonal solve between launches, eagerly and replayed from a graph
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -19,6 +22,8 @@
 
 #define OPS_2D
 #include <ops_seq_v2.h>
+#include <ops_hip_rt.h>
+#include <ops_tridiag.h>
 
 #include "apitest_kernels.h"
 
@@ -334,4 +339,105 @@ extern "C" int api_random(int argc, const char **argv, unsigned seed, int nx, in
 extern "C" int api_random_normal(int argc, const char **argv, unsigned seed, int nx, int ny,
                                  double *out) {
   return random_fill(argc, argv, seed, nx, ny, out, 1);
+}
+
+// Loop batches. nloops dats O_j (extents x 3 + 37j mod 70, y 2 + 11j mod 9: narrow and wide
+// launches) are set to O_j = c_j S_j + x + 1000 y from sources S_j = j + x / 7 - y / 3 by
+// api_bset, alternately through its WRITE stub (even j) and its RW stub (odd j), in one
+// sequence. mode bit 0: the sequence runs between ops_hip_batch_begin and _end; bit 1: a
+// tridiagonal solve along x of O_1 (coefficients a = c = -1, b = 4) is issued after the
+// launch of O_1 and before the later launches (the solve reads what the batched launch
+// wrote); bit 2: the sequence is captured as a graph (after one eager run) and replayed
+// twice, O_j cleared in between, the second replay's result returned. out: every O_j's
+// interior, x fastest, j after j.
+extern "C" int api_batch(int argc, const char **argv, int nloops, int mode, double *out) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block blk = ops_decl_block(2, "batch");
+    int base[2] = {0, 0}, dm[2] = {-1, -1}, dp[2] = {1, 1};
+    double *none = nullptr;
+    std::vector<ops_dat> S, O;
+    std::vector<std::vector<int>> sizes;
+    for (int j = 0; j < nloops; j++) {
+      int size[2] = {3 + (37 * j) % 70, 2 + (11 * j) % 9};
+      sizes.push_back({size[0], size[1]});
+      S.push_back(ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "s"));
+      O.push_back(ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "o"));
+      std::vector<double> v((size_t)size[0] * size[1]);
+      for (int y = 0; y < size[1]; y++)
+        for (int x = 0; x < size[0]; x++) v[(size_t)y * size[0] + x] = j + x / 7.0 - y / 3.0;
+      int full[4] = {0, size[0], 0, size[1]};
+      ops_dat_set_data_slab_memspace(S[j], 0, (char *)v.data(), full, OPS_HOST);
+    }
+    int s00[] = {0, 0};
+    ops_stencil S00 = ops_decl_stencil(2, 1, s00, "00");
+    ops_dat TA = nullptr, TB = nullptr, TC = nullptr;
+    const bool solve = (mode & 2) && nloops > 1;
+    if (solve) {
+      int size[2] = {sizes[1][0], sizes[1][1]};
+      TA = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "ta");
+      TB = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "tb");
+      TC = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "tc");
+      std::vector<double> m1((size_t)size[0] * size[1], -1.0), m4((size_t)size[0] * size[1], 4.0);
+      int full[4] = {0, size[0], 0, size[1]};
+      ops_dat_set_data_slab_memspace(TA, 0, (char *)m1.data(), full, OPS_HOST);
+      ops_dat_set_data_slab_memspace(TB, 0, (char *)m4.data(), full, OPS_HOST);
+      ops_dat_set_data_slab_memspace(TC, 0, (char *)m1.data(), full, OPS_HOST);
+    }
+    std::vector<double> cs(nloops);
+    for (int j = 0; j < nloops; j++) cs[j] = 0.5 + j;
+    ops_tridsolver_params ctx(blk);
+    auto sequence = [&]() {
+      if (mode & 1) ops_hip_batch_begin();
+      for (int j = 0; j < nloops; j++) {
+        int r[4] = {0, sizes[j][0], 0, sizes[j][1]};
+        if (j % 2 == 0)
+          ops_par_loop(api_bset, "api_bset", blk, 2, r, ops_arg_dat(O[j], 1, S00, "double", OPS_WRITE),
+                       ops_arg_dat(S[j], 1, S00, "double", OPS_READ), ops_arg_idx(),
+                       ops_arg_gbl(&cs[j], 1, "double", OPS_READ));
+        else
+          ops_par_loop(api_bset, "api_bset", blk, 2, r, ops_arg_dat(O[j], 1, S00, "double", OPS_RW),
+                       ops_arg_dat(S[j], 1, S00, "double", OPS_READ), ops_arg_idx(),
+                       ops_arg_gbl(&cs[j], 1, "double", OPS_READ));
+        if (solve && j == 1) ops_tridMultiDimBatch(2, 0, r, TA, TB, TC, O[1], &ctx);
+      }
+      if (mode & 1) ops_hip_batch_end();
+    };
+    auto clear = [&]() {
+      for (int j = 0; j < nloops; j++) {
+        std::vector<double> z((size_t)sizes[j][0] * sizes[j][1], 0.0);
+        int full[4] = {0, sizes[j][0], 0, sizes[j][1]};
+        ops_dat_set_data_slab_memspace(O[j], 0, (char *)z.data(), full, OPS_HOST);
+      }
+    };
+    sequence();
+    if (mode & 4) {
+      void *g = nullptr;
+      if (ops_hip_graph_begin() == 0) {
+        sequence();
+        g = ops_hip_graph_end();
+      }
+      if (g) {
+        for (int rep = 0; rep < 2; rep++) {
+          clear();
+          if (ops_hip_graph_replay(g, 1) != 0) throw OPSException(OPS_RUNTIME_ERROR, "replay failed");
+          ops_hip_device_sync();
+        }
+        ops_hip_graph_free(g);
+      } else {
+        clear();
+        sequence();
+      }
+    }
+    size_t off = 0;
+    for (int j = 0; j < nloops; j++) {
+      int full[4] = {0, sizes[j][0], 0, sizes[j][1]};
+      ops_dat_fetch_data_slab_memspace(O[j], 0, (char *)(out + off), full, OPS_HOST);
+      off += (size_t)sizes[j][0] * sizes[j][1];
+    }
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_batch", e);
+  }
 }

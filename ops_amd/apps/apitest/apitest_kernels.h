@@ -25,4 +25,10 @@ void api_reduce(const ACC<double> &v, const ACC<int> &w, double *sum, double *mn
 // the kernel variant without no-alias pointers)
 void api_twice_plus_one(const ACC<double> &in, ACC<double> &out) { out(0, 0) = 2.0 * in(0, 0) + 1.0; }
 
+// o = c * i + idx[0] + 1000 * idx[1] (the loop-batch sequence: launched with `o` as WRITE
+// and as RW, which the translator turns into two stubs of one kernel)
+void api_bset(ACC<double> &o, const ACC<double> &i, const int *idx, const double *c) {
+  o(0, 0) = c[0] * i(0, 0) + (double)idx[0] + 1000.0 * (double)idx[1];
+}
+
 #endif

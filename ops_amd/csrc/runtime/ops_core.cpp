@@ -36,8 +36,8 @@ std::vector<ops_kernel_entry> &ops_kernel_table() {
   static std::vector<ops_kernel_entry> t;
   return t;
 }
-std::map<std::string, ops_const_entry> &ops_const_table() {
-  static std::map<std::string, ops_const_entry> t;
+std::map<const void *, ops_const_entry> &ops_const_table() {
+  static std::map<const void *, ops_const_entry> t;
   return t;
 }
 
@@ -659,16 +659,26 @@ extern "C" void ops_halo_transfer(ops_halo_group group) {
 
 // ---------------------------------------------------------------- consts
 
+// An application without translated kernels has no device home for its constants: its
+// ops_decl_const calls land here and are ignored. Translated applications define their own
+// ops_decl_const_char over their symbols (generated code), which calls ops_hip_const_set.
 extern "C" void ops_decl_const_char(OPS_instance *instance, int dim, char const *type,
                                     int type_size, char *data, char const *name) {
   (void)instance;
   (void)type;
-  auto &tab = ops_const_table();
-  auto it = tab.find(name);
-  if (it == tab.end()) return;  // not referenced by any generated kernel
-  ops_const_entry &e = it->second;
+  (void)dim;
+  (void)type_size;
+  (void)data;
+  (void)name;
+}
+
+extern "C" void ops_hip_const_set(const void *symbol, long nbytes, const char *name,
+                                  const char *data, long bytes) {
+  if (symbol == nullptr) return;
+  ops_const_entry &e = ops_const_table()[symbol];
+  e.symbol = symbol;
+  e.nbytes = nbytes;
   if (e.dev == nullptr) e.dev = ops_dev_symbol_address(e.symbol);
-  long bytes = (long)dim * type_size;
   if (e.nbytes >= 0) {
     if (bytes > e.nbytes)
       ops_throw(OPS_INVALID_ARGUMENT, std::string("Error: ops_decl_const -- constant ") + name +
@@ -688,15 +698,20 @@ extern "C" void ops_decl_const_char(OPS_instance *instance, int dim, char const 
 
 // device address of a fixed-size ops_decl_const symbol, for kernels that update a constant
 // on the device (a captured time step cannot upload it from the host)
-extern "C" void *ops_hip_const_device_ptr(const char *name) {
-  auto &tab = ops_const_table();
-  auto it = tab.find(name);
-  if (it == tab.end() || it->second.nbytes < 0)
+extern "C" void *ops_hip_const_symbol_ptr(const void *symbol, long nbytes, const char *name) {
+  if (symbol == nullptr || nbytes < 0)
     ops_throw(OPS_INVALID_ARGUMENT, std::string("ops_hip_const_device_ptr: no fixed-size "
-                                                "device constant named ") + name);
-  ops_const_entry &e = it->second;
+                                                "device constant named ") + (name ? name : ""));
+  ops_const_entry &e = ops_const_table()[symbol];
+  e.symbol = symbol;
+  e.nbytes = nbytes;
   if (e.dev == nullptr) e.dev = ops_dev_symbol_address(e.symbol);
   return e.dev;
+}
+
+// an application without translated kernels has no device constants
+extern "C" void *ops_hip_const_device_ptr(const char *name) {
+  return ops_hip_const_symbol_ptr(nullptr, 0, name);
 }
 
 extern "C" void *ops_hip_reduction_device_ptr(ops_reduction h) { return h->data_d; }
@@ -707,13 +722,6 @@ extern "C" void *ops_hip_reduction_device_ptr(ops_reduction h) { return h->data_
 extern "C" void ops_hip_reduction_discard(ops_reduction h) {
   h->dirty_d = 0;
   h->initialized = 0;
-}
-
-extern "C" void ops_hip_register_const(const char *name, const void *symbol, long nbytes) {
-  ops_const_entry &e = ops_const_table()[name];
-  e.symbol = symbol;
-  e.dev = nullptr;
-  e.nbytes = nbytes;
 }
 
 // ---------------------------------------------------------------- output / timers

@@ -87,6 +87,8 @@ extern "C" void ops_hip_check_launch(const char *name) {
   }
 }
 
+static void loop_batch_reset();
+
 void ops_dev_init(OPS_instance *inst) {
   if (g_dev.live) return;
   int n = 0;
@@ -104,12 +106,15 @@ void ops_dev_init(OPS_instance *inst) {
   g_dev.stage_off = 0;
   HIPCHECK(hipHostMalloc((void **)&g_dev.pinned_small, 1u << 16, hipHostMallocDefault));
   g_dev.live = true;
+  loop_batch_reset();
 }
 
 void ops_dev_exit(OPS_instance *inst) {
   (void)inst;
   if (!g_dev.live) return;
+  loop_batch_reset();
   ops_trid_flush();
+  ops_trid_release();
   ops_dev_timing_flush();
   HIPCHECK(hipStreamSynchronize(g_dev.stream));
   for (hipEvent_t ev : g_dev.pool) (void)hipEventDestroy(ev);
@@ -200,17 +205,25 @@ extern "C" void ops_hip_device_sync(void) { ops_dev_sync(); }
 
 namespace {
 bool g_batch_on = false;
-int g_batch_kid = -1;             // kernel whose batch is pending
-void (*g_batch_fn)(void) = nullptr;  // the pending batch's launcher
-bool g_batch_keep = false;        // loop_begin of the pending kernel: do not flush it
+// the pending batch's launcher: one per generated stub (a kernel translated for two
+// argument signatures has two stubs, and each keeps its own pending launches)
+void (*g_batch_fn)(void) = nullptr;
+bool g_batch_keep = false;  // loop_begin of the pending stub: do not flush it
 }  // namespace
 
 void ops_loop_batch_flush() {
   if (!g_batch_fn || g_batch_keep) return;
   void (*f)(void) = g_batch_fn;
   g_batch_fn = nullptr;
-  g_batch_kid = -1;
   f();
+}
+
+// batching off and nothing pending: at ops_init and ops_exit, so that a batch left open by
+// an exception does not outlive its session (ops_exit launches what is pending first)
+static void loop_batch_reset() {
+  g_batch_keep = false;
+  ops_loop_batch_flush();
+  g_batch_on = false;
 }
 
 extern "C" void ops_hip_batch_begin(void) {
@@ -224,9 +237,8 @@ extern "C" void ops_hip_batch_end(void) {
 }
 
 extern "C" int ops_hip_batch_join(int kernel_id, void (*flush)(void)) {
-  if (!g_batch_on || kernel_id < 0) return 0;
-  if (g_batch_fn && g_batch_kid != kernel_id) ops_loop_batch_flush();
-  g_batch_kid = kernel_id;
+  if (!g_batch_on || kernel_id < 0 || flush == nullptr) return 0;
+  if (g_batch_fn && g_batch_fn != flush) ops_loop_batch_flush();
   g_batch_fn = flush;
   return 1;
 }
@@ -413,9 +425,9 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
                                   ops_block block, int dim, const int *range, ops_arg *args,
                                   int nargs) {
   OPS_instance *in = ops_instance_get();
-  // another launch of the kernel whose batch is pending may join that batch: everything
+  // another launch of the stub whose batch is pending may join that batch: everything
   // else pending runs first
-  g_batch_keep = g_batch_on && kernel_id >= 0 && kernel_id == g_batch_kid;
+  g_batch_keep = g_batch_on && lp->batch != nullptr && lp->batch == g_batch_fn;
   ops_trid_flush();
   g_batch_keep = false;
   if (nargs > OPS_HIP_MAX_ARGS) ops_throw(OPS_NOT_IMPLEMENTED, "too many loop arguments");
@@ -605,6 +617,20 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   lp->grid[1] = (lp->count[1] + by - 1) / by;
   lp->grid[2] = (lp->count[2] + bz - 1) / bz;
   lp->nblocks = lp->grid[0] * lp->grid[1] * lp->grid[2];
+  lp->tiles = lp->nblocks;
+  for (int d = 0; d < 3; d++) lp->tgrid[d] = lp->grid[d];
+  // a loop with reductions runs on a 1-D grid of at most OPS_RED_GRID workgroups (a
+  // multiple of the 8 XCDs) that walk the tiles grid-stride: its finisher then folds at
+  // most that many partials per argument
+  bool has_red = false;
+  for (int i = 0; i < nargs; i++)
+    if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) has_red = true;
+  if (has_red) {
+    const unsigned g = lp->tiles < OPS_RED_GRID ? lp->tiles : OPS_RED_GRID;
+    lp->grid[0] = g;
+    lp->grid[1] = lp->grid[2] = 1;
+    lp->nblocks = g;
+  }
 
   // reduction scratch
   size_t need = 0;
@@ -632,6 +658,8 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       lp->red_value[i] = h->data_d;
       lp->red_init[i] = h->dirty_d ? 0 : 1;
     }
+  // a timed launch runs alone: the pending batch goes first, outside its time
+  if (lp->tune_cand >= 0 || g_timing) ops_loop_batch_flush();
   if (lp->tune_cand >= 0) {
     ops_tune_begin(kernel_id, lp->tune_cand,
                    (double)lp->count[0] * (double)lp->count[1] * (double)lp->count[2]);
@@ -646,74 +674,75 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
 }
 
 // ------------------------------------------------------------------ reduction finish
-// Every workgroup of a loop with reductions writes its partial; one 1024-lane workgroup
-// then folds them in a fixed order (eight independent lane-strided accumulators, then the
-// workgroup tree) and combines the result into the handle's device value: overwritten on
-// the handle's first use, else combined with it. No atomics, no device-scope fences.
+// Every workgroup of a loop with reductions writes its partial (at most OPS_RED_GRID per
+// argument); one launch then folds the partials of all the loop's reduction arguments, one
+// workgroup per argument: each lane combines a lane-strided share in a fixed order, then
+// the workgroup tree, and the result is combined into the handle's device value
+// (overwritten on the handle's first use). No atomics, no device-scope fences.
+
+struct RedEntry {
+  const void *part;
+  void *value;
+  unsigned nb;  // partials (workgroups of the loop)
+  int dim;
+  int op;       // OPS_INC, OPS_MIN, OPS_MAX
+  int type;     // 0 double, 1 float, 2 int
+  int init;
+};
+struct RedTable {
+  RedEntry e[OPS_HIP_MAX_ARGS];
+};
 
 template <typename OP, typename T>
-__global__ __launch_bounds__(1024) void red_finish_kernel(const T *__restrict__ part,
-                                                          unsigned nb, int dim, T *value,
-                                                          int init) {
-  __shared__ T lds[16];
+__device__ __forceinline__ void red_fold(const RedEntry &E, T *lds) {
+  const T *part = (const T *)E.part;
+  T *value = (T *)E.value;
   const unsigned tid = threadIdx.x;
-  for (int d = 0; d < dim; d++) {
-    T acc[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) acc[u] = OP::identity();
-    for (unsigned base = 0; base < nb; base += 8 * 1024) {
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const unsigned i = base + u * 1024 + tid;
-        if (i < nb) acc[u] = OP::op(acc[u], part[(size_t)i * dim + d]);
-      }
-    }
-    T a = acc[0];
-#pragma unroll
-    for (int u = 1; u < 8; u++) a = OP::op(a, acc[u]);
-    T r = ops_block_reduce<OP>(a, lds);
-    if (tid == 0) value[d] = init ? r : OP::op(value[d], r);
+  for (int d = 0; d < E.dim; d++) {
+    T a = OP::identity();
+    for (unsigned i = tid; i < E.nb; i += 256) a = OP::op(a, part[(size_t)i * E.dim + d]);
+    const T r = ops_block_reduce<OP>(a, lds);
+    if (tid == 0) value[d] = E.init ? r : OP::op(value[d], r);
   }
 }
 
-template <typename T>
-static void launch_finish(int acc, const ops_hip_loop *lp, int i, int dim) {
-  const T *part = (const T *)lp->red_part[i];
-  T *value = (T *)lp->red_value[i];
-  unsigned nb = (unsigned)lp->nblocks;
-  int init = lp->red_init[i];
-  hipStream_t st = g_dev.stream;
-  switch (acc) {
-    case OPS_INC:
-      hipLaunchKernelGGL((red_finish_kernel<ops_red_inc<T>, T>), dim3(1), dim3(1024), 0, st, part,
-                         nb, dim, value, init);
-      break;
-    case OPS_MIN:
-      hipLaunchKernelGGL((red_finish_kernel<ops_red_min<T>, T>), dim3(1), dim3(1024), 0, st, part,
-                         nb, dim, value, init);
-      break;
-    case OPS_MAX:
-      hipLaunchKernelGGL((red_finish_kernel<ops_red_max<T>, T>), dim3(1), dim3(1024), 0, st, part,
-                         nb, dim, value, init);
-      break;
-    default: ops_throw(OPS_INVALID_ARGUMENT, "reduction with an unsupported access mode");
+template <typename T> __device__ __forceinline__ void red_fold_op(const RedEntry &E, T *lds) {
+  if (E.op == OPS_INC) red_fold<ops_red_inc<T>, T>(E, lds);
+  else if (E.op == OPS_MIN) red_fold<ops_red_min<T>, T>(E, lds);
+  else red_fold<ops_red_max<T>, T>(E, lds);
+}
+
+__global__ __launch_bounds__(256) void red_finish_kernel(const RedTable t) {
+  __shared__ double lds[8];
+  const RedEntry &E = t.e[blockIdx.x];
+  if (E.type == 0) red_fold_op<double>(E, lds);
+  else if (E.type == 1) red_fold_op<float>(E, (float *)lds);
+  else red_fold_op<int>(E, (int *)lds);
+}
+
+static void finish_reductions(const ops_hip_loop *lp, const ops_arg *args, int nargs) {
+  RedTable t;
+  unsigned n = 0;
+  for (int i = 0; i < nargs; i++) {
+    const ops_arg &a = args[i];
+    if (a.argtype != OPS_ARG_GBL || a.acc == OPS_READ) continue;
+    if (a.acc != OPS_INC && a.acc != OPS_MIN && a.acc != OPS_MAX)
+      ops_throw(OPS_INVALID_ARGUMENT, "reduction with an unsupported access mode");
+    const char *ty = ((ops_reduction)a.data)->type;
+    int type;
+    if (!strcmp(ty, "double")) type = 0;
+    else if (!strcmp(ty, "float")) type = 1;
+    else if (!strcmp(ty, "int")) type = 2;
+    else ops_throw(OPS_NOT_IMPLEMENTED, std::string("reduction of type ") + ty);
+    t.e[n++] = {lp->red_part[i], lp->red_value[i], lp->nblocks, a.dim, a.acc, type, lp->red_init[i]};
   }
+  if (n == 0) return;
+  hipLaunchKernelGGL(red_finish_kernel, dim3(n), dim3(256), 0, g_dev.stream, t);
   ops_hip_check_launch("reduction finish");
 }
 
-static void finish_reduction(const ops_hip_loop *lp, const ops_arg &a, int i) {
-  ops_reduction h = (ops_reduction)a.data;
-  const char *t = h->type;
-  if (!strcmp(t, "double")) launch_finish<double>(a.acc, lp, i, a.dim);
-  else if (!strcmp(t, "float")) launch_finish<float>(a.acc, lp, i, a.dim);
-  else if (!strcmp(t, "int")) launch_finish<int>(a.acc, lp, i, a.dim);
-  else ops_throw(OPS_NOT_IMPLEMENTED, std::string("reduction of type ") + t);
-}
-
 extern "C" void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs) {
-  if (lp->nblocks > 0)
-    for (int i = 0; i < nargs; i++)
-      if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) finish_reduction(lp, args[i], i);
+  if (lp->nblocks > 0) finish_reductions(lp, args, nargs);
   if (lp->timed && lp->tune_cand >= 0) {
     ops_tune_end(lp->kernel_id, lp->ndim >= 2 ? 5 : 3, 2);
   } else if (lp->timed) {

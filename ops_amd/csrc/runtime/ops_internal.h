@@ -9,6 +9,10 @@
 #include "ops_hip_rt.h"
 #include "ops_lib_core.h"
 
+// workgroups of a loop with reductions (a multiple of the 8 XCDs): its tiles are walked
+// grid-stride, so the finisher folds at most this many partials per argument
+#define OPS_RED_GRID 2048u
+
 struct ops_kernel_entry {
   std::string name;
   long calls = 0;
@@ -58,10 +62,10 @@ OPS_instance *ops_instance_get(bool create = false);
 void ops_check_loop_bounds(const char *name, int dim, const int *start, const int *end,
                            const ops_arg *args, int nargs);
 
-// kernel + const registries live for the whole process (generated code registers them
-// from static initialisers, before any ops_init)
+// kernel registry (generated stubs register on first call) and the device homes of
+// constants, keyed by symbol (each application module has its own), for the whole process
 std::vector<ops_kernel_entry> &ops_kernel_table();
-std::map<std::string, ops_const_entry> &ops_const_table();
+std::map<const void *, ops_const_entry> &ops_const_table();
 
 // ---- device layer (ops_device.hip) ----
 void ops_dev_init(OPS_instance *inst);
@@ -81,6 +85,8 @@ void *ops_hip_get_stream_raw();
 void ops_trid_flush();
 // launches the pending loop batch, if any (ops_trid_flush calls it first)
 void ops_loop_batch_flush();
+// frees the tridiagonal solver's scratch (ops_exit; graphs that used it are gone by then)
+void ops_trid_release();
 // Autotuning seam. ops_tune_next returns the candidate (0..ncand-1) the next launch of
 // kernel kid should use while it is being measured, the chosen one afterwards, or -1 when
 // tuning is off (or the kernel is not tunable). A measured launch is bracketed by

@@ -230,16 +230,20 @@ __global__ __launch_bounds__(64) void thomas_x(const Batch<T> B, T *__restrict__
   }
 }
 
+// c', d' scratch. A graph captured earlier may still hold an outgrown buffer, so it is
+// retired, not freed, until ops_exit.
 struct Scratch {
   char *p = nullptr;
   size_t cap = 0;
+  std::vector<char *> retired;
 };
 Scratch g_scratch;
 
 void *scratch(size_t bytes) {
   if (bytes > g_scratch.cap) {
-    ops_dev_sync();
-    if (g_scratch.p) (void)hipFree(g_scratch.p);
+    if (g_scratch.p) g_scratch.retired.push_back(g_scratch.p);
+    g_scratch.p = nullptr;
+    g_scratch.cap = 0;
     HIPCHECK(hipMalloc((void **)&g_scratch.p, bytes));
     g_scratch.cap = bytes;
   }
@@ -381,6 +385,8 @@ void enqueue(int ndim, int solvedim, const int *range, ops_dat a, ops_dat b, ops
   C.c = c;
   C.d = d;
   C.u = u;
+  // a loop batch issued before this call runs before it
+  ops_loop_batch_flush();
   if (!compatible(C)) ops_trid_flush();
   g_pending.calls.push_back(C);
   if (inc) u->dirty_hd = 2;
@@ -417,6 +423,14 @@ void ops_trid_flush() {
 }
 
 void ops_trid_discard() { g_pending.calls.clear(); }
+
+void ops_trid_release() {
+  for (char *p : g_scratch.retired) (void)hipFree(p);
+  g_scratch.retired.clear();
+  if (g_scratch.p) (void)hipFree(g_scratch.p);
+  g_scratch.p = nullptr;
+  g_scratch.cap = 0;
+}
 
 extern "C" void ops_tridMultiDimBatch(int ndim, int solvedim, int *range, ops_dat a, ops_dat b,
                                       ops_dat c, ops_dat d, ops_tridsolver_params *ctx) {

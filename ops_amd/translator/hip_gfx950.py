@@ -278,6 +278,8 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
     # kernel-argument struct
     L.append(f"struct ops_kp_{k} {{")
     L.append("  int cnt0, cnt1, cnt2, st0, st1, st2;")
+    if any(a.kind == "reduce" for a in loop.args):
+        L.append("  unsigned ntiles, tg0, tg1;  // tile count and tile grid (grid-stride walk)")
     for i, a in enumerate(loop.args):
         if a.kind == "dat":
             t = params[i].ctype
@@ -392,8 +394,17 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
         d = const_dim(a, loop)
         L.append(f"  {t} r{i}[{d}];")
         L.append(f"  for (int d = 0; d < {d}; d++) r{i}[d] = {RED_OP[a.acc]}<{t}>::identity();")
-    L.append("  unsigned bx, by, bz;")
-    L.append("  ops_block_coords(bx, by, bz);")
+    if red:
+        # loops with reductions walk their tiles grid-stride over a capped 1-D grid (the
+        # runtime sets p.ntiles and the tile grid p.tg0 x p.tg1 x ...), so the partials to
+        # fold stay few; tile vt is placed with the XCD remap of the full tile count (the
+        # grid is a multiple of 8 workgroups, so all of a workgroup's tiles sit on its XCD)
+        L.append("  for (unsigned vt = blockIdx.x; vt < p.ntiles; vt += gridDim.x) {")
+        L.append("  unsigned bx, by, bz;")
+        L.append("  ops_tile_coords(vt, p.ntiles, p.tg0, p.tg1, bx, by, bz);")
+    else:
+        L.append("  unsigned bx, by, bz;")
+        L.append("  ops_block_coords(bx, by, bz);")
     # the workgroup's origin (x0, y0, z0) is wave-uniform, so its part of every address is
     # scalar arithmetic; the lane's part uses 24-bit multiplies (the runtime checks that the
     # x and y steps fit), where a full 32-bit multiply per argument and dimension would be
@@ -486,6 +497,8 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
     L.append("  }")
     if vx > 1:
         L.append("  }")
+    if red:
+        L.append("  }")
     for i in red:
         a = loop.args[i]
         t = CTYPES.get(a.typ, a.typ)
@@ -523,10 +536,13 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     L.append(f"  lp.vx = {loop_vx(loop)};")
     mask = sum(1 << i for i, a in enumerate(loop.args) if a.kind == "dat" and a.prolong)
     L.append(f"  lp.prolong_mask = {mask}u;")
+    L.append(f"  lp.batch = {'ops_batch_flush_' + k if bat else 'nullptr'};")
     L.append(f"  if (ops_hip_loop_begin(&lp, kid, name, block, dim, range, args, {n})) {{")
     L.append(f"    {ns}::ops_kp_{k} p;")
     L.append("    p.cnt0 = lp.count[0]; p.cnt1 = lp.count[1]; p.cnt2 = lp.count[2];")
     L.append("    p.st0 = lp.start[0]; p.st1 = lp.start[1]; p.st2 = lp.start[2];")
+    if any(a.kind == "reduce" for a in loop.args):
+        L.append("    p.ntiles = lp.tiles; p.tg0 = lp.tgrid[0]; p.tg1 = lp.tgrid[1];")
     for i, a in enumerate(loop.args):
         if a.kind == "dat":
             t = params[i].ctype
@@ -677,7 +693,7 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
     for c in consts.values():
         decl, size = const_decl(c)
         H.append(decl)
-        regs.append(f"    ops_hip_register_const(\"{c.name}\", (const void *)&{ns}::{c.name}, {size});")
+        regs.append(f"    {{\"{c.name}\", (const void *)&{ns}::{c.name}, {size}}},")
     H.append("")
     H.append("// ---- user kernels, compiled for host and gfx950")
     H.append("#pragma clang force_cuda_host_device begin")
@@ -707,13 +723,32 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
     H.append("")
     H += stub_defs
     H.append("")
+    # this application's ops_decl_const homes: the application defines ops_decl_const_char
+    # itself, as the reference's generated master file does (cuda/master_kernel.cpp.j2:
+    # 40-67), so two applications in one process that declare constants of one name (of
+    # different types) each write their own symbols (the library links -Bsymbolic)
     H.append("namespace {")
-    H.append("struct ops_const_registration {")
-    H.append("  ops_const_registration() {")
-    H += regs
-    H.append("  }")
-    H.append("} ops_const_registration_instance;")
+    H.append("struct ops_const_home { const char *name; const void *symbol; long nbytes; };")
+    H.append("const ops_const_home ops_const_homes[] = {")
+    H += regs or ["    {nullptr, nullptr, 0},"]
+    H.append("};")
+    H.append("const ops_const_home *ops_const_find(const char *name) {")
+    H.append("  for (const ops_const_home &h : ops_const_homes)")
+    H.append("    if (h.name && !strcmp(h.name, name)) return &h;")
+    H.append("  return nullptr;")
+    H.append("}")
     H.append("}  // namespace")
+    H.append("")
+    H.append("extern \"C\" void ops_decl_const_char(OPS_instance *, int dim, char const *, int type_size, "
+             "char *data, char const *name) {")
+    H.append("  const ops_const_home *h = ops_const_find(name);")
+    H.append("  if (h) ops_hip_const_set(h->symbol, h->nbytes, h->name, data, (long)dim * type_size);")
+    H.append("}")
+    H.append("")
+    H.append("extern \"C\" void *ops_hip_const_device_ptr(const char *name) {")
+    H.append("  const ops_const_home *h = ops_const_find(name);")
+    H.append("  return ops_hip_const_symbol_ptr(h ? h->symbol : nullptr, h ? h->nbytes : 0, name);")
+    H.append("}")
     kpath = out / f"{app_name}_kernels.hip"
     _write_if_changed(kpath, "\n".join(H) + "\n")
     # ---- rewritten host sources

@@ -793,6 +793,7 @@ extern "C" void ops_cpu_set_threads(int n) {
 // it comes
 extern "C" void ops_hip_batch_begin(void) {}
 extern "C" void ops_hip_batch_end(void) {}
+extern "C" void ops_hip_device_sync(void) {}
 extern "C" int ops_hip_graph_begin(void) { return 1; }
 extern "C" void *ops_hip_graph_end(void) { return nullptr; }
 extern "C" int ops_hip_graph_replay(void *, int) { return 1; }

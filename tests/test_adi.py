@@ -135,15 +135,45 @@ def test_gpu_autotuned_uneven_blocks_equal_oracle(monkeypatch):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.gpu
-def test_gpu_48_block_configuration_first_block_equals_oracle():
-    # the benchmark configuration: 48 blocks with extents in {64..256}, 100 steps; the
-    # field is drawn block after block, so the oracle reproduces block 0 alone
+GAMMA = 0x9E3779B97F4A7C15  # splitmix64 increment: the state after k draws is seed + k * GAMMA
+
+
+def skip_seed(seed, sizes, k):
+    """splitmix64 seed that starts block k's draws: the field is drawn block after block, x
+    fastest, so block k alone is generated by a generator advanced by the cells of blocks
+    0..k-1 (splitmix64's state is a counter, so skipping ahead is one multiply)."""
+    skip = sum(a * b * c for a, b, c in sizes[:k])
+    return (seed + skip * GAMMA) % (1 << 64)
+
+
+def test_skip_seed_reproduces_the_continued_stream():
+    # two small blocks drawn together equal the second block drawn alone from its skip seed
+    sizes = [(5, 4, 3), (6, 2, 7)]
+    both = run("cpu", sizes, 0)
+    alone = run("cpu", sizes[1:], 0, seed=skip_seed(5, sizes, 1))
+    assert np.array_equal(both[1], alone[0])
+    assert not np.array_equal(run("cpu", sizes[1:], 0)[0], both[1])
+
+
+# the 48-block benchmark configuration: batch boundaries of the 16-system launches (15/16,
+# 31/32), the first and the last block
+ADI_BLOCKS = (0, 15, 16, 31, 32, 47)
+
+
+@pytest.fixture(scope="module")
+def adi48_gpu():
     sizes = adi_sizes()
-    gpu = run("gpu", sizes, 100, fetch=[0, 47])
-    cpu = run("cpu", sizes[:1], 100)
-    assert np.array_equal(gpu[0], cpu[0])
-    assert np.isfinite(gpu[1]).all()
+    return sizes, dict(zip(ADI_BLOCKS, run("gpu", sizes, 100, fetch=list(ADI_BLOCKS))))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", ADI_BLOCKS)
+def test_gpu_48_block_configuration_equals_oracle(adi48_gpu, k):
+    # the benchmark configuration, 100 steps: block k of the GPU run against the oracle
+    # running block k alone from its skip-ahead seed
+    sizes, gpu = adi48_gpu
+    cpu = run("cpu", [sizes[k]], 100, seed=skip_seed(5, sizes, k), threads=16)[0]
+    assert np.array_equal(gpu[k], cpu)
 
 
 @pytest.mark.gpu

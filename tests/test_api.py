@@ -145,6 +145,60 @@ def test_oracle_halo3d_moves_every_value_once(case):
     assert np.array_equal(got, np.sort(src.ravel()))
 
 
+def halo3d_expected(c, A):
+    """B after the transfer, element by element, from the orientation rule of
+    ops_halo_transfer (ops/c/src/core/ops_device_singlenode_common.cpp:286-336 with
+    ops_halo_copy_tobuf/frombuf, doc/opsapi.md:192-212): iteration point q (one coordinate
+    per iteration axis) is read in dat dimension i of the source at
+    s_i + q[|dir_i| - 1] when dir_i > 0 and at s_i + iter[|dir_i| - 1] - 1 - q[|dir_i| - 1]
+    when dir_i < 0, with s_i = base_i - d_m_i (allocation index; dat base 0), and written
+    at the destination's point given by the same rule; a value crossing precisions goes
+    through the narrower type. Arrays are [z, y, x] of the allocations (2-deep halos)."""
+    nb = [s + 4 for s in c["sb"]]
+    B = np.zeros(nb[::-1])
+    it = c["it"]
+    q = np.indices(it).reshape(3, -1)  # q[a] = coordinate along iteration axis a
+
+    def side(base, dirs):
+        idx = []
+        for i in range(3):
+            a = abs(dirs[i]) - 1
+            s = base[i] + 2
+            idx.append(s + q[a] if dirs[i] > 0 else s + it[a] - 1 - q[a])
+        return idx
+
+    fx, fy, fz = side(c["fb"], c["fd"])
+    tx, ty, tz = side(c["tbse"], c["td"])
+    v = A[fz, fy, fx]
+    if c["ta"] != c["tb"]:
+        v = v.astype(np.float32).astype(np.float64)
+    B[tz, ty, tx] = v
+    return B
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("case", range(len(CASES3D)))
+def test_halo3d_positional(kind, case):
+    # every element of B, halo and interior, at the position the orientation rule gives
+    c = CASES3D[case]
+    A, B = halo3d(kind, c)
+    assert np.array_equal(B, halo3d_expected(c, A))
+
+
+def test_halo3d_expected_detects_a_flipped_axis():
+    # the restatement is sensitive to the orientation: flipping one axis of the destination
+    # or of the source changes the expected B
+    for case in (1, 2):
+        c = dict(CASES3D[case])
+        A, _ = halo3d("cpu", c)
+        ref = halo3d_expected(c, A)
+        flipped = dict(c, td=tuple(-d if k == 0 else d for k, d in enumerate(c["td"])))
+        assert not np.array_equal(halo3d_expected(flipped, A), ref)
+        if case == 2:
+            swapped = dict(c, fd=(c["fd"][0], -c["fd"][1], c["fd"][2]))
+            assert not np.array_equal(halo3d_expected(swapped, A), ref)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", range(len(CASES3D)))
 def test_gpu_halo3d_equals_oracle(case):
@@ -345,3 +399,56 @@ def test_random_normal_is_libstdcxx_polar(kind):
     assert L.api_random_normal(argc, av, 4321, nx, ny, p(out)) == 0
     # the whole allocation, halos included, in memory order (ops_lib_core.cpp:2452-2474)
     assert np.array_equal(out, mt_normal(4321, out.size))
+
+
+# ---------------------------------------------------------------- loop batches
+
+def batch_sizes(n):
+    return [(3 + (37 * j) % 70, 2 + (11 * j) % 9) for j in range(n)]
+
+
+def batch_run(kind, n, mode):
+    L = lib("apitest", kind)
+    out = np.zeros(sum(a * b for a, b in batch_sizes(n)))
+    argc, av = argv()
+    assert L.api_batch(argc, av, n, mode, p(out)) == 0
+    res, off = [], 0
+    for nx, ny in batch_sizes(n):
+        res.append(out[off:off + nx * ny].reshape(ny, nx))
+        off += nx * ny
+    return res
+
+
+def batch_expected(n, solve):
+    """O_j = c_j S_j + x + 1000 y with S_j = j + x/7 - y/3, c_j = 0.5 + j; with `solve`, O_1's
+    rows then go through the dense solve of tridiag(-1, 4, -1) along x."""
+    res = []
+    for j, (nx, ny) in enumerate(batch_sizes(n)):
+        y, x = np.mgrid[0:ny, 0:nx].astype(np.float64)
+        o = (0.5 + j) * (j + x / 7.0 - y / 3.0) + x + 1000.0 * y
+        if solve and j == 1:
+            M = 4.0 * np.eye(nx) - np.eye(nx, k=1) - np.eye(nx, k=-1)
+            o = np.linalg.solve(M, o.T).T
+        res.append(o)
+    return res
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_oracle_batch_sequence_closed_form(mode):
+    n = 11
+    got = batch_run("cpu", n, mode)
+    for g, e in zip(got, batch_expected(n, mode & 2)):
+        assert np.max(np.abs(g - e)) <= 1e-12 * np.max(np.abs(e))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", range(8))
+def test_gpu_batch_sequence_equals_oracle(mode):
+    # 11 (more than one batch of 8) and 17 launches of one kernel of mixed extents, its
+    # WRITE and RW stubs interleaved, batched or not, with a tridiagonal solve of a batched
+    # launch's output in between, eagerly or replayed from a graph: bitwise the oracle's
+    for n in (11, 17):
+        cpu = batch_run("cpu", n, mode & 2)
+        gpu = batch_run("gpu", n, mode)
+        for a, b in zip(cpu, gpu):
+            assert np.array_equal(a, b)

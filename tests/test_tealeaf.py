@@ -139,6 +139,44 @@ def test_gpu_plain_path_vs_reference(golden, tmp_path, name):
     assert np.max(np.abs(out["summary"] - ref) / np.abs(ref)) < 1e-10
 
 
+def maxrel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+FIELD_BAR = 1e-12  # FP64 fields and temperature against the reference, relative
+
+
+@pytest.mark.parametrize("name", ["tea_cg_64", "tea_cg_200"])
+def test_oracle_reordered_sums_stay_within_field_bar(golden, tmp_path, name):
+    # the reference's CG sums in one order; the GPU sums the same products in another. The
+    # oracle on 8 threads (partials combined in thread order) shows what reordering alone
+    # does to u, energy1 and the temperature over whole solves: far inside 1e-12
+    fx = golden(f"{name}.json")
+    one = tu.run("cpu", fx["deck"], dump_dir=str(tmp_path), threads=1)
+    assert check_dump(tmp_path, fx) == []  # one thread: bitwise the reference
+    eight = tu.run("cpu", fx["deck"], threads=8)
+    for k in ("u", "energy1"):
+        assert maxrel(eight["fields"][k], one["fields"][k]) < FIELD_BAR / 100, k
+    assert abs(eight["summary"][3] / one["summary"][3] - 1.0) < FIELD_BAR / 100
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["tea_cg_64", "tea_cg_200"])
+def test_gpu_fields_within_1e12_of_reference(golden, tmp_path, name):
+    # u, energy1 and the temperature of the GPU plain path against the reference's values:
+    # the oracle on one thread, whose every dat is checked here bitwise against the
+    # reference dev_seq fixture
+    fx = golden(f"{name}.json")
+    ref = tu.run("cpu", fx["deck"], dump_dir=str(tmp_path), threads=1)
+    assert check_dump(tmp_path, fx) == []
+    got = tu.run("gpu", fx["deck"])
+    assert [s[2] for s in got["per_step"]] == [s[2] for s in ref["per_step"]]
+    for k in ("u", "energy1"):
+        assert maxrel(got["fields"][k], ref["fields"][k]) < FIELD_BAR, k
+    assert abs(got["summary"][3] / ref["summary"][3] - 1.0) < FIELD_BAR
+    assert np.max(np.abs(got["summary"] - ref["summary"]) / np.abs(ref["summary"])) < FIELD_BAR
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("solver", [1, 2, 3])
 def test_gpu_native_solvers(solver):
