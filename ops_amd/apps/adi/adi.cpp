@@ -67,7 +67,11 @@ void preproc(Blk &b) {
 // blocks before the next phase, which lets the runtime solve the lines of all blocks of a
 // sweep together
 void step_all() {
+  // the blocks' preproc loops are independent (each writes only its own block's dats): they
+  // go to the runtime as one loop batch (a few launches instead of one per block)
+  ops_hip_batch_begin();
   for (Blk &b : A.blocks) preproc(b);
+  ops_hip_batch_end();
   for (Blk &b : A.blocks) {
     int range[] = {0, b.n[0], 0, b.n[1], 0, b.n[2]};
     ops_tridMultiDimBatch(3, 0, range, b.ax, b.bx, b.cx, b.du, A.ctx);

@@ -300,7 +300,7 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
                  f"(int)(3000 / sizeof(ops_kp_{k})) : 8;")
         L.append(f"struct ops_kb_{k} {{")
         L.append(f"  ops_kp_{k} p[ops_nb_{k} > 0 ? ops_nb_{k} : 1];")
-        L.append("  unsigned first[9], g0[8], g1[8], b0[8], b1[8], b2[8];")
+        L.append("  unsigned first[9], nb[8], g0[8], g1[8], b0[8], b1[8], b2[8];")
         L.append("  int n;")
         L.append("};")
         L.append("")
@@ -318,7 +318,7 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
         L.append(kernel_body(loop, params, ndim, k, False, v1))
     if batchable(loop):
         L.append("")
-        L.append(batch_kernel(loop, params, ndim, v1))
+        L.append(batch_kernel(loop, params, ndim, v1, pointwise or set(), written_once or set()))
     return "\n".join(L)
 
 
@@ -329,14 +329,19 @@ def batchable(loop: Loop) -> bool:
             not any(a.kind == "dat" and a.prolong for a in loop.args) and loop_vx(loop) == 1)
 
 
-def batch_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool) -> str:
+def batch_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool, pointwise: set = frozenset(),
+                 written_once: set = frozenset()) -> str:
     """The batch variant: the aliasing-safe body (pointers from the argument struct) behind
-    a prologue that finds the workgroup's loop in the batch (prefix sums of workgroups),
-    then its tile (row-major over that loop's grid, no XCD remap: batches are small) and the
-    lane's position in that loop's workgroup shape (threads beyond it return; the body has
-    no barriers)."""
+    a prologue that finds the workgroup's loop in the batch (prefix sums of workgroups, each
+    loop's share rounded up to a multiple of the 8 XCDs, so workgroup b of a loop runs on
+    XCD b % 8), then its tile (the loop's XCD remap over its own tiles; the padding
+    workgroups return) and the lane's position in that loop's workgroup shape (threads
+    beyond it return; the body has no barriers)."""
     k = loop.stub
-    body = kernel_body(loop, params, ndim, k + "_batch", False, v1)
+    # pointers from the argument struct; the streamed loads and non-temporal stores of the
+    # no-alias kernel still apply (a launch whose dats alias does not join a batch)
+    body = kernel_body(loop, params, ndim, k + "_batch", False, v1, pointwise, written_once,
+                       stream_from_struct=True)
     sig = f"void ops_kern_{k}_batch(const ops_kp_{k} p) {{"
     assert sig in body
     body = body.replace(sig, f"void ops_kern_{k}_batch(const ops_kb_{k} B) {{")
@@ -349,7 +354,9 @@ def batch_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool) -> str:
         f"  const ops_kp_{k} &p = B.p[j];",
         "  const unsigned bdx = B.b0[j], bdy = B.b1[j], bdz = B.b2[j];",
         "  const unsigned lb = blockIdx.x - B.first[j];",
-        "  const unsigned bx = lb % B.g0[j], by = (lb / B.g0[j]) % B.g1[j], bz = lb / (B.g0[j] * B.g1[j]);",
+        "  if (lb >= B.nb[j]) return;",
+        "  const unsigned lt = ops_xcd_remap(lb, B.nb[j]);",
+        "  const unsigned bx = lt % B.g0[j], by = (lt / B.g0[j]) % B.g1[j], bz = lt / (B.g0[j] * B.g1[j]);",
         "  const unsigned tx = threadIdx.x % bdx, ty = (threadIdx.x / bdx) % bdy, tz = threadIdx.x / (bdx * bdy);",
         "  if (tz >= bdz) return;"]))
     for d in "xyz":
@@ -381,8 +388,13 @@ def dat_offset(i: int, ndim: int) -> str:
 
 def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: bool,
                 v1: bool = False, pointwise: set = frozenset(),
-                written_once: set = frozenset()) -> str:
+                written_once: set = frozenset(), stream_from_struct: bool = False) -> str:
+    """One generated kernel. noalias: dats are __restrict__ pointer parameters q<i>, and the
+    point-wise arguments may be streamed (non-temporal loads / deferred non-temporal
+    stores); stream_from_struct: pointers from the argument struct with the same streaming
+    (the batch kernel: its launches never alias)."""
     k = loop.stub
+    nt_ok = noalias or stream_from_struct
     L = []
     red = [i for i, a in enumerate(loop.args) if a.kind == "reduce"]
     extra = "".join(f", {params[i].ctype} *__restrict__ q{i}" for i, a in enumerate(loop.args)
@@ -449,13 +461,13 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
             # (pointer + uniform part) + lane part: the first sum stays scalar
             ptr = f"{'q' if noalias else 'p.p'}{i} + {off}" if not a.prolong else \
                 f"{'q' if noalias else 'p.p'}{i} + ({off})"
-            if noalias and streamed(loop, i, pointwise):
+            if nt_ok and streamed(loop, i, pointwise):
                 L.append(f"    {t} *const e{i} = {ptr};")
                 L.append(f"    {t} v{i} = __builtin_nontemporal_load(e{i});")
                 if a.acc == "OPS_RW":
                     after.append(f"    *e{i} = v{i};")
                 ptr = f"&v{i}"
-            elif noalias and nt_stored(loop, i, written_once):
+            elif nt_ok and nt_stored(loop, i, written_once):
                 L.append(f"    {t} *const e{i} = {ptr};")
                 L.append(f"    {t} v{i};")
                 after.append(f"    __builtin_nontemporal_store(v{i}, e{i});")
@@ -561,7 +573,7 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     if bat:
         # join the pending batch (an untimed launch while batching is on), else launch alone
         # after whatever this kernel has pending
-        L.append(f"    if ({ns}::ops_nb_{k} >= 2 && !lp.timed && "
+        L.append(f"    if ({ns}::ops_nb_{k} >= 2 && !lp.timed && !lp.alias && "
                  f"ops_hip_batch_join(kid, ops_batch_flush_{k})) {{")
         L.append(f"      if (ops_batch_n_{k} == {ns}::ops_nb_{k}) ops_batch_flush_{k}();")
         L.append(f"      const int j = ops_batch_n_{k}++;")
@@ -570,7 +582,8 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
         L.append("      B.p[j] = p;")
         L.append("      B.g0[j] = lp.grid[0]; B.g1[j] = lp.grid[1];")
         L.append("      B.b0[j] = lp.block[0]; B.b1[j] = lp.block[1]; B.b2[j] = lp.block[2];")
-        L.append("      B.first[j + 1] = B.first[j] + (unsigned)lp.nblocks;")
+        L.append("      B.nb[j] = lp.nblocks;")
+        L.append("      B.first[j + 1] = B.first[j] + ((lp.nblocks + 7u) & ~7u);")
         L.append(f"      ops_batch_stream_{k} = lp.stream;")
         L.append("      ops_hip_loop_end(&lp, args, " + str(n) + ");")
         L.append("      return;")

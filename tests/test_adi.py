@@ -1,7 +1,10 @@
 """Batched 3-D ADI (preproc 7-point loop + x, y, z tridiagonal solves, the z solve adding
 into u). The oracle is checked against per-line dense solves of the same scheme written
 out in numpy, a constant fixed point, linearity and axis-permutation symmetry; the GPU
-runtime against the oracle bitwise, including the 48-block configuration."""
+runtime against the oracle: bitwise with the sequential sweeps (OPS_TRID_ALGO=thomas), and
+within 1e-12 with the default partitioned solve, including the 48-block configuration."""
+import os
+
 import numpy as np
 import pytest
 
@@ -113,26 +116,39 @@ def test_adi_sizes_generator():
     assert len({v for t in s for v in t}) > 40
 
 
+FIELD_BAR = 1e-12  # FP64 fields, relative to the field's largest value
+
+
+def close(a, b):
+    return np.max(np.abs(a - b)) <= FIELD_BAR * np.max(np.abs(b))
+
+
 @pytest.mark.gpu
-def test_gpu_uneven_blocks_equal_oracle():
+@pytest.mark.parametrize("algo", ["thomas", "partitioned"])
+def test_gpu_uneven_blocks_equal_oracle(monkeypatch, algo):
+    if algo == "thomas":
+        monkeypatch.setenv("OPS_TRID_ALGO", "thomas")
     sizes = [(33, 17, 9), (16, 40, 7), (70, 3, 20)]
     cpu = run("cpu", sizes, 4)
     gpu = run("gpu", sizes, 4)
     for a, b in zip(cpu, gpu):
-        assert np.array_equal(a, b)
+        assert np.array_equal(a, b) if algo == "thomas" else close(a, b)
 
 
 @pytest.mark.gpu
-def test_gpu_autotuned_uneven_blocks_equal_oracle(monkeypatch):
+@pytest.mark.parametrize("algo", ["thomas", "partitioned"])
+def test_gpu_autotuned_uneven_blocks_equal_oracle(monkeypatch, algo):
     # the autotuner tries its workgroup shapes on the first launches (blocks of different
-    # sizes share one kernel); the fields stay bitwise the oracle's
+    # sizes share one kernel); the fields stay the oracle's
+    if algo == "thomas":
+        monkeypatch.setenv("OPS_TRID_ALGO", "thomas")
     monkeypatch.setenv("OPS_AUTOTUNE", "1")
     sizes = [(33, 17, 9), (16, 40, 7), (70, 3, 20), (64, 24, 40)]
     gpu = run("gpu", sizes, 12)
     monkeypatch.delenv("OPS_AUTOTUNE")
     cpu = run("cpu", sizes, 12)
     for a, b in zip(cpu, gpu):
-        assert np.array_equal(a, b)
+        assert np.array_equal(a, b) if algo == "thomas" else close(a, b)
 
 
 GAMMA = 0x9E3779B97F4A7C15  # splitmix64 increment: the state after k draws is seed + k * GAMMA
@@ -162,18 +178,32 @@ ADI_BLOCKS = (0, 15, 16, 31, 32, 47)
 
 @pytest.fixture(scope="module")
 def adi48_gpu():
+    # the GPU run with the default partitioned solve and with the sequential sweeps
     sizes = adi_sizes()
-    return sizes, dict(zip(ADI_BLOCKS, run("gpu", sizes, 100, fetch=list(ADI_BLOCKS))))
+    out = {}
+    for algo in ("partitioned", "thomas"):
+        old = os.environ.pop("OPS_TRID_ALGO", None)
+        try:
+            if algo == "thomas":
+                os.environ["OPS_TRID_ALGO"] = "thomas"
+            out[algo] = dict(zip(ADI_BLOCKS, run("gpu", sizes, 100, fetch=list(ADI_BLOCKS))))
+        finally:
+            os.environ.pop("OPS_TRID_ALGO", None)
+            if old is not None:
+                os.environ["OPS_TRID_ALGO"] = old
+    return sizes, out
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("k", ADI_BLOCKS)
 def test_gpu_48_block_configuration_equals_oracle(adi48_gpu, k):
     # the benchmark configuration, 100 steps: block k of the GPU run against the oracle
-    # running block k alone from its skip-ahead seed
+    # running block k alone from its skip-ahead seed: bitwise with the sequential sweeps,
+    # within 1e-12 with the partitioned solve
     sizes, gpu = adi48_gpu
     cpu = run("cpu", [sizes[k]], 100, seed=skip_seed(5, sizes, k), threads=16)[0]
-    assert np.array_equal(gpu[k], cpu)
+    assert np.array_equal(gpu["thomas"][k], cpu)
+    assert close(gpu["partitioned"][k], cpu)
 
 
 @pytest.mark.gpu

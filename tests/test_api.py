@@ -282,13 +282,65 @@ def test_oracle_tridiag_matches_dense_solve(case):
     assert np.max(np.abs(out - ref)) < 1e-13
 
 
+# line lengths around the partition boundaries of the partitioned solve (16 partitions of
+# ceil(n/16) rows: empty, one-row and two-row partitions) and its 256-row limit
+TRI += [(2, 0, (n, 3), (0, n, 0, 3), n % 2) for n in (1, 2, 3, 15, 16, 17, 31, 33, 47, 65, 250, 256)]
+TRI += [(3, 2, (20, 3, 257), (0, 20, 0, 3, 0, 257), 1), (3, 1, (33, 100, 2), (0, 33, 0, 100, 0, 2), 0)]
+
+
+def test_partitioned_restatement_matches_dense_solve():
+    # the numpy restatement of the partitioned algorithm against dense solves, line lengths
+    # 1..256 (every partition shape), diagonally dominant and ADI-like coefficients
+    from trid_ref import part_solve
+    rng = np.random.default_rng(8)
+    for n in list(range(1, 40)) + [63, 64, 65, 127, 128, 129, 200, 255, 256]:
+        a = rng.uniform(-1, 0, (5, n))
+        c = rng.uniform(-1, 0, (5, n))
+        b = 2.5 + rng.uniform(0, 1, (5, n))
+        d = rng.uniform(-1, 1, (5, n))
+        for red in ("pcr", "thomas"):
+            x = part_solve(a, b, c, d, reduced=red)
+            for i in range(5):
+                M = np.diag(b[i]) + np.diag(a[i][1:], -1) + np.diag(c[i][:-1], 1)
+                ref = np.linalg.solve(M, d[i])
+                assert np.max(np.abs(x[i] - ref)) <= 1e-14 * max(1.0, np.max(np.abs(ref))), (n, red)
+    # a dropped coupling term (the partition's last row ignoring the next partition) fails
+    n = 64
+    a, c = -0.5 * np.ones((1, n)), -0.5 * np.ones((1, n))
+    b, d = 2.0 * np.ones((1, n)), rng.uniform(-1, 1, (1, n))
+    x = part_solve(a, b, c, d)
+    M = np.diag(b[0]) + np.diag(a[0][1:], -1) + np.diag(c[0][:-1], 1)
+    cut = M.copy()
+    cut[3, 4] = 0.0  # rows 0-3 and 4-7 form the first two partitions
+    assert np.max(np.abs(x[0] - np.linalg.solve(M, d[0]))) < 1e-14
+    assert np.max(np.abs(x[0] - np.linalg.solve(cut, d[0]))) > 1e-3
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", TRI)
-def test_gpu_tridiag_equals_oracle(case):
+def test_gpu_tridiag_sequential_sweeps_equal_oracle(case, monkeypatch):
+    # OPS_TRID_ALGO=thomas: the sequential sweeps, bitwise the oracle's sequential solve
+    monkeypatch.setenv("OPS_TRID_ALGO", "thomas")
     ndim, sd, size, box, inc = case
     _, oc = tri_run("cpu", ndim, sd, size, box, inc)
     _, og = tri_run("gpu", ndim, sd, size, box, inc)
     assert np.array_equal(oc, og)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", TRI)
+def test_gpu_tridiag_partitioned(case):
+    # the default solve: bitwise the restatement of its algorithm where it applies (lines of
+    # at most 256 rows; longer lines take the sequential sweeps, bitwise the oracle), and
+    # within 1e-12 of the oracle's sequential solve
+    from trid_ref import part_dims
+    ndim, sd, size, box, inc = case
+    inputs, oc = tri_run("cpu", ndim, sd, size, box, inc)
+    _, og = tri_run("gpu", ndim, sd, size, box, inc)
+    n = box[2 * sd + 1] - box[2 * sd]
+    exp = part_dims(inputs, ndim, sd, size, box, inc) if n <= 256 else oc
+    assert np.array_equal(og, exp)
+    assert np.max(np.abs(og - oc)) <= 1e-12 * np.max(np.abs(oc))
 
 
 # ---------------------------------------------------------------- reductions
@@ -443,10 +495,12 @@ def test_oracle_batch_sequence_closed_form(mode):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", range(8))
-def test_gpu_batch_sequence_equals_oracle(mode):
+def test_gpu_batch_sequence_equals_oracle(mode, monkeypatch):
     # 11 (more than one batch of 8) and 17 launches of one kernel of mixed extents, its
     # WRITE and RW stubs interleaved, batched or not, with a tridiagonal solve of a batched
-    # launch's output in between, eagerly or replayed from a graph: bitwise the oracle's
+    # launch's output in between (sequential sweeps, as the oracle solves), eagerly or
+    # replayed from a graph: bitwise the oracle's
+    monkeypatch.setenv("OPS_TRID_ALGO", "thomas")
     for n in (11, 17):
         cpu = batch_run("cpu", n, mode & 2)
         gpu = batch_run("gpu", n, mode)

@@ -151,7 +151,10 @@ void k_pw(const ACC<double> &a, ACC<double> &b, const ACC<double> &c, ACC<double
     assert "double v3 = __builtin_nontemporal_load(e3);" in body and "*e3 = v3;" in body
     assert "v1 = " not in body and "v2 = " not in body
     alias = hip[hip.index("void ops_kern_k_pw_alias("):]
-    assert "nontemporal" not in alias.split("void ops_par_loop_")[0]
+    assert "nontemporal" not in alias.split("__global__")[0]
+    # the batch kernel (its launches never alias) streams like the no-alias one
+    batch = hip[hip.index("void ops_kern_k_pw_batch("):]
+    assert "double v0 = __builtin_nontemporal_load(e0);" in batch.split("__global__")[0]
 
 
 def test_write_once_arguments_get_non_temporal_stores(tmp_path):
@@ -191,7 +194,9 @@ void k_ret(const ACC<double> &a, ACC<double> &b) {
     # written once but its access is RW, so it stays on the dat's memory
     assert "double v1;" in body and "__builtin_nontemporal_store(v1, e1);" in body
     assert body.count("__builtin_nontemporal_store") == 1
-    alias = hip[hip.index("void ops_kern_k_w_alias("):].split("void ops_par_loop_")[0]
+    alias = hip[hip.index("void ops_kern_k_w_alias("):].split("__global__")[0]
     assert "nontemporal" not in alias
+    batch = hip[hip.index("void ops_kern_k_w_batch("):].split("__global__")[0]
+    assert batch.count("__builtin_nontemporal_store(v1, e1);") == 1
     off = Path(translate_app("tapp", [src], str(tmp_path / "o2"), ntstore=False)["kernels"]).read_text()
     assert "nontemporal_store" not in off
