@@ -4,17 +4,21 @@ Default workload: CloverLeaf 2D, deck clover_bm16_short.in (3840 x 3840 cells, 8
 steps, FP64), through libops_hip.so and the translated application. A step is one hydro
 time step of the whole mesh; a cell-update is one cell advanced by one step.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu] [--cpu-threads T]
-                    [--no-tealeaf] [--tea-cells N] [--tea-steps S] [--no-learned]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-laplace] [--no-tealeaf]
+                    [--tea-cells N] [--tea-steps S] [--tea-solvers 0,1,3,4] [--no-learned]
                     [--unet-pairs P] [--unet-epochs E] [--no-adi] [--adi-steps S]
+                    [--no-clover3d]
 
 Prints one JSON line. `value` is measured with the inputs resident in HBM: the deck is
 generated on the device before the timed region; the K timed steps are bracketed by
 device synchronisation. Per-kernel durations come from HIP events recorded on the
 runtime's stream around every launch inside the timed region; `roofline` is computed for
 the kernel with the largest total time from its ops_compute_transfer bytes per launch.
-`cpu_baseline` runs the same application on the oracle's host backend (OpenMP) for a
-bounded sample on this machine's cores. `tealeaf` reports the second half of the metric:
+`cpu_baseline` runs the same application on the oracle's host backend (OpenMP, one thread
+per physical core of the process's CPU share, bound close, lscpu model recorded) for a
+bounded sample: one warm-up, then the median of three runs; each other workload carries its
+own `cpu_baseline` the same way. `laplace` is the 2D Laplace Jacobi workload (1024^2, 1000
+iterations). `tealeaf` reports the second half of the metric:
 TeaLeaf 4096^2 time-to-solution of plain FP64 CG against the multigrid-preconditioned
 solver (FP64 outer CG, FP32 V-cycle), with residuals and field differences;
 `tealeaf_learned` the learned warm start (data generation, U-Net training throughput
@@ -155,6 +159,126 @@ def whole_step_bw(stats, seconds):
             "GBps_over_wall": round(tot_bytes / seconds / 1e9, 1)}
 
 
+_HOST_CPU = None
+
+
+def host_cpu():
+    """The host CPU the baselines run on: model, sockets and physical cores from lscpu; the
+    OpenMP threads used are one per physical core of this process's CPU share (at most the
+    OMP_NUM_THREADS the environment sets), bound close to one core each (OMP_PROC_BIND=close,
+    OMP_PLACES=cores, set before the OpenMP runtime of the oracle libraries loads). Computed
+    once: after the first OpenMP region the binding pins the main thread to one core, so the
+    process's affinity no longer shows its CPU share."""
+    global _HOST_CPU
+    if _HOST_CPU is not None:
+        return _HOST_CPU
+    import subprocess
+    info = {"model": None, "sockets": None, "physical_cores": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                info["model"] = v.strip()
+            elif k.strip() == "Socket(s)":
+                info["sockets"] = int(v.strip())
+        cores = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                               timeout=10).stdout
+        info["physical_cores"] = len({l for l in cores.splitlines() if l and not l.startswith("#")})
+    except Exception:
+        pass
+    share = len(os.sched_getaffinity(0))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    phys = info["physical_cores"] or share
+    threads = max(1, min(phys, share, cap))
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    info.update({"threads": threads, "cpu_share": share, "OMP_PROC_BIND": "close",
+                 "OMP_PLACES": "cores"})
+    _HOST_CPU = info
+    return info
+
+
+def median_time(fn, runs=3):
+    """One warm-up call, then the median wall time of `runs` calls."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_lib(app, threads):
+    """The oracle's host build of an application (the CPU baseline: a port, OpenMP)."""
+    from ops_amd.apps import SIGNATURES
+    lib = ctypes.CDLL(str(ROOT / "oracle" / "_build" / f"lib{app}_cpu.so"), mode=ctypes.RTLD_LOCAL)
+    for fn, (res, argt) in SIGNATURES[app].items():
+        if hasattr(lib, fn):
+            getattr(lib, fn).restype = res
+            getattr(lib, fn).argtypes = argt
+    lib.ops_cpu_set_threads.argtypes = [ctypes.c_int]
+    lib.ops_cpu_set_threads(threads)
+    return lib
+
+
+def laplace_gpu(args):
+    """Laplace 2D Jacobi, 1024^2 interior, 1000 iterations of the reference's loop: the
+    5-point stencil with its MAX reduction, ops_reduction_result (read on the host every
+    iteration, as the reference's stop test does), the copy back. 32 algorithmic bytes per
+    cell-update (16 stencil + 16 copy)."""
+    import ops_amd
+    L = ops_amd.app("laplace2d")
+    R = ops_amd.runtime()
+    n, iters = args.lap_cells, args.lap_iters
+    argc, argv = ops_amd.argv()
+    err = (ctypes.c_double * 1)()
+    assert L.laplace2d_init(argc, argv, n, n, 0) == 0
+    assert L.laplace2d_iterate(20, err) == 0  # warm-up (autotuning candidates)
+    assert L.laplace2d_finalize() == 0
+    assert L.laplace2d_init(argc, argv, n, n, 0) == 0
+    R.ops_hip_device_sync()
+    t0 = time.perf_counter()
+    assert L.laplace2d_iterate(iters, err) == 0
+    R.ops_hip_device_sync()
+    t1 = time.perf_counter()
+    final_err = err[0]
+    assert L.laplace2d_finalize() == 0
+    # per-kernel durations from an eager pass of the same iterations with HIP events
+    assert L.laplace2d_init(argc, argv, n, n, 0) == 0
+    R.ops_hip_device_sync()
+    R.ops_hip_reset_stats()
+    R.ops_hip_set_timing(1)
+    assert L.laplace2d_iterate(iters, err) == 0
+    R.ops_hip_device_sync()
+    R.ops_hip_set_timing(0)
+    assert L.laplace2d_finalize() == 0
+    stats = {k: v for k, v in ops_amd.kernel_stats().items() if k in ("apply_stencil", "copy")}
+    secs = t1 - t0
+    _, roof = roofline({"apply_stencil": stats["apply_stencil"]})
+    return {"workload": f"laplace2d {n}x{n}, {iters} Jacobi iterations (stencil + MAX reduction + copy)",
+            "value": n * n * iters / secs, "unit": "cell-updates/s", "s": round(secs, 4),
+            "us_per_iteration": round(secs * 1e6 / iters, 2),
+            "algorithmic_GBps_over_wall": round(32.0 * n * n * iters / secs / 1e9, 1),
+            "roofline": roof, "top_kernels": top_kernels(stats, 2), "final_error": final_err}
+
+
+def laplace_cpu(args, cpu):
+    import numpy as np
+    import oracle
+    n, iters = args.lap_cells, args.lap_cpu_iters
+    A = np.zeros((n + 2, n + 2))
+    An = np.zeros_like(A)
+    pa, pn = A.ctypes.data_as(ctypes.c_void_p), An.ctypes.data_as(ctypes.c_void_p)
+    t = median_time(lambda: oracle.lib().oracle_laplace(n, n, iters, 0, pa, pn, cpu["threads"]))
+    return {"value": n * n * iters / t, "unit": "cell-updates/s", "cores": cpu["threads"],
+            "kind": "port", "cpu": cpu,
+            "sample": f"Laplace {n}x{n}, {iters} iterations of the oracle's C restatement "
+                      f"(OpenMP), median of 3 runs after a warm-up: {t:.3f} s"}
+
+
 def clover_gpu(args, R):
     import ops_amd
     L = ops_amd.app("cloverleaf")
@@ -259,25 +383,80 @@ def tealeaf_gpu(args):
         log(f"[bench] tealeaf solver {solver} eps {eps}: {t1 - t0:.3f} s, {sum(iters)} iterations")
         return {"s": t1 - t0, "iters": iters, "exact": exact, "rel": rel, "temp": s4[3], "u": u}
 
-    base = run(0, 1e-10)
-    mg = run(3, 1e-10)
+    names = {0: "fp64_cg_plain", 1: "fp64_cg_device", 2: "fp32_cg_fp64_refinement",
+             3: "fp64_pcg_fp32_multigrid", 4: "fp64_pcg_fp64_multigrid"}
+    runs = {}
+    for solver in args.tea_solvers:
+        runs[solver] = run(solver, 1e-10)
     tight = run(3, 1e-12)
 
     def maxrel(a, b):
         return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
-    return {
+    out = {
         "workload": f"tealeaf_2d {n}x{n}, {steps} steps, 8 seeded rectangles, tl_eps 1e-10",
-        "time_to_solution_s": {"fp64_cg_plain": round(base["s"], 4), "fp64_pcg_fp32_multigrid": round(mg["s"], 4)},
-        "speedup_vs_fp64_cg": round(base["s"] / mg["s"], 2),
-        "iterations": {"fp64_cg_plain": sum(base["iters"]), "fp64_pcg_fp32_multigrid": sum(mg["iters"])},
-        "max_relative_cg_residual": {"fp64_cg_plain": max(base["rel"]), "fp64_pcg_fp32_multigrid": max(mg["rel"])},
-        "final_exact_residual": {"fp64_cg_plain": base["exact"][-1], "fp64_pcg_fp32_multigrid": mg["exact"][-1]},
-        "field_max_rel_diff_vs_fp64_cg": maxrel(mg["u"], base["u"]),
-        "field_max_rel_err_vs_eps1e-12_solution": {"fp64_cg_plain": maxrel(base["u"], tight["u"]),
-                                                   "fp64_pcg_fp32_multigrid": maxrel(mg["u"], tight["u"])},
-        "temperature": {"fp64_cg_plain": base["temp"], "fp64_pcg_fp32_multigrid": mg["temp"]},
+        "time_to_solution_s": {names[k]: round(v["s"], 4) for k, v in runs.items()},
+        "iterations": {names[k]: sum(v["iters"]) for k, v in runs.items()},
+        "max_relative_cg_residual": {names[k]: max(v["rel"]) for k, v in runs.items()},
+        "final_exact_residual": {names[k]: v["exact"][-1] for k, v in runs.items()},
+        "field_max_rel_err_vs_eps1e-12_solution": {names[k]: maxrel(v["u"], tight["u"]) for k, v in runs.items()},
+        "field_target": "1e-9 of plain FP64 CG; plain FP64 CG itself stops at the error above from "
+                        "the converged (eps 1e-12) solution",
+        "temperature": {names[k]: v["temp"] for k, v in runs.items()},
     }
+    if 0 in runs:
+        out["field_max_rel_diff_vs_fp64_cg"] = {names[k]: maxrel(v["u"], runs[0]["u"])
+                                               for k, v in runs.items() if k != 0}
+        out["speedup_vs_fp64_cg_plain"] = {names[k]: round(runs[0]["s"] / v["s"], 2)
+                                           for k, v in runs.items() if k != 0}
+    if 1 in runs:
+        out["speedup_vs_fp64_cg_device"] = {names[k]: round(runs[1]["s"] / v["s"], 2)
+                                            for k, v in runs.items() if k not in (0, 1)}
+    if 3 in runs and 4 in runs:
+        # what FP32 storage and arithmetic in the V-cycle buy: the same preconditioned solve
+        # with the V-cycle in FP64
+        out["fp32_vcycle_gain"] = {"time_ratio_fp64_over_fp32": round(runs[4]["s"] / runs[3]["s"], 3),
+                                   "iterations_fp32": sum(runs[3]["iters"]),
+                                   "iterations_fp64": sum(runs[4]["iters"])}
+    return out
+
+
+def tealeaf_cpu(args, cpu):
+    """Plain FP64 CG of the oracle's host build on the 4096^2 deck, time per CG iteration:
+    the first implicit step solved with the iteration cap at 1 and at 1 + --tea-cpu-iters,
+    the difference of the two step times over the extra iterations (the step's set-up
+    cancels); one warm-up pair, then the median of three. The time-to-solution beside it is
+    that time the GPU run's plain-CG iteration count (an extrapolation)."""
+    from ops_amd.apps.tealeaf.decks import tea_deck
+    import ops_amd
+    L = cpu_lib("tealeaf", cpu["threads"])
+    n, its = args.tea_cells, args.tea_cpu_iters
+    argc, argv = ops_amd.argv()
+
+    def step_time(cap):
+        with tempfile.TemporaryDirectory() as t:
+            deck = Path(t) / "tea.in"
+            deck.write_text(tea_deck(n, 1, eps=1e-10, max_iters=cap))
+            assert L.tealeaf_init(argc, argv, str(deck).encode()) == 0
+            assert L.tealeaf_set_solver(0) == 0
+            t0 = time.perf_counter()
+            assert L.tealeaf_step(1) == 0
+            dt = time.perf_counter() - t0
+            assert L.tealeaf_finalize() == 0
+        return dt
+
+    diffs = []
+    for k in range(4):
+        d = (step_time(1 + its) - step_time(1)) / its
+        if k:
+            diffs.append(d)
+    per = sorted(diffs)[1]
+    return {"value": n * n / per, "unit": "cell-iterations/s", "cores": cpu["threads"],
+            "kind": "port", "cpu": cpu, "s_per_cg_iteration": per,
+            "sample": f"TeaLeaf {n}x{n}, plain FP64 CG on the oracle host build (OpenMP): the "
+                      f"first implicit step capped at 1 and at {1 + its} iterations, difference "
+                      f"per iteration, median of 3 after a warm-up: {per * 1e3:.2f} ms per "
+                      f"iteration"}
 
 
 def tealeaf_learned_gpu(args):
@@ -360,6 +539,26 @@ def adi_gpu(args):
             "algorithmic_GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def adi_cpu(args, cpu):
+    """The oracle host build of the ADI application on the first --adi-cpu-blocks blocks of
+    the 48-block configuration (the same generator and extents), one step timed."""
+    import numpy as np
+    from ops_amd.apps.adi.config import adi_sizes
+    import ops_amd
+    L = cpu_lib("adi", cpu["threads"])
+    sizes = adi_sizes()[:args.adi_cpu_blocks]
+    cells = sum(a * b * c for a, b, c in sizes)
+    sz = np.array([v for s_ in sizes for v in s_], dtype=np.int32)
+    argc, argv = ops_amd.argv()
+    assert L.adi_init(argc, argv, len(sizes), sz.ctypes.data_as(ctypes.c_void_p), 1, 5, 1.0) == 0
+    t = median_time(lambda: L.adi_step(1))
+    assert L.adi_finalize() == 0
+    return {"value": cells / t, "unit": "cell-steps/s", "cores": cpu["threads"], "kind": "port",
+            "cpu": cpu, "sample": f"ADI, blocks 0..{len(sizes) - 1} of the 48 ({cells} cells), one "
+                                  f"step on the oracle host build (OpenMP), median of 3 after a "
+                                  f"warm-up: {t:.3f} s"}
+
+
 def clover3d_gpu(args, R):
     """CloverLeaf 3D (apps/c/CloverLeaf_3D, an API v1 application) on the reference's deck
     clover.in (96^3, 87 steps, known answer ke 20.0546870878964) and on a larger mesh of
@@ -429,42 +628,29 @@ def clover3d_gpu(args, R):
             e["ke_pass_0.001pct"] = e["ke_pct_diff"] < 0.001
         out[tag] = e
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        lib = ctypes.CDLL(str(ROOT / "oracle" / "_build" / "libcloverleaf3d_cpu.so"),
-                          mode=ctypes.RTLD_LOCAL)
-        lib.ops_cpu_set_threads.argtypes = [ctypes.c_int]
-        lib.ops_cpu_set_threads(threads)
-        lib.cloverleaf3d_init.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
-                                          ctypes.c_char_p]
-        lib.cloverleaf3d_step.argtypes = [ctypes.c_int]
-        steps = 20
+        cpu = host_cpu()
+        lib = cpu_lib("cloverleaf3d", cpu["threads"])
+        steps = args.c3_cpu_steps
         with tempfile.TemporaryDirectory() as t:
             deck = Path(t) / "clover.in"
             deck.write_text(bm_deck(96, 87))
             argc, argv = ops_amd.argv()
             assert lib.cloverleaf3d_init(argc, argv, str(deck).encode()) == 0
-            assert lib.cloverleaf3d_step(1) == 0
-            t0 = time.perf_counter()
-            assert lib.cloverleaf3d_step(steps) == 0
-            t1 = time.perf_counter()
+            dt = median_time(lambda: lib.cloverleaf3d_step(steps))
             assert lib.cloverleaf3d_finalize() == 0
-        out["cpu_port"] = {"value": 96 ** 3 * steps / (t1 - t0), "unit": "cell-updates/s",
-                           "cores": threads, "kind": "port",
-                           "sample": f"CloverLeaf 3D 96^3, steps 2..{steps + 1} of clover.in on "
-                                     f"the oracle host backend ({threads} threads), "
-                                     f"{t1 - t0:.2f} s"}
+        out["cpu_port"] = {"value": 96 ** 3 * steps / dt, "unit": "cell-updates/s",
+                           "cores": cpu["threads"], "kind": "port", "cpu": cpu,
+                           "sample": f"CloverLeaf 3D 96^3, clover.in, median of 3 runs of {steps} "
+                                     f"steps after a warm-up run on the oracle host backend "
+                                     f"(OpenMP, {cpu['threads']} threads): {dt:.2f} s"}
     return out
 
 
-def clover_cpu(args, threads):
-    """Oracle host backend, same application and deck, bounded sample of steps."""
-    lib = ctypes.CDLL(str(ROOT / "oracle" / "_build" / "libcloverleaf_cpu.so"),
-                      mode=ctypes.RTLD_LOCAL)
-    lib.ops_cpu_set_threads.argtypes = [ctypes.c_int]
-    lib.ops_cpu_set_threads(threads)
+def clover_cpu(args, cpu):
+    """Oracle host backend, same application and deck, bounded sample of steps: one warm-up
+    step, then the median of three runs of --cpu-steps steps."""
     import ops_amd
-    lib.cloverleaf_init.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p]
-    lib.cloverleaf_step.argtypes = [ctypes.c_int]
+    lib = cpu_lib("cloverleaf", cpu["threads"])
     n = args.cells
     steps = args.cpu_steps
     with tempfile.TemporaryDirectory() as t:
@@ -473,14 +659,18 @@ def clover_cpu(args, threads):
         argc, argv = ops_amd.argv()
         assert lib.cloverleaf_init(argc, argv, str(deck).encode()) == 0
         assert lib.cloverleaf_step(1) == 0  # warm-up step (page faults, first touch)
-        t0 = time.perf_counter()
-        assert lib.cloverleaf_step(steps) == 0
-        t1 = time.perf_counter()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            assert lib.cloverleaf_step(steps) == 0
+            ts.append(time.perf_counter() - t0)
         assert lib.cloverleaf_finalize() == 0
-    return {"value": n * n * steps / (t1 - t0), "unit": "cell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"CloverLeaf {n}x{n}, steps 2..{steps + 1} of clover_bm16_short.in on the "
-                      f"oracle host backend (OpenMP, {threads} threads), {t1 - t0:.2f} s"}
+    dt = sorted(ts)[1]
+    return {"value": n * n * steps / dt, "unit": "cell-updates/s", "cores": cpu["threads"],
+            "kind": "port", "cpu": cpu,
+            "sample": f"CloverLeaf {n}x{n}, clover_bm16_short.in, after one warm-up step the median "
+                      f"of 3 runs of {steps} steps on the oracle host backend (OpenMP, "
+                      f"{cpu['threads']} threads): {dt:.2f} s"}
 
 
 def main():
@@ -494,8 +684,19 @@ def main():
     ap.add_argument("--workload", default="cloverleaf", choices=["cloverleaf"])
     ap.add_argument("--cells", type=int, default=3840)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="CloverLeaf 2D steps per timed CPU-baseline run (median of 3)")
+    ap.add_argument("--c3-cpu-steps", type=int, default=5)
+    ap.add_argument("--no-laplace", action="store_true")
+    ap.add_argument("--lap-cells", type=int, default=1024)
+    ap.add_argument("--lap-iters", type=int, default=1000)
+    ap.add_argument("--lap-cpu-iters", type=int, default=200)
+    ap.add_argument("--tea-solvers", type=lambda v: [int(x) for x in v.split(",")],
+                    default=[0, 1, 3, 4],
+                    help="TeaLeaf solvers to time (0 plain CG, 1 device CG, 2 FP32 CG + FP64 "
+                         "refinement, 3 FP64 PCG + FP32 multigrid, 4 FP64 PCG + FP64 multigrid)")
+    ap.add_argument("--tea-cpu-iters", type=int, default=30)
+    ap.add_argument("--adi-cpu-blocks", type=int, default=6)
     ap.add_argument("--no-tealeaf", action="store_true")
     ap.add_argument("--no-adi", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
@@ -558,18 +759,32 @@ def main():
         "top_kernels": top_kernels(stats, 10),
         "checks": checks,
     }
+    cpu = None if args.no_cpu else host_cpu()
+    if not args.no_laplace:
+        log("[bench] laplace2d")
+        out["laplace"] = laplace_gpu(args)
+        if cpu:
+            out["laplace"]["cpu_baseline"] = laplace_cpu(args, cpu)
     if not args.no_tealeaf:
         out["tealeaf"] = tealeaf_gpu(args)
+        if cpu:
+            log("[bench] tealeaf cpu sample")
+            out["tealeaf"]["cpu_baseline"] = tc = tealeaf_cpu(args, cpu)
+            plain = out["tealeaf"]["iterations"].get("fp64_cg_plain")
+            if plain:
+                tc["time_to_solution_s_extrapolated"] = round(tc["s_per_cg_iteration"] * plain, 2)
     if not args.no_learned:
         out["tealeaf_learned"] = tealeaf_learned_gpu(args)
     if not args.no_adi:
         out["adi"] = adi_gpu(args)
+        if cpu:
+            log("[bench] adi cpu sample")
+            out["adi"]["cpu_baseline"] = adi_cpu(args, cpu)
     if not args.no_clover3d:
         out["cloverleaf_3d"] = clover3d_gpu(args, R)
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        log(f"[bench] cpu baseline: {threads} threads, {args.cpu_steps} steps")
-        out["cpu_baseline"] = clover_cpu(args, threads)
+    if cpu:
+        log(f"[bench] cpu baseline: {cpu['threads']} threads, 3 x {args.cpu_steps} steps")
+        out["cpu_baseline"] = clover_cpu(args, cpu)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -88,7 +88,8 @@ struct Sim {
   // reflective halo sources, one stencil per side: bottom, top, left, right
   ops_stencil H[4];
   ops_stencil SX, SY, SX_P10, SY_P01, SX_P10_M10, SY_P01_M01;
-  ops_reduction r_dt, r_vol, r_mass, r_ie, r_ke, r_press;
+  ops_reduction r_dt, r_vol, r_mass, r_ie, r_ke, r_press, r_xl, r_yl;
+  double summary[5] = {0, 0, 0, 0, 0};  // the last field summary of the step loop
 } S;
 
 // ---------------------------------------------------------------- input deck
@@ -284,6 +285,8 @@ void build_field() {
   S.r_ie = ops_decl_reduction_handle(sizeof(double), "double", "ie");
   S.r_ke = ops_decl_reduction_handle(sizeof(double), "double", "ke");
   S.r_press = ops_decl_reduction_handle(sizeof(double), "double", "press");
+  S.r_xl = ops_decl_reduction_handle(sizeof(double), "double", "xl_pos");
+  S.r_yl = ops_decl_reduction_handle(sizeof(double), "double", "yl_pos");
 }
 
 // ---------------------------------------------------------------- set-up loops
@@ -496,12 +499,22 @@ void calc_dt_loops() {
   ops_par_loop(cl_dt_min, "calc_dt_kernel_min", S.blk, 2, in,
                ops_arg_dat(S.work1, 1, S.S00, "double", OPS_READ),
                ops_arg_reduce(S.r_dt, 1, "double", OPS_MIN));
+  // the controlling cell's position (calc_dt.cpp:72-87): jk_control is the constant 1.1,
+  // so it is always the first interior cell
+  int pt[] = {field.x_min, field.x_min + 1, field.y_min, field.y_min + 1};
+  ops_par_loop(cl_dt_get, "calc_dt_kernel_getx", S.blk, 2, pt,
+               ops_arg_dat(S.cellx, 1, S.SX, "double", OPS_READ),
+               ops_arg_dat(S.celly, 1, S.SY, "double", OPS_READ),
+               ops_arg_reduce(S.r_xl, 1, "double", OPS_INC),
+               ops_arg_reduce(S.r_yl, 1, "double", OPS_INC));
 }
 
 double calc_dt() {
   calc_dt_loops();
-  double local_dt;
+  double local_dt, xl, yl;
   ops_reduction_result(S.r_dt, &local_dt);
+  ops_reduction_result(S.r_xl, &xl);
+  ops_reduction_result(S.r_yl, &yl);
   local_dt = MIN(local_dt, g_big);
   return local_dt;
 }
@@ -881,6 +894,8 @@ void hydro_step() {
   reset_field();
   S.advect_x = !S.advect_x;
   S.time = S.time + dt;
+  // clover_leaf.cpp:206-208: a field summary every summary_frequency steps
+  if (S.summary_frequency != 0 && S.step % S.summary_frequency == 0) field_summary(S.summary);
 }
 
 // the launches of one step with dt, dtold and time kept on the device
@@ -903,7 +918,11 @@ void capture_steps() {
     if (S.step_graph[ax]) continue;
     const int keep = S.advect_x;
     S.advect_x = ax;
-    ops_hip_reduction_discard(S.r_dt);  // the captured MIN starts from its own result
+    // the captured reductions start from their own results (nothing reads the cell
+    // position of a captured step)
+    ops_hip_reduction_discard(S.r_dt);
+    ops_hip_reduction_discard(S.r_xl);
+    ops_hip_reduction_discard(S.r_yl);
     if (ops_hip_graph_begin())
       throw OPSException(OPS_RUNTIME_ERROR, "cloverleaf: graph capture unavailable");
     try {
@@ -915,6 +934,8 @@ void capture_steps() {
     }
     S.step_graph[ax] = ops_hip_graph_end();
     ops_hip_reduction_discard(S.r_dt);
+    ops_hip_reduction_discard(S.r_xl);
+    ops_hip_reduction_discard(S.r_yl);
     S.advect_x = keep;
     if (!S.step_graph[ax]) throw OPSException(OPS_RUNTIME_ERROR, "cloverleaf: graph capture failed");
   }
@@ -928,6 +949,9 @@ void graph_steps(int n) {
       throw OPSException(OPS_RUNTIME_ERROR, "cloverleaf: graph replay failed");
     S.advect_x = !S.advect_x;
     S.step++;
+    // the field summary of every summary_frequency steps runs eagerly between replays (its
+    // sums are read on the host)
+    if (S.summary_frequency != 0 && S.step % S.summary_frequency == 0) field_summary(S.summary);
   }
   double st[4];
   clover_dt_state_get(S.dt_state, st);

@@ -216,6 +216,12 @@ void cl_calc_dt(const ACC<double> &cdx, const ACC<double> &cdy, const ACC<double
 
 void cl_dt_min(const ACC<double> &dtc, double *dt_min) { *dt_min = MIN(*dt_min, dtc(0, 0)); }
 
+// position of the controlling cell (calc_dt_kernel.h: calc_dt_kernel_get)
+void cl_dt_get(const ACC<double> &cellx, const ACC<double> &celly, double *xl_pos, double *yl_pos) {
+  *xl_pos = cellx(0, 0);
+  *yl_pos = celly(0, 0);
+}
+
 // ------------------------------------------------------------ Lagrangian step
 // (PdV_kernel.h, revert_kernel.h, accelerate_kernel.h)
 

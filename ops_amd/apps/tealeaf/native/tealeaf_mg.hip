@@ -31,6 +31,16 @@
 #include "ops_hip_rt.h"
 #include "tealeaf_native.h"
 
+// V-cycle arithmetic: FP32 (tea_native_mgcg), or FP64 when built with TEA_MG_FP64
+// (tea_native_mgcg64; the same preconditioner in double, to measure what FP32 storage buys)
+#ifdef TEA_MG_FP64
+typedef double vreal;
+#define TEA_MGCG_ENTRY tea_native_mgcg64
+#else
+typedef float vreal;
+#define TEA_MGCG_ENTRY tea_native_mgcg
+#endif
+
 #define CK(x) ops_hip_check((int)(x), #x, __FILE__, __LINE__)
 
 namespace {
@@ -40,12 +50,12 @@ constexpr int FT = 1024;
 constexpr int POLL = 8;          // replays of the two-iteration graph between polls
 constexpr int COARSEST = 16;     // stop coarsening at this size (or at an odd size)
 constexpr int COARSE_SWEEPS = 8; // symmetric sweep pairs on the coarsest level
-constexpr float ALPHA_C = 1.5f;  // coarse-correction scaling
+constexpr vreal ALPHA_C = 1.5;  // coarse-correction scaling
 
 struct Level {
   int n, s;      // cells per side, pitch
-  float m;       // mass term
-  float *cx, *cy, *x, *b;
+  vreal m;       // mass term
+  vreal *cx, *cy, *x, *b;
 };
 
 struct Rec {
@@ -97,26 +107,26 @@ __device__ __forceinline__ double fold(const double *__restrict__ part, unsigned
 
 // level 0 coefficients from the FP64 conduction coefficients
 __global__ __launch_bounds__(NT) void k_fine_coef(const double *__restrict__ Kx,
-                                                  const double *__restrict__ Ky, float *cx, float *cy,
+                                                  const double *__restrict__ Ky, vreal *cx, vreal *cy,
                                                   int n, double rx, double ry) {
   int x, y;
   // faces: x and y up to n + 1 (the far boundary faces sit in the halo column / row)
   if (!cell2(n + 1, n + 1, x, y)) return;
   const long i = x + (long)y * (n + 2);
-  cx[i] = (y <= n) ? (float)(rx * Kx[i]) : 0.f;
-  cy[i] = (x <= n) ? (float)(ry * Ky[i]) : 0.f;
+  cx[i] = (y <= n) ? (vreal)(rx * Kx[i]) : vreal(0);
+  cy[i] = (x <= n) ? (vreal)(ry * Ky[i]) : vreal(0);
 }
 
 // coarse faces: the sum of the two fine faces each covers
-__global__ __launch_bounds__(NT) void k_coarsen(const float *__restrict__ fcx, const float *__restrict__ fcy,
-                                                int fs, float *ccx, float *ccy, int nc) {
+__global__ __launch_bounds__(NT) void k_coarsen(const vreal *__restrict__ fcx, const vreal *__restrict__ fcy,
+                                                int fs, vreal *ccx, vreal *ccy, int nc) {
   int X, Y;
   if (!cell2(nc + 1, nc + 1, X, Y)) return;
   const int cs = nc + 2;
   const long ic = X + (long)Y * cs;
   const int fx = 2 * X - 1, fy = 2 * Y - 1;  // first fine child (1-based)
-  ccx[ic] = (Y <= nc) ? fcx[fx + (long)fy * fs] + fcx[fx + (long)(fy + 1) * fs] : 0.f;
-  ccy[ic] = (X <= nc) ? fcy[fx + (long)fy * fs] + fcy[fx + 1 + (long)fy * fs] : 0.f;
+  ccx[ic] = (Y <= nc) ? fcx[fx + (long)fy * fs] + fcx[fx + (long)(fy + 1) * fs] : vreal(0);
+  ccy[ic] = (X <= nc) ? fcy[fx + (long)fy * fs] + fcy[fx + 1 + (long)fy * fs] : vreal(0);
 }
 
 // ---------------------------------------------------------------- V-cycle kernels
@@ -138,13 +148,13 @@ __global__ __launch_bounds__(NT) void k_smooth(Level L, int color, int init, con
   const int s = L.s;
   if (x <= L.n) {
     const long i = x + (long)y * s;
-    const float c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
-    const float diag = L.m + ((c0 + c1) + (c2 + c3));
-    float v = L.b[i];
+    const vreal c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
+    const vreal diag = L.m + ((c0 + c1) + (c2 + c3));
+    vreal v = L.b[i];
     if (!init) v += (c0 * L.x[i - 1] + c1 * L.x[i + 1]) + (c2 * L.x[i - s] + c3 * L.x[i + s]);
     L.x[i] = v / diag;
   }
-  if (init && xo <= L.n) L.x[xo + (long)y * s] = 0.f;
+  if (init && xo <= L.n) L.x[xo + (long)y * s] = vreal(0);
 }
 
 // b_coarse = sum over the 2 x 2 children of (b - A x)
@@ -153,15 +163,15 @@ __global__ __launch_bounds__(NT) void k_restrict(Level F, Level C, const Rec *S)
   int X, Y;
   if (!cell2(C.n, C.n, X, Y)) return;
   const int s = F.s;
-  float acc = 0.f;
+  vreal acc = vreal(0);
 #pragma unroll
   for (int dy = 0; dy < 2; dy++)
 #pragma unroll
     for (int dx = 0; dx < 2; dx++) {
       const long i = (2 * X - 1 + dx) + (long)(2 * Y - 1 + dy) * s;
-      const float c0 = F.cx[i], c1 = F.cx[i + 1], c2 = F.cy[i], c3 = F.cy[i + s];
-      const float diag = F.m + ((c0 + c1) + (c2 + c3));
-      const float ax = diag * F.x[i] - ((c0 * F.x[i - 1] + c1 * F.x[i + 1]) + (c2 * F.x[i - s] + c3 * F.x[i + s]));
+      const vreal c0 = F.cx[i], c1 = F.cx[i + 1], c2 = F.cy[i], c3 = F.cy[i + s];
+      const vreal diag = F.m + ((c0 + c1) + (c2 + c3));
+      const vreal ax = diag * F.x[i] - ((c0 * F.x[i - 1] + c1 * F.x[i + 1]) + (c2 * F.x[i - s] + c3 * F.x[i + s]));
       acc += F.b[i] - ax;
     }
   C.b[X + (long)Y * C.s] = acc;
@@ -190,14 +200,14 @@ __device__ __forceinline__ bool fcell(const Fine &g, long &i) {
   return ok;
 }
 
-// start: b32 = (float) r, r.r partials
-__global__ __launch_bounds__(NT) void k_start(const double *__restrict__ r, float *__restrict__ b32, Fine g,
+// start: b32 = (vreal) r, r.r partials
+__global__ __launch_bounds__(NT) void k_start(const double *__restrict__ r, vreal *__restrict__ b32, Fine g,
                                               double *part) {
   long i;
   double v = 0.0;
   if (fcell(g, i)) {
     const double ri = r[i];
-    b32[i] = (float)ri;
+    b32[i] = (vreal)ri;
     v = ri * ri;
   }
   write_partial(v, part);
@@ -216,8 +226,8 @@ __global__ __launch_bounds__(FT) void k_fold_start(const double *part, unsigned 
 }
 
 // partials of r.z and r.z_prev (z = V-cycle output, z_prev the previous one)
-__global__ __launch_bounds__(NT) void k_dots(const double *__restrict__ r, const float *__restrict__ z,
-                                             const float *__restrict__ zp, Fine g, double *part,
+__global__ __launch_bounds__(NT) void k_dots(const double *__restrict__ r, const vreal *__restrict__ z,
+                                             const vreal *__restrict__ zp, Fine g, double *part,
                                              const Rec *S) {
   if (S->stopped) return;
   long i;
@@ -263,7 +273,7 @@ __global__ __launch_bounds__(FT) void k_fold_dots(const double *part, unsigned n
 
 // p_out = z + beta p_in at the five stencil points, w = A p_out (FP64), p.w partials
 __global__ __launch_bounds__(NT) void k_pw(const double *__restrict__ Kx, const double *__restrict__ Ky,
-                                           const float *__restrict__ z, const double *__restrict__ pin,
+                                           const vreal *__restrict__ z, const double *__restrict__ pin,
                                            double *__restrict__ pout, double *__restrict__ w, Fine g,
                                            double *part, const Rec *S) {
   if (S->stopped) return;
@@ -296,10 +306,10 @@ __global__ __launch_bounds__(FT) void k_fold_pw(const double *part, unsigned n, 
   }
 }
 
-// u += alpha p, r -= alpha w, b32 = (float) r for the next V-cycle, r.r partials
+// u += alpha p, r -= alpha w, b32 = (vreal) r for the next V-cycle, r.r partials
 __global__ __launch_bounds__(NT) void k_ur(double *__restrict__ u, double *__restrict__ r,
                                            const double *__restrict__ p, const double *__restrict__ w,
-                                           float *__restrict__ b32, Fine g, double *part, const Rec *S) {
+                                           vreal *__restrict__ b32, Fine g, double *part, const Rec *S) {
   if (S->stopped) return;
   const double alpha = S->alpha;
   long i;
@@ -308,7 +318,7 @@ __global__ __launch_bounds__(NT) void k_ur(double *__restrict__ u, double *__res
     u[i] = u[i] + alpha * p[i];
     const double ri = r[i] - alpha * w[i];
     r[i] = ri;
-    b32[i] = (float)ri;
+    b32[i] = (vreal)ri;
     v = ri * ri;
   }
   write_partial(v, part);
@@ -355,7 +365,7 @@ char *carve(char *&p, size_t bytes) {
 }
 
 // V-cycle on the levels; level 0's b must hold the right-hand side, x0 receives z
-void vcycle(hipStream_t st, float *x0) {
+void vcycle(hipStream_t st, vreal *x0) {
   const dim3 blk(BX, BY);
   const int L = (int)G.lv.size();
   std::vector<Level> lv = G.lv;
@@ -384,7 +394,7 @@ void vcycle(hipStream_t st, float *x0) {
 
 }  // namespace
 
-void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat Kx, ops_dat Ky,
+void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat Kx, ops_dat Ky,
                      const tea_native_solve_params *P, tea_native_result *R) {
   (void)u0;
   if (P->nx != P->ny)
@@ -406,8 +416,8 @@ void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops
   size_t bytes = 4 * (((size_t)nb * 2 * sizeof(double) + 255) & ~(size_t)255);
   const size_t fine64 = (((size_t)(n + 2) * (n + 2) * sizeof(double)) + 255) & ~(size_t)255;
   bytes += fine64;  // second p buffer
-  for (int s : sizes) bytes += 4 * ((((size_t)(s + 2) * (s + 2) * sizeof(float)) + 255) & ~(size_t)255);
-  bytes += (((size_t)(n + 2) * (n + 2) * sizeof(float)) + 255) & ~(size_t)255;  // second z buffer
+  for (int s : sizes) bytes += 4 * ((((size_t)(s + 2) * (s + 2) * sizeof(vreal)) + 255) & ~(size_t)255);
+  bytes += (((size_t)(n + 2) * (n + 2) * sizeof(vreal)) + 255) & ~(size_t)255;  // second z buffer
   if (bytes > G.cap || G.n != n) {
     ops_hip_device_sync();
     drop_graphs();
@@ -424,16 +434,16 @@ void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops
   double *part = (double *)carve(q, (size_t)nb * 2 * sizeof(double));
   double *pb = (double *)carve(q, fine64);
   G.lv.clear();
-  float mass = 1.f;
+  vreal mass = 1;
   for (int s : sizes) {
-    const size_t lb = (size_t)(s + 2) * (s + 2) * sizeof(float);
-    Level L{s, s + 2, mass, (float *)carve(q, lb), (float *)carve(q, lb), (float *)carve(q, lb),
-            (float *)carve(q, lb)};
+    const size_t lb = (size_t)(s + 2) * (s + 2) * sizeof(vreal);
+    Level L{s, s + 2, mass, (vreal *)carve(q, lb), (vreal *)carve(q, lb), (vreal *)carve(q, lb),
+            (vreal *)carve(q, lb)};
     G.lv.push_back(L);
     mass *= 4.f;
   }
-  float *z2 = (float *)carve(q, (size_t)(n + 2) * (n + 2) * sizeof(float));
-  float *zbuf[2] = {G.lv[0].x, z2};
+  vreal *z2 = (vreal *)carve(q, (size_t)(n + 2) * (n + 2) * sizeof(vreal));
+  vreal *zbuf[2] = {G.lv[0].x, z2};
   // zero every array once (halos stay zero), then build the coefficients
   CK(hipMemsetAsync(G.mem, 0, bytes, st));
   double *u64 = (double *)u->data_d, *r64 = (double *)r->data_d, *pa = (double *)p->data_d,
@@ -452,7 +462,7 @@ void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops
   // one PCG iteration with z in zbuf[par], the previous z in zbuf[1-par], p written to
   // the buffer of parity par
   auto iteration = [&](int par) {
-    float *z = zbuf[par], *zp = zbuf[1 - par];
+    vreal *z = zbuf[par], *zp = zbuf[1 - par];
     const double *pin = par ? pa : pb;
     double *pout = par ? pb : pa;
     vcycle(st, z);

@@ -13,6 +13,7 @@
 //   1  FP64 CG with the scalars kept on the device (native kernels, tealeaf_native.hip)
 //   2  FP32-storage CG iterations with FP64 reliable updates (residual replacement)
 //   3  FP64 flexible PCG with an FP32 multigrid V-cycle preconditioner
+//   4  the same with the V-cycle in FP64 (what FP32 storage buys is the difference to 3)
 // Solver 1 and 2 stop on the same test as the reference: sqrt(r.r) below eps times the
 // residual norm of the step's initial guess.
 #include <math.h>
@@ -435,6 +436,8 @@ void solve() {
     tea_native_result R;
     if (T.solver == 3)
       tea_native_mgcg(T.u, T.u0, T.r, T.p, T.w, T.Kx, T.Ky, &P, &R);
+    else if (T.solver == 4)
+      tea_native_mgcg64(T.u, T.u0, T.r, T.p, T.w, T.Kx, T.Ky, &P, &R);
     else
       tea_native_cg(T.u, T.u0, T.r, T.p, T.w, T.z, T.Kx, T.Ky, &P, &R);
     n = R.iterations + 1;
@@ -589,7 +592,7 @@ extern "C" int tealeaf_init(int argc, const char **argv, const char *deck) {
 }
 
 extern "C" int tealeaf_set_solver(int solver) {
-  if (solver < 0 || solver > 3) return 1;
+  if (solver < 0 || solver > 4) return 1;
   T.solver = solver;
   return 0;
 }

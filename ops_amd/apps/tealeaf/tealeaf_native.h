@@ -44,6 +44,9 @@ void tea_native_cg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_d
 // R->outer reports the number of multigrid levels.
 void tea_native_mgcg(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat Kx, ops_dat Ky,
                      const tea_native_solve_params *P, tea_native_result *R);
+// solver 4: the same with the V-cycle in FP64 storage and arithmetic
+void tea_native_mgcg64(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_dat Kx, ops_dat Ky,
+                       const tea_native_solve_params *P, tea_native_result *R);
 
 // Learned warm start (the U-Net of tealeaf_unet.h). capture_features stores the network
 // input of the solve about to start (density, u0 and its residual r = u0 - A u0) as

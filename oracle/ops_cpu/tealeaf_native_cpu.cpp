@@ -359,3 +359,10 @@ void tea_unet_capture_target(int, ops_dat, ops_dat, ops_dat, ops_dat, double, do
 void tea_unet_guess_dats(ops_dat, ops_dat, ops_dat, ops_dat, ops_dat, double, double, ops_dat, int, int, double *) {
   throw OPSException(OPS_NOT_IMPLEMENTED, "tealeaf learned warm start: device build only");
 }
+
+// solver 4 (the FP64 V-cycle) is a device measurement of what FP32 storage buys; the host
+// backend restates solver 3 only
+void tea_native_mgcg64(ops_dat, ops_dat, ops_dat, ops_dat, ops_dat, ops_dat, ops_dat,
+                       const tea_native_solve_params *, tea_native_result *) {
+  throw OPSException(OPS_NOT_IMPLEMENTED, "tealeaf solver 4: not provided by the host backend");
+}

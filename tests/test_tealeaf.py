@@ -178,7 +178,7 @@ def test_gpu_fields_within_1e12_of_reference(golden, tmp_path, name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("solver", [1, 2, 3])
+@pytest.mark.parametrize("solver", [1, 2, 3, 4])
 def test_gpu_native_solvers(solver):
     from ops_amd.apps.tealeaf.decks import tea_deck
     deck = tea_deck(256, 2)
