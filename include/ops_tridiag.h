@@ -57,6 +57,22 @@ void ops_tridMultiDimBatch_Inc(int ndim, int solvedim, int *range, ops_dat a, op
                                ops_dat c, ops_dat d, ops_dat u,
                                ops_tridsolver_params *tridsolver_ctx);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+   under graph capture, tight serving loops, harnesses that forbid hipMalloc): the solve of
+   ops_tridMultiDimBatch (u == NULL: x overwrites d) or of ops_tridMultiDimBatch_Inc (u
+   given: x is added to u), launched at once (not deferred), by the sequential sweeps, whose
+   c', d' go to a static 64 MiB device array of the library (one per GPU, held until the
+   process ends) instead of a buffer it allocates: results are bitwise those of the
+   queueing entry points with OPS_TRID_ALGO=thomas. A solve needing more than 64 MiB of c',
+   d' (2 * sizeof(type) per point of the box) throws OPSException(OPS_INVALID_ARGUMENT).
+   Calls on one GPU share that array: calls on the runtime's stream are ordered, calls that
+   overlap on one GPU from other streams or processes race. Work issued before the call runs
+   first (deferred solves of the queueing entry points are launched, with their own
+   buffers). */
+void ops_tridMultiDimBatch_noalloc(int ndim, int solvedim, int *range, ops_dat a, ops_dat b,
+                                   ops_dat c, ops_dat d, ops_dat u,
+                                   ops_tridsolver_params *tridsolver_ctx);
+
 #ifdef __cplusplus
 }
 #endif

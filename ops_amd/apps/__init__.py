@@ -65,6 +65,7 @@ SIGNATURES = {
         "api_random": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
         "api_random_normal": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
         "api_batch": (_i, [_i, _pp, _i, _i, _p]),
+        "api_tridiag2": (_i, [_i, _pp, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p]),
     },
     "multicomp": {
         "multicomp_run": (_i, [_i, _pp, _i, _i, _p]),

@@ -11,6 +11,8 @@
 //                  accumulation across two loops before ops_reduction_result
 //   api_random[_normal]  ops_randomgen_init + ops_fill_random_uniform / _normal on a dat, read back with
 //                  its halos
+//   api_tridiag2   two solves in a row (the second on a smaller box) through the queueing
+//                  entry points or the allocation-free one
 //   api_batch      loop batches (ops_hip_batch_begin/end): many independent launches of one
 //                  kernel of mixed extents, two stubs of that kernel interleaved, a
 //                  tridiagonal solve between launches, eagerly and replayed from a graph
@@ -439,5 +441,54 @@ extern "C" int api_batch(int argc, const char **argv, int nloops, int mode, doub
     return 0;
   } catch (std::exception &e) {
     return fail("api_batch", e);
+  }
+}
+
+// Two solves in a row on the same dats: the box range1, then (coefficients and right-hand
+// side reset to the inputs) the box range2; mode 0 through ops_tridMultiDimBatch[_Inc],
+// mode 1 through ops_tridMultiDimBatch_noalloc. out1, out2 receive d (or u with inc) after
+// each solve, whole interior, x fastest.
+extern "C" int api_tridiag2(int argc, const char **argv, int ndim, int solvedim, const int *size,
+                            const int *range1, const int *range2, const double *a, const double *b,
+                            const double *c, const double *d, const double *u, int inc, int mode,
+                            double *out1, double *out2) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block blk = ops_decl_block(ndim, "trid2");
+    int sz[3], base[3] = {0, 0, 0}, dm[3] = {-1, -1, -1}, dp[3] = {1, 1, 1}, full[6];
+    for (int i = 0; i < ndim; i++) {
+      sz[i] = size[i];
+      full[2 * i] = 0;
+      full[2 * i + 1] = size[i];
+    }
+    double *none = nullptr;
+    ops_dat A = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "a");
+    ops_dat B = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "b");
+    ops_dat C = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "c");
+    ops_dat D = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "d");
+    ops_dat U = ops_decl_dat(blk, 1, sz, base, dm, dp, none, "double", "u");
+    ops_tridsolver_params ctx(blk);
+    const int *ranges[2] = {range1, range2};
+    double *outs[2] = {out1, out2};
+    for (int k = 0; k < 2; k++) {
+      ops_dat_set_data_slab_memspace(A, 0, (char *)a, full, OPS_HOST);
+      ops_dat_set_data_slab_memspace(B, 0, (char *)b, full, OPS_HOST);
+      ops_dat_set_data_slab_memspace(C, 0, (char *)c, full, OPS_HOST);
+      ops_dat_set_data_slab_memspace(D, 0, (char *)d, full, OPS_HOST);
+      ops_dat_set_data_slab_memspace(U, 0, (char *)u, full, OPS_HOST);
+      int r[6];
+      for (int i = 0; i < 2 * ndim; i++) r[i] = ranges[k][i];
+      if (mode == 1)
+        ops_tridMultiDimBatch_noalloc(ndim, solvedim, r, A, B, C, D, inc ? U : nullptr, &ctx);
+      else if (inc)
+        ops_tridMultiDimBatch_Inc(ndim, solvedim, r, A, B, C, D, U, &ctx);
+      else
+        ops_tridMultiDimBatch(ndim, solvedim, r, A, B, C, D, &ctx);
+      ops_dat_fetch_data_slab_memspace(inc ? U : D, 0, (char *)outs[k], full, OPS_HOST);
+    }
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_tridiag2", e);
   }
 }

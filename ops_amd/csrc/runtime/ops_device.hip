@@ -100,6 +100,7 @@ void ops_dev_init(OPS_instance *inst) {
     ops_throw(OPS_RUNTIME_CONFIGURATION_ERROR, "Error: OPS_DEVICE out of range");
   g_dev.device = inst->device_id;
   HIPCHECK(hipSetDevice(g_dev.device));
+  ops_trid_device_init(g_dev.device);
   HIPCHECK(hipStreamCreate(&g_dev.stream));
   g_dev.stage_cap = 8u << 20;
   HIPCHECK(hipHostMalloc((void **)&g_dev.stage, g_dev.stage_cap, hipHostMallocDefault));

@@ -87,6 +87,9 @@ void ops_trid_flush();
 void ops_loop_batch_flush();
 // frees the tridiagonal solver's scratch (ops_exit; graphs that used it are gone by then)
 void ops_trid_release();
+// takes the device address of the allocation-free solve's static scratch on `device`
+// (ops_dev_init, with that device current; once per device)
+void ops_trid_device_init(int device);
 // Autotuning seam. ops_tune_next returns the candidate (0..ncand-1) the next launch of
 // kernel kid should use while it is being measured, the chosen one afterwards, or -1 when
 // tuning is off (or the kernel is not tunable). A measured launch is bracketed by

@@ -29,6 +29,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "ops_internal.h"
@@ -551,7 +552,7 @@ struct Scratch {
 };
 Scratch g_scratch;
 
-void *scratch(size_t bytes) {
+void *sweep_scratch(size_t bytes) {
   if (bytes > g_scratch.cap) {
     if (g_scratch.p) g_scratch.retired.push_back(g_scratch.p);
     g_scratch.p = nullptr;
@@ -587,14 +588,17 @@ bool use_thomas() {
   return e != nullptr && !strcmp(e, "thomas");
 }
 
-// launches the systems calls[first, first + count) (same solvedim, kind and type)
+// launches the systems calls[first, first + count) (same solvedim, kind and type).
+// fixed != nullptr: the allocation-free form, sequential sweeps with c', d' in `fixed`
+// (fixed_bytes of them), no autotuning
 template <typename T, bool INC>
-void launch_chunk(const std::vector<Call> &calls, size_t first, size_t count) {
+void launch_chunk(const Call *calls, size_t first, size_t count, T *fixed = nullptr,
+                  size_t fixed_bytes = 0) {
   Batch<T> B;
   B.nsys = 0;
   B.solvedim = calls[first].solvedim;
   const int sd = B.solvedim;
-  bool part = !use_thomas();
+  bool part = fixed == nullptr && !use_thomas();
   for (size_t k = first; k < first + count; k++) {
     const Call &C = calls[k];
     if (C.range[2 * sd + 1] - C.range[2 * sd] > PP * PM) part = false;
@@ -647,8 +651,15 @@ void launch_chunk(const std::vector<Call> &calls, size_t first, size_t count) {
     ops_hip_check_launch("ops_tridMultiDimBatch");
     return;
   }
-  T *scr_p = (T *)scratch(sizeof(T) * (size_t)scr);
-  if (sd == 0) {
+  if (fixed != nullptr && sizeof(T) * (size_t)scr > fixed_bytes)
+    ops_throw(OPS_INVALID_ARGUMENT,
+              "ops_tridMultiDimBatch_noalloc: the solve needs " +
+                  std::to_string(sizeof(T) * (size_t)scr) + " bytes of c', d' scratch, more than the " +
+                  std::to_string(fixed_bytes) + " bytes of the static scratch");
+  T *scr_p = fixed != nullptr ? fixed : (T *)sweep_scratch(sizeof(T) * (size_t)scr);
+  if (sd == 0 && fixed != nullptr) {
+    hipLaunchKernelGGL((thomas_x<T, INC, 8>), dim3((unsigned)(line / XL)), dim3(XL), 0, st, B, scr_p);
+  } else if (sd == 0) {
     // candidates: LDS tiles of 16, 8 or 4 rows. One lane per line without LDS (the y/z
     // kernel with unit stride, the loads of a wavefront spread over 64 lines) measured
     // 0.049 ns per point on ADI's x solves against 0.026 for 8-row tiles
@@ -748,11 +759,11 @@ void ops_trid_flush() {
     for (size_t first = 0; first < calls.size(); first += MAXSYS) {
       const size_t count = calls.size() - first < (size_t)MAXSYS ? calls.size() - first : MAXSYS;
       if (c0.is_double) {
-        if (c0.inc) launch_chunk<double, true>(calls, first, count);
-        else launch_chunk<double, false>(calls, first, count);
+        if (c0.inc) launch_chunk<double, true>(calls.data(), first, count);
+        else launch_chunk<double, false>(calls.data(), first, count);
       } else {
-        if (c0.inc) launch_chunk<float, true>(calls, first, count);
-        else launch_chunk<float, false>(calls, first, count);
+        if (c0.inc) launch_chunk<float, true>(calls.data(), first, count);
+        else launch_chunk<float, false>(calls.data(), first, count);
       }
     }
   } catch (...) {
@@ -770,6 +781,61 @@ void ops_trid_release() {
   if (g_scratch.p) (void)hipFree(g_scratch.p);
   g_scratch.p = nullptr;
   g_scratch.cap = 0;
+}
+
+// ---------------------------------------------------------------- allocation-free form
+// c', d' of ops_tridMultiDimBatch_noalloc live in this file-scope device array, one copy per
+// GPU, whose device address is taken once per device when the runtime initialises it
+// (ops_trid_device_init); host code uses that address, never the array's name.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+void *g_noalloc_scratch[MAX_DEVICES] = {};
+}  // namespace
+
+void ops_trid_device_init(int device) {
+  if (device < 0 || device >= MAX_DEVICES)
+    ops_throw(OPS_RUNTIME_CONFIGURATION_ERROR, "ops_tridiag: device index beyond the supported count");
+  if (g_noalloc_scratch[device] != nullptr) return;
+  void *p = nullptr;
+  HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_noalloc_scratch[device] = p;
+}
+
+extern "C" void ops_tridMultiDimBatch_noalloc(int ndim, int solvedim, int *range, ops_dat a,
+                                              ops_dat b, ops_dat c, ops_dat d, ops_dat u,
+                                              ops_tridsolver_params *ctx) {
+  (void)ctx;
+  check_args(ndim, solvedim, a, b, c, d, u);
+  // work issued before this call (a pending loop batch, deferred solves) runs first
+  ops_trid_flush();
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES || g_noalloc_scratch[dev] == nullptr)
+    ops_throw(OPS_RUNTIME_ERROR, "ops_tridMultiDimBatch_noalloc: called before ops_init on this device");
+  Call C;
+  C.ndim = ndim;
+  C.solvedim = solvedim;
+  C.inc = u != nullptr;
+  C.is_double = !strcmp(a->type, "double");
+  for (int i = 0; i < 6; i++) C.range[i] = i < 2 * ndim ? range[i] : 0;
+  C.a = a;
+  C.b = b;
+  C.c = c;
+  C.d = d;
+  C.u = u;
+  void *scr = g_noalloc_scratch[dev];
+  if (C.is_double) {
+    if (C.inc) launch_chunk<double, true>(&C, 0, 1, (double *)scr, NOALLOC_SCRATCH_BYTES);
+    else launch_chunk<double, false>(&C, 0, 1, (double *)scr, NOALLOC_SCRATCH_BYTES);
+  } else {
+    if (C.inc) launch_chunk<float, true>(&C, 0, 1, (float *)scr, NOALLOC_SCRATCH_BYTES);
+    else launch_chunk<float, false>(&C, 0, 1, (float *)scr, NOALLOC_SCRATCH_BYTES);
+  }
+  if (C.inc) u->dirty_hd = 2;
+  else d->dirty_hd = 2;
 }
 
 extern "C" void ops_tridMultiDimBatch(int ndim, int solvedim, int *range, ops_dat a, ops_dat b,

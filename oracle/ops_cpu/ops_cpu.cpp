@@ -755,6 +755,12 @@ extern "C" void ops_tridMultiDimBatch_Inc(int ndim, int sd, int *range, ops_dat 
                                           ops_dat c, ops_dat d, ops_dat u, ops_tridsolver_params *) {
   thomas<true>(ndim, sd, range, a, b, c, d, u);
 }
+// the allocation-free device form is the same solve on the host
+extern "C" void ops_tridMultiDimBatch_noalloc(int ndim, int sd, int *range, ops_dat a, ops_dat b,
+                                              ops_dat c, ops_dat d, ops_dat u, ops_tridsolver_params *) {
+  if (u) thomas<true>(ndim, sd, range, a, b, c, d, u);
+  else thomas<false>(ndim, sd, range, a, b, c, d, nullptr);
+}
 
 static void dump_all(const char *dir) {
   std::string idx = std::string(dir) + "/index.txt";
