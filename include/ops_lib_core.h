@@ -29,6 +29,7 @@
 #include <sstream>
 #include <string>
 #include <ostream>
+#include <iostream>
 #include <complex>
 #endif
 
@@ -106,8 +107,23 @@ typedef std::complex<float> complexf;
 extern "C" {
 #endif
 
-struct OPS_instance;
+#ifdef __cplusplus
+}
+/* The part of the reference's OPS_instance (ops/c/include/ops_instance.h:67-377) that
+   applications use: the singleton created by ops_init, the diagnostics level (OPS_diags,
+   ops_init's diags_level) and the structure-of-arrays flag (OPS_soa, as ops_set_soa) they
+   read or set, and the power-counter hook (no counters on this backend). Each backend keeps
+   the rest of its state in a type derived from this one. */
+struct OPS_instance {
+  static OPS_instance *getOPSInstance();
+  int OPS_diags = 1;
+  int OPS_soa = 0;
+  void reset_power_counters() {}
+};
+extern "C" {
+#else
 typedef struct OPS_instance OPS_instance;
+#endif
 
 /* ---- handle types: same field names and meaning as the reference (ops_lib_core.h:183-520) ---- */
 
@@ -507,5 +523,9 @@ private:
 };
 
 #endif /* __cplusplus */
+
+/* the reference's core header brings these in as well (ops_util.h, ops_hdf5.h) */
+#include "ops_util.h"
+#include "ops_hdf5.h"
 
 #endif /* OPS_AMD_LIB_CORE_H */

@@ -28,7 +28,7 @@
 // a kernel parameter, so this is only referenced by the host copies of user kernels
 struct ops_v1_shape ops_v1;
 
-static OPS_instance *g_inst = nullptr;
+static ops_runtime_instance *g_inst = nullptr;
 static std::string g_last_error;
 static std::mt19937 g_rng;
 
@@ -51,58 +51,69 @@ extern "C" const char *ops_get_last_error(void) { return g_last_error.c_str(); }
 // The reference addresses dats without a range check (ops_seq_v2.h:55-62); on the GPU an
 // access outside the allocation is a silent corruption or a memory fault, so every loop
 // checks, on the host, the extent its stencils reach in each dimension.
-void ops_check_loop_bounds(const char *name, int dim, const int *start, const int *end,
-                           const ops_arg *args, int nargs) {
-  for (int i = 0; i < nargs; i++) {
-    const ops_arg &a = args[i];
-    if (a.argtype != OPS_ARG_DAT || !a.opt || a.dat == nullptr || a.stencil == nullptr) continue;
-    const ops_stencil s = a.stencil;
-    const ops_dat dat = a.dat;
-    for (int d = 0; d < dim && d < dat->block->dims; d++) {
-      int stride = d < s->dims ? s->stride[d] : 1;
-      // multigrid stencils: restrict reads index * m, prolong index / m (C division)
-      const int m = (s->type != 0 && d < s->dims) ? s->mgrid_stride[d] : 1;
-      int pmin = 0, pmax = 0;
-      if (d < s->dims)
-        for (int p = 0; p < s->points; p++) {
-          int v = s->stencil[p * s->dims + d];
-          if (p == 0 || v < pmin) pmin = v;
-          if (p == 0 || v > pmax) pmax = v;
-        }
-      long off = -(long)dat->base[d] - dat->d_m[d];
-      long lo = (long)start[d] * stride + pmin + off;
-      long hi = (long)(end[d] - 1) * stride + pmax + off;
-      if (s->type == 2) {
-        lo = (long)start[d] * stride * m + pmin + off;
-        hi = (long)(end[d] - 1) * stride * m + pmax + off;
-      } else if (s->type == 1) {
-        lo = (long)(start[d] / m) * stride + pmin + off;
-        hi = (long)((end[d] - 1) / m) * stride + pmax + off;
+bool ops_loop_arg_in_bounds(const char *name, int dim, const int *start, const int *end,
+                            const ops_arg &a, std::string *why) {
+  if (a.argtype != OPS_ARG_DAT || a.dat == nullptr || a.stencil == nullptr) return true;
+  const ops_stencil s = a.stencil;
+  const ops_dat dat = a.dat;
+  for (int d = 0; d < dim && d < dat->block->dims; d++) {
+    int stride = d < s->dims ? s->stride[d] : 1;
+    // multigrid stencils: restrict reads index * m, prolong index / m (C division)
+    const int m = (s->type != 0 && d < s->dims) ? s->mgrid_stride[d] : 1;
+    int pmin = 0, pmax = 0;
+    if (d < s->dims)
+      for (int p = 0; p < s->points; p++) {
+        int v = s->stencil[p * s->dims + d];
+        if (p == 0 || v < pmin) pmin = v;
+        if (p == 0 || v > pmax) pmax = v;
       }
-      if (stride == 0) {
-        lo = pmin + off;
-        hi = pmax + off;
-      }
-      if (lo < 0 || hi >= dat->size[d]) {
+    long off = -(long)dat->base[d] - dat->d_m[d];
+    long lo = (long)start[d] * stride + pmin + off;
+    long hi = (long)(end[d] - 1) * stride + pmax + off;
+    if (s->type == 2) {
+      lo = (long)start[d] * stride * m + pmin + off;
+      hi = (long)(end[d] - 1) * stride * m + pmax + off;
+    } else if (s->type == 1) {
+      lo = (long)(start[d] / m) * stride + pmin + off;
+      hi = (long)((end[d] - 1) / m) * stride + pmax + off;
+    }
+    if (stride == 0) {
+      lo = pmin + off;
+      hi = pmax + off;
+    }
+    if (lo < 0 || hi >= dat->size[d]) {
+      if (why) {
         char buf[512];
         snprintf(buf, sizeof buf,
                  "Error: ops_par_loop %s: dat %s is addressed at %ld..%ld in dimension %d, "
                  "outside its allocation 0..%d (size %d, d_m %d, d_p %d)",
                  name ? name : "", dat->name, lo, hi, d, dat->size[d] - 1, dat->size[d],
                  dat->d_m[d], dat->d_p[d]);
-        ops_throw(OPS_INVALID_ARGUMENT, buf);
+        *why = buf;
       }
+      return false;
     }
   }
+  return true;
 }
 
-OPS_instance *ops_instance_get(bool create) {
+void ops_check_loop_bounds(const char *name, int dim, const int *start, const int *end,
+                           const ops_arg *args, int nargs) {
+  std::string why;
+  for (int i = 0; i < nargs; i++)
+    if (args[i].opt && !ops_loop_arg_in_bounds(name, dim, start, end, args[i], &why))
+      ops_throw(OPS_INVALID_ARGUMENT, why);
+}
+
+ops_runtime_instance *ops_instance_get(bool create) {
   if (g_inst == nullptr) {
     if (!create) ops_throw(OPS_RUNTIME_ERROR, "Error: OPS used before ops_init");
-    g_inst = new OPS_instance();
+    g_inst = new ops_runtime_instance();
   }
   return g_inst;
 }
+
+OPS_instance *OPS_instance::getOPSInstance() { return ops_instance_get(); }
 
 static char *copy_str(const char *s) {
   if (s == nullptr) return nullptr;
@@ -114,12 +125,12 @@ static char *copy_str(const char *s) {
 
 // ---------------------------------------------------------------- init / exit
 
-static void parse_arg(OPS_instance *in, const char *a) {
+static void parse_arg(ops_runtime_instance *in, const char *a) {
   const char *p;
   if ((p = strstr(a, "OPS_BLOCK_SIZE_X="))) in->block_size_x = atoi(p + 17);
   if ((p = strstr(a, "OPS_BLOCK_SIZE_Y="))) in->block_size_y = atoi(p + 17);
   if ((p = strstr(a, "OPS_BLOCK_SIZE_Z="))) in->block_size_z = atoi(p + 17);
-  if ((p = strstr(a, "-OPS_DIAGS="))) in->diags = atoi(p + 11);
+  if ((p = strstr(a, "-OPS_DIAGS="))) in->OPS_diags = atoi(p + 11);
   if ((p = strstr(a, "OPS_DEVICE="))) in->device_id = atoi(p + 11);
   if ((p = strstr(a, "OPS_DUMP_DIR="))) in->dump_dir = std::string(p + 13);
   if ((p = strstr(a, "OPS_HIP_TIMING="))) in->timing = atoi(p + 15);
@@ -130,8 +141,8 @@ static void parse_arg(OPS_instance *in, const char *a) {
 
 extern "C" void ops_init(const int argc, const char *const argv[], const int diags_level) {
   if (g_inst != nullptr) ops_exit();
-  OPS_instance *in = ops_instance_get(true);
-  in->diags = diags_level;
+  ops_runtime_instance *in = ops_instance_get(true);
+  in->OPS_diags = diags_level;
   const char *env = getenv("OPS_DUMP_DIR");
   if (env) in->dump_dir = env;
   // autotuning is per session: off unless this ops_init's environment or argv asks
@@ -161,7 +172,7 @@ static void free_dat(ops_dat d) {
 
 extern "C" void ops_exit(void) {
   if (g_inst == nullptr) return;
-  OPS_instance *in = g_inst;
+  ops_runtime_instance *in = g_inst;
   ops_dev_sync();
   if (!in->dump_dir.empty()) ops_dump_all_dats(in, in->dump_dir.c_str());
   for (ops_dat d : in->dats)
@@ -199,7 +210,7 @@ extern "C" void ops_exit(void) {
   g_inst = nullptr;
 }
 
-extern "C" void ops_set_soa(const int soa_val) { ops_instance_get()->soa = soa_val; }
+extern "C" void ops_set_soa(const int soa_val) { ops_instance_get()->OPS_soa = soa_val; }
 
 extern "C" void ops_partition(const char *routine) { (void)routine; }
 extern "C" int ops_is_root(void) { return 1; }
@@ -208,7 +219,7 @@ extern "C" void ops_execute(OPS_instance *instance) { (void)instance; }
 // ---------------------------------------------------------------- declarations
 
 extern "C" ops_block ops_decl_block(int dims, const char *name) {
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   if (dims < 1 || dims > OPS_MAX_DIM)
     ops_throw(OPS_INVALID_ARGUMENT, "Error: ops_decl_block -- dims must be between 1 and 5");
   ops_block b = (ops_block)calloc(1, sizeof(ops_block_core));
@@ -292,13 +303,13 @@ static size_t dat_bytes(ops_dat dat) {
 extern "C" ops_dat ops_decl_dat_char(ops_block block, int dim, int *block_size, int *base,
                                      int *d_m, int *d_p, int *stride, char *data,
                                      int type_size, char const *type, char const *name) {
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   if (block == nullptr) ops_throw(OPS_INVALID_ARGUMENT, "Error: ops_decl_dat -- invalid block");
   ops_dat dat = new ops_dat_core();
   memset((void *)dat, 0, sizeof(ops_dat_core));
   dat->index = (int)in->dats.size();
   init_dat_metadata(dat, block, dim, block_size, base, d_m, d_p, stride, type_size, type, name);
-  compute_base_offset(dat, in->soa);
+  compute_base_offset(dat, in->OPS_soa);
   size_t bytes = dat_bytes(dat);
   dat->mem = bytes;
   // HBM allocation, zero-filled (ops_device_singlenode_common.cpp:84-104)
@@ -343,7 +354,7 @@ void ops_put_data(ops_dat dat) {
 }
 
 extern "C" void ops_free_dat(ops_dat dat) {
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   ops_dev_sync();
   for (auto &d : in->dats)
     if (d == dat) d = nullptr;
@@ -379,7 +390,7 @@ extern "C" void ops_dat_deep_copy(ops_dat target, ops_dat orig) {
 }
 
 extern "C" ops_dat ops_dat_copy(ops_dat orig) {
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   ops_dat dat = new ops_dat_core();
   memset((void *)dat, 0, sizeof(ops_dat_core));
   dat->index = (int)in->dats.size();
@@ -391,7 +402,7 @@ extern "C" ops_dat ops_dat_copy(ops_dat orig) {
 
 static ops_stencil make_stencil(int dims, int points, const int *sten, const int *stride,
                                 const char *name, int type) {
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   ops_stencil s = (ops_stencil)calloc(1, sizeof(ops_stencil_core));
   s->index = (int)in->stencils.size();
   s->dims = dims;
@@ -529,7 +540,7 @@ extern "C" ops_arg ops_arg_reduce(ops_reduction handle, int dim, const char *typ
 
 extern "C" ops_reduction ops_decl_reduction_handle(int size, const char *type,
                                                    const char *name) {
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   ops_reduction r = (ops_reduction)calloc(1, sizeof(ops_reduction_core));
   r->size = size;
   r->data = (char *)calloc((size_t)size, 1);
@@ -550,6 +561,52 @@ extern "C" void ops_reduction_result_char(ops_reduction handle, int type_size, c
   }
   memcpy(ptr, handle->data, (size_t)handle->size);
   handle->initialized = 0;
+}
+
+// ---------------------------------------------------------------- ops_util.h, ops_hdf5.h
+
+static void *alloc_or_exit(void *p, size_t size, const char *what) {
+  if (p == nullptr && size != 0) {
+    fprintf(stderr, "%s: allocation of %zu bytes failed\n", what, size);
+    exit(-1);
+  }
+  return p;
+}
+extern "C" void *xmalloc(size_t size) { return alloc_or_exit(malloc(size), size, "xmalloc"); }
+extern "C" void *xrealloc(void *ptr, size_t size) {
+  return alloc_or_exit(realloc(ptr, size), size, "xrealloc");
+}
+extern "C" void *xcalloc(size_t number, size_t size) {
+  return alloc_or_exit(calloc(number, size), number * size, "xcalloc");
+}
+
+[[noreturn]] static void no_hdf5(const char *fn) {
+  ops_throw(OPS_HDF5_ERROR, std::string(fn) + ": HDF5 I/O is not available (this build has no HDF5 library)");
+}
+extern "C" ops_dat ops_decl_dat_hdf5(ops_block, int, char const *, char const *, char const *) {
+  no_hdf5("ops_decl_dat_hdf5");
+}
+extern "C" ops_block ops_decl_block_hdf5(int, const char *, char const *) { no_hdf5("ops_decl_block_hdf5"); }
+extern "C" ops_stencil ops_decl_stencil_hdf5(int, int, const char *, char const *) {
+  no_hdf5("ops_decl_stencil_hdf5");
+}
+extern "C" ops_stencil ops_decl_strided_stencil_hdf5(int, int, const char *, char const *) {
+  no_hdf5("ops_decl_strided_stencil_hdf5");
+}
+extern "C" ops_halo ops_decl_halo_hdf5(ops_dat, ops_dat, char const *) { no_hdf5("ops_decl_halo_hdf5"); }
+extern "C" void ops_fetch_dat_hdf5_file(ops_dat, char const *) { no_hdf5("ops_fetch_dat_hdf5_file"); }
+extern "C" void ops_fetch_block_hdf5_file(ops_block, char const *) { no_hdf5("ops_fetch_block_hdf5_file"); }
+extern "C" void ops_fetch_stencil_hdf5_file(ops_stencil, char const *) {
+  no_hdf5("ops_fetch_stencil_hdf5_file");
+}
+extern "C" void ops_fetch_halo_hdf5_file(ops_halo, char const *) { no_hdf5("ops_fetch_halo_hdf5_file"); }
+extern "C" void ops_read_dat_hdf5(ops_dat) { no_hdf5("ops_read_dat_hdf5"); }
+extern "C" void ops_dump_to_hdf5(char const *) { no_hdf5("ops_dump_to_hdf5"); }
+extern "C" void ops_write_const_hdf5(char const *, int, char const *, char *, char const *) {
+  no_hdf5("ops_write_const_hdf5");
+}
+extern "C" void ops_get_const_hdf5(char const *, int, char const *, char *, char const *) {
+  no_hdf5("ops_get_const_hdf5");
 }
 
 // ---------------------------------------------------------------- checkpointing
@@ -609,7 +666,7 @@ static void check_halo_side(ops_dat dat, const int *base, const int *dir, const 
 
 extern "C" ops_halo ops_decl_halo(ops_dat from, ops_dat to, int *iter_size, int *from_base,
                                   int *to_base, int *from_dir, int *to_dir) {
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   ops_halo h = (ops_halo)calloc(1, sizeof(ops_halo_core));
   h->index = (int)in->halos.size();
   h->from = from;
@@ -636,7 +693,7 @@ extern "C" ops_halo ops_decl_halo(ops_dat from, ops_dat to, int *iter_size, int 
 }
 
 extern "C" ops_halo_group ops_decl_halo_group(int nhalos, ops_halo *halos) {
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   ops_halo_group g = (ops_halo_group)calloc(1, sizeof(ops_halo_group_core));
   g->nhalos = nhalos;
   g->halos = (ops_halo *)calloc((size_t)std::max(1, nhalos), sizeof(ops_halo));
@@ -753,8 +810,8 @@ extern "C" void ops_timers(double *cpu, double *et) {
 }
 
 extern "C" void ops_diagnostic_output(void) {
-  OPS_instance *in = ops_instance_get();
-  if (in->diags <= 1) return;
+  ops_runtime_instance *in = ops_instance_get();
+  if (in->OPS_diags <= 1) return;
   printf("\n OPS diagnostic output\n --------------------\n");
   printf("\n block dimension\n -------------------\n");
   for (ops_block b : in->blocks) printf(" %-10s %dD\n", b->name, b->dims);
@@ -807,7 +864,7 @@ extern "C" void ops_print_dat_to_txtfile(ops_dat dat, const char *file_name) {
 }
 
 // full-precision dump of every dat (same format as the oracle's reference dumps)
-void ops_dump_all_dats(OPS_instance *in, const char *dir) {
+void ops_dump_all_dats(ops_runtime_instance *in, const char *dir) {
   std::string idx = std::string(dir) + "/index.txt";
   FILE *fi = fopen(idx.c_str(), "w");
   if (!fi) return;
@@ -828,6 +885,18 @@ void ops_dump_all_dats(OPS_instance *in, const char *dir) {
     }
   }
   fclose(fi);
+  // each reduction handle's last value read by ops_reduction_result: name, type, size and
+  // its bytes in hex
+  std::string rfn = std::string(dir) + "/reductions.txt";
+  FILE *fr = fopen(rfn.c_str(), "w");
+  if (!fr) return;
+  for (ops_reduction r : in->reductions) {
+    if (!r) continue;
+    fprintf(fr, "%s %s %d", r->name, r->type, r->size);
+    for (int b = 0; b < r->size; b++) fprintf(fr, " %02x", (unsigned char)r->data[b]);
+    fprintf(fr, "\n");
+  }
+  fclose(fr);
 }
 
 // ---------------------------------------------------------------- data access

@@ -89,7 +89,7 @@ extern "C" void ops_hip_check_launch(const char *name) {
 
 static void loop_batch_reset();
 
-void ops_dev_init(OPS_instance *inst) {
+void ops_dev_init(ops_runtime_instance *inst) {
   if (g_dev.live) return;
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
@@ -110,7 +110,7 @@ void ops_dev_init(OPS_instance *inst) {
   loop_batch_reset();
 }
 
-void ops_dev_exit(OPS_instance *inst) {
+void ops_dev_exit(ops_runtime_instance *inst) {
   (void)inst;
   if (!g_dev.live) return;
   loop_batch_reset();
@@ -422,10 +422,15 @@ static void lowdim_init(const ops_arg &a) {
   else ops_throw(OPS_NOT_IMPLEMENTED, std::string("low-dim reduction into a dat of type ") + d->type);
 }
 
+// a dat argument the loop addresses (active or not: see ops_hip_loop_begin)
+static inline bool live(const ops_arg &a) {
+  return a.argtype == OPS_ARG_DAT && a.dat != nullptr && a.stencil != nullptr;
+}
+
 extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *name,
                                   ops_block block, int dim, const int *range, ops_arg *args,
                                   int nargs) {
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   // another launch of the stub whose batch is pending may join that batch: everything
   // else pending runs first
   g_batch_keep = g_batch_on && lp->batch != nullptr && lp->batch == g_batch_fn;
@@ -438,10 +443,16 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   lp->timed = 0;
   lp->bytes = 0;
   lp->alias = 0;
+  // A dat argument is addressed whatever its opt flag, as the reference's sequential path
+  // does (ops_seq.h:184-196 computes every dat argument's pointer; ops_lib_core.cpp:
+  // 2319-2331 and ops_dummy_singlenode.cpp:140-148 exchange and mark it): a kernel may guard
+  // its accesses with another flag than the one its argument was passed with
+  // (apps/c/CloverLeaf_3D/update_halo.cpp:187-190 swaps energy0's and energy1's), and
+  // then reads and writes the dat there too.
   for (int i = 0; i < nargs; i++)
     for (int j = i + 1; j < nargs; j++)
-      if (args[i].argtype == OPS_ARG_DAT && args[j].argtype == OPS_ARG_DAT && args[i].opt && args[j].opt &&
-          args[i].dat == args[j].dat && (args[i].acc != OPS_READ || args[j].acc != OPS_READ))
+      if (live(args[i]) && live(args[j]) && args[i].dat == args[j].dat &&
+          (args[i].acc != OPS_READ || args[j].acc != OPS_READ))
         lp->alias = 1;
   int start[OPS_MAX_DIM] = {0, 0, 0, 0, 0}, end[OPS_MAX_DIM] = {1, 1, 1, 1, 1};
   long total = 1;
@@ -465,7 +476,12 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
     lp->red_part[i] = nullptr;
     lp->red_value[i] = nullptr;
     lp->red_init[i] = 0;
-    if (a.argtype == OPS_ARG_DAT && a.opt) {
+    for (int d = 0; d < 3; d++) {
+      lp->step[i][d] = 0;
+      lp->mg[i][d] = 1;
+    }
+    lp->xdim[i] = lp->ydim[i] = lp->zdim[i] = lp->mdim[i] = 0;
+    if (live(a)) {
       ops_dat dat = a.dat;
       if (dat->locked_hd != 0)
         ops_throw(OPS_RUNTIME_ERROR,
@@ -485,7 +501,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
                       (stype == 1 ? " has a prolong stencil but was not translated as one"
                                   : " was translated for a prolong stencil but has another"));
       auto mgs = [&](int d) { return (stype != 0 && d < a.stencil->dims) ? a.stencil->mgrid_stride[d] : 1; };
-      size_t esz = in->soa ? (size_t)dat->type_size : (size_t)dat->elem_size;
+      size_t esz = in->OPS_soa ? (size_t)dat->type_size : (size_t)dat->elem_size;
       long long off = 0, cum = 1;
       for (int d = 0; d < dim; d++) {
         int s = (d < a.stencil->dims) ? sst[d] : 1;
@@ -495,7 +511,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
         cum *= dat->size[d];
       }
       lp->ptr[i] = dat->data_d + off;
-      int comp = in->soa ? 1 : dat->dim;
+      int comp = in->OPS_soa ? 1 : dat->dim;
       long long cumd = 1;
       for (int d = 0; d < 3; d++) {
         int s = (d < dim && d < a.stencil->dims) ? sst[d] : 0;
@@ -515,7 +531,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       lp->ydim[i] = dat->size[1];
       lp->zdim[i] = dat->size[2];
       lp->mdim[i] = dat->dim;
-      if (total > 0) lp->bytes += transfer_bytes(dim, start, end, &a);
+      if (total > 0 && a.opt) lp->bytes += transfer_bytes(dim, start, end, &a);
     }
   }
   if (total == 0) {
@@ -523,6 +539,12 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
     return 0;
   }
   ops_check_loop_bounds(name, dim, start, end, args, nargs);
+  // an inactive argument whose stencil would leave the allocation gets no address (its
+  // kernel is expected not to touch it) rather than an error
+  for (int i = 0; i < nargs; i++)
+    if (lp->ptr[i] != nullptr && !args[i].opt &&
+        !ops_loop_arg_in_bounds(name, dim, start, end, args[i], nullptr))
+      lp->ptr[i] = nullptr;
   // workgroup shape: tuned, else 64 lanes along x (coalesced, one wavefront per row
   // segment) and the remaining threads stacked along y/z
   auto &kt = ops_kernel_table();
@@ -602,7 +624,7 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   // taken as 24-bit products along x and y (full-rate multiplies): the x and y steps must
   // be below 2^24 elements and the lane offset must fit 32 bits.
   for (int i = 0; i < nargs; i++) {
-    if (args[i].argtype != OPS_ARG_DAT || !args[i].opt) continue;
+    if (lp->ptr[i] == nullptr || args[i].argtype != OPS_ARG_DAT) continue;
     const long long lane_max = (long long)(bx - 1) * lp->step[i][0] +
                                (long long)(by - 1) * lp->step[i][1] +
                                (long long)(bz - 1) * lp->step[i][2];
@@ -757,9 +779,9 @@ extern "C" void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs) {
   }
   for (int i = 0; i < nargs; i++) {
     ops_arg &a = args[i];
-    if (a.argtype == OPS_ARG_DAT && a.opt && a.acc != OPS_READ) {
+    if (live(a) && a.acc != OPS_READ) {
       a.dat->dirty_hd = 2;
-      if (a.dat->e_dat) a.dat->edat_acc = a.acc;
+      if (a.opt && a.dat->e_dat) a.dat->edat_acc = a.acc;
     }
     if (a.argtype == OPS_ARG_GBL && a.acc != OPS_READ && lp->nblocks > 0)
       ((ops_reduction)a.data)->dirty_d = 1;
@@ -864,7 +886,7 @@ void ops_dev_halo_copy(ops_halo h) {
   int ncomp = f->dim < t->dim ? f->dim : t->dim;
   // element strides in units of one component: AoS, or SoA with each component one
   // allocation plane further on (ops_host_singlenode.cpp:124-160)
-  const int soa = ops_instance_get()->soa;
+  const int soa = ops_instance_get()->OPS_soa;
   long long fplane = 1, tplane = 1;
   for (int a = 0; a < OPS_MAX_DIM; a++) {
     fplane *= f->size[a];
@@ -897,12 +919,12 @@ void ops_dev_halo_copy(ops_halo h) {
 // interior coordinates [begin, end) per dimension (ops_util.cpp:353-500).
 void ops_dev_slab_copy(ops_dat dat, char *buf, int buf_on_device, const int *range, int dir) {
   ops_trid_flush();
-  OPS_instance *in = ops_instance_get();
+  ops_runtime_instance *in = ops_instance_get();
   int nd = dat->block->dims;
   AffineCopy c;
   c.ndim = nd;
   long long pts = 1, cum = 1, bcum = 1, base = 0;
-  int soa = in->soa;
+  int soa = in->OPS_soa;
   long long plane = 1;
   for (int d = 0; d < nd; d++) plane *= dat->size[d];
   for (int d = 0; d < nd; d++) {

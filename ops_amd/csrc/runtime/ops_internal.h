@@ -39,9 +39,8 @@ struct ops_const_entry {
   long owned_bytes = 0;
 };
 
-struct OPS_instance {
-  int diags = 1;
-  int soa = 0;
+// the runtime's state of an OPS session (OPS_instance holds OPS_diags and OPS_soa)
+struct ops_runtime_instance : OPS_instance {
   int block_size_x = 0, block_size_y = 0, block_size_z = 0;
   int device_id = 0;
   int timing = 0;  // per-kernel HIP events
@@ -55,10 +54,14 @@ struct OPS_instance {
   std::string dump_dir;                          // OPS_DUMP_DIR= (dats written at ops_exit)
 };
 
-OPS_instance *ops_instance_get(bool create = false);
+ops_runtime_instance *ops_instance_get(bool create = false);
 [[noreturn]] void ops_throw(int code, const std::string &msg);
-// Throws OPS_INVALID_ARGUMENT when a dat argument of a loop over [start, end) would be
-// addressed outside its allocation (stencil extent included).
+// Whether a dat argument of a loop over [start, end) stays inside its allocation (stencil
+// extent included); the message for the first dimension that leaves it goes to *why.
+bool ops_loop_arg_in_bounds(const char *name, int dim, const int *start, const int *end,
+                            const ops_arg &a, std::string *why);
+// Throws OPS_INVALID_ARGUMENT when an active (opt != 0) dat argument of a loop over
+// [start, end) would be addressed outside its allocation.
 void ops_check_loop_bounds(const char *name, int dim, const int *start, const int *end,
                            const ops_arg *args, int nargs);
 
@@ -68,8 +71,8 @@ std::vector<ops_kernel_entry> &ops_kernel_table();
 std::map<const void *, ops_const_entry> &ops_const_table();
 
 // ---- device layer (ops_device.hip) ----
-void ops_dev_init(OPS_instance *inst);
-void ops_dev_exit(OPS_instance *inst);
+void ops_dev_init(ops_runtime_instance *inst);
+void ops_dev_exit(ops_runtime_instance *inst);
 void *ops_dev_malloc(size_t bytes);
 void ops_dev_free(void *p);
 void ops_dev_memset(void *p, int v, size_t bytes);
@@ -108,6 +111,6 @@ void ops_dev_timing_flush();
 // ---- host-side helpers (ops_core.cpp) ----
 void ops_put_data(ops_dat dat);  // host-dirty -> device
 void ops_dat_host_alloc(ops_dat dat);
-void ops_dump_all_dats(OPS_instance *inst, const char *dir);
+void ops_dump_all_dats(ops_runtime_instance *inst, const char *dir);
 
 #endif

@@ -141,7 +141,8 @@ def find_functions(text: str, masked: Optional[str] = None) -> List[FuncDef]:
         if j >= len(masked) or masked[j] != "{":
             continue  # a declaration
         be = match_bracket(masked, j)
-        params = [text[s:e].strip() for s, e in split_top(masked, op + 1, cl)]
+        # parameter text without its comments (a parameter list may carry `// ...` notes)
+        params = [" ".join(masked[s:e].split()) for s, e in split_top(masked, op + 1, cl)]
         if params == ["void"]:
             params = []
         out.append(FuncDef(m.group(2), m.start(), be + 1, params, j, cl))

@@ -138,6 +138,25 @@ def kernel_headers(app_dir: str, sources: List[str]) -> Dict[str, Tuple[str, cpa
     return out
 
 
+def strip_include_guard(text: str) -> str:
+    """The header without its own include guard (`#ifndef G` / `#define G` before any other
+    code, and the file's last `#endif`): headers are pasted into one translation unit, and
+    two headers of an application may share a guard name (the reference's TeaLeaf
+    tea_leaf_jacobi_kernels.h reuses tea_leaf_cheby_kernels.h's), which would drop the
+    second one."""
+    masked = cparse.mask(text)
+    m = re.match(r"\s*#\s*ifndef\s+(\w+)\s*\n\s*#\s*define\s+(\w+)[^\n]*\n", masked)
+    if not m or m.group(1) != m.group(2):
+        return text
+    ends = [e for e in re.finditer(r"(?m)^\s*#\s*endif\b[^\n]*$", masked)]
+    if not ends or masked[ends[-1].end():].strip():
+        return text
+    e = ends[-1]
+    # blank the guard lines, keeping every other character (and line) in place
+    return (" " * 0 + "\n" * text[:m.end()].count("\n") + text[m.end():e.start()] +
+            "\n" * text[e.start():e.end()].count("\n") + text[e.end():])
+
+
 def is_v1_kernel(text: str, f: cparse.FuncDef) -> bool:
     """API v1 kernel: only pointer parameters, dats addressed with the OPS_ACC<n> macros
     (directly or through macros of the application's own)."""
@@ -669,10 +688,14 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
     v1_kernels = set()
     hoisted, bodies = [], []
     for h in used_headers:
-        text = Path(h).read_text()
-        # API v1 kernels take the accessor sizes as one more (by-value) parameter
+        text = strip_include_guard(Path(h).read_text())
+        # API v1 kernels take the accessor sizes as one more (by-value) parameter; so do the
+        # header's other pointer-only functions that address through OPS_ACC<n> (kernels no
+        # loop of this application uses still compile for the device)
         fdefs = [f for f in cparse.find_functions(text)
-                 if f.name in used_kernels and is_v1_kernel(text, f)]
+                 if is_v1_kernel(text, f) and
+                 (f.name in used_kernels or
+                  re.search(r"\bOPS_ACC(?:_MD)?\d+\s*\(", cparse.mask(text[f.body_open:f.end])))]
         for f in sorted(fdefs, key=lambda f: f.start, reverse=True):
             text = add_v1_param(text, f)
             v1_kernels.add(f.name)

@@ -8,7 +8,10 @@
  *   - every index of the half-open range is visited, x fastest;
  *   - ops_arg_gbl READ passes the user's pointer, ops_arg_reduce the handle's running
  *     value (:117-126), ops_arg_idx the loop index (:127-134);
- *   - writes mark the dat host-dirty (:299-301).
+ *   - writes mark the dat host-dirty (:299-301);
+ *   - a dat argument is addressed whatever its opt flag (ops_seq.h:184-196 of the
+ *     reference computes every dat argument's pointer), unless its stencil would leave
+ *     the allocation.
  * Kernels written against API v1 (plain pointer parameters, ops/c/include/ops_seq.h:129-296)
  * receive the element pointer at the current point; the sizes behind their OPS_ACC<n>
  * macros (size[0..2] and the component count of each dat argument) are set in `ops_v1`
@@ -43,6 +46,7 @@
 
 extern "C" void ops_cpu_before_loop(ops_arg *args, int nargs);
 extern "C" void ops_cpu_check_bounds(int dim, const int *range, const ops_arg *args, int nargs);
+extern "C" int ops_cpu_arg_in_bounds(int dim, const int *range, const ops_arg *a);
 extern "C" void ops_cpu_after_loop(ops_arg *args, int nargs);
 extern "C" int ops_cpu_soa(void);
 extern "C" int ops_cpu_lowdim_reduction(const ops_arg *a);
@@ -67,9 +71,10 @@ struct DatSt {
   int prolong, mg[3], st[3];
 };
 
-inline DatSt dat_prepare(const ops_arg &a, const int *start, int ndim) {
+inline DatSt dat_prepare(const ops_arg &a, const int *start, const int *range, int ndim) {
   DatSt s{};
-  if (a.argtype != OPS_ARG_DAT || !a.opt) return s;
+  if (a.argtype != OPS_ARG_DAT || a.dat == nullptr || a.stencil == nullptr) return s;
+  if (!a.opt && !ops_cpu_arg_in_bounds(ndim, range, &a)) return s;
   ops_dat d = a.dat;
   int soa = ops_cpu_soa();
   long esz = soa ? d->type_size : d->elem_size;
@@ -113,8 +118,8 @@ template <typename T> inline T *dat_at(const DatSt &s, const int *lc) {
 
 template <typename T> struct Param<ACC<T>> {
   using St = DatSt;
-  static St prepare(const ops_arg &a, const int *start, int ndim) {
-    return dat_prepare(a, start, ndim);
+  static St prepare(const ops_arg &a, const int *start, const int *range, int ndim) {
+    return dat_prepare(a, start, range, ndim);
   }
   static ACC<T> &get(const St &s, const ops_arg &, Ctx &c, const int *lc, const int *, int) {
     static_assert(sizeof(ACC<T>) <= sizeof(c.acc), "ACC view does not fit its slot");
@@ -134,8 +139,8 @@ template <typename T> struct Param<ACC<T>> {
 // OPS_ACC<n> macros; ops_seq.h:129-296 of the reference), gbl/reduce data and ops_arg_idx
 template <typename T> struct Param<T *> {
   using St = DatSt;
-  static St prepare(const ops_arg &a, const int *start, int ndim) {
-    return dat_prepare(a, start, ndim);
+  static St prepare(const ops_arg &a, const int *start, const int *range, int ndim) {
+    return dat_prepare(a, start, range, ndim);
   }
   static T *get(const St &s, const ops_arg &a, Ctx &c, const int *lc, const int *start,
                 int direct) {
@@ -171,7 +176,7 @@ void run(std::index_sequence<I...>, void (*kernel)(P...), ops_block block, int d
   (void)block;
   ops_cpu_check_bounds(dim, range, args, N);
   ops_cpu_before_loop(args, N);
-  auto st = std::make_tuple(Param<typename Strip<P>::type>::prepare(args[I], start, dim)...);
+  auto st = std::make_tuple(Param<typename Strip<P>::type>::prepare(args[I], start, range, dim)...);
   // sizes behind the OPS_ACC<n> macros of API v1 kernels (xdim<n> = size[0], ...)
   for (int i = 0; i < N && i < OPS_V1_MAXARGS; i++)
     if (args[i].argtype == OPS_ARG_DAT) {

@@ -33,8 +33,8 @@
 // API v1 accessor sizes, filled by the loop driver (ops_seq_v2.h) before each loop
 struct ops_v1_shape ops_v1;
 
-struct OPS_instance {
-  int soa = 0;
+// the host backend's state of an OPS session (OPS_instance holds OPS_diags and OPS_soa)
+struct ops_cpu_instance : OPS_instance {
   std::vector<ops_block> blocks;
   std::vector<ops_dat> dats;
   std::vector<ops_stencil> stencils;
@@ -44,7 +44,7 @@ struct OPS_instance {
   std::string dump_dir;
 };
 
-static OPS_instance *g = nullptr;
+static ops_cpu_instance *g = nullptr;
 static std::string g_err;
 static std::mt19937 g_rng;
 
@@ -62,7 +62,7 @@ static char *dup(const char *s) {
 }
 
 extern "C" const char *ops_get_last_error(void) { return g_err.c_str(); }
-extern "C" int ops_cpu_soa(void) { return g ? g->soa : 0; }
+extern "C" int ops_cpu_soa(void) { return g ? g->OPS_soa : 0; }
 // a dat argument that reduces into an edge dat along a collapsed dimension
 // (ops_dummy_singlenode.cpp:247-252)
 extern "C" int ops_cpu_lowdim_reduction(const ops_arg *a) {
@@ -81,7 +81,7 @@ template <typename T> static void fill_identity(ops_dat d, int acc) {
 
 extern "C" void ops_cpu_before_loop(ops_arg *args, int n) {
   for (int i = 0; i < n; i++)
-    if (args[i].argtype == OPS_ARG_DAT && args[i].opt && args[i].dat->locked_hd)
+    if (args[i].argtype == OPS_ARG_DAT && args[i].dat && args[i].dat->locked_hd)
       fail(OPS_RUNTIME_ERROR, "ops_par_loop on a dat locked by ops_dat_get_raw_pointer");
   // the first low-dim reduction of a kind since the dat was last written otherwise starts
   // from the reduction's identity over the whole allocation (ops_dummy_singlenode.cpp:248-345)
@@ -96,56 +96,71 @@ extern "C" void ops_cpu_before_loop(ops_arg *args, int n) {
     else fail(OPS_NOT_IMPLEMENTED, std::string("low-dim reduction into a dat of type ") + d->type);
   }
 }
-// Extent check of every dat argument (loop range plus stencil reach, per dimension);
-// an out-of-allocation access raises OPS_INVALID_ARGUMENT, as the device runtime does.
-extern "C" void ops_cpu_check_bounds(int dim, const int *range, const ops_arg *args, int n) {
-  for (int i = 0; i < n; i++) {
-    const ops_arg &a = args[i];
-    if (a.argtype != OPS_ARG_DAT || !a.opt || !a.dat || !a.stencil) continue;
-    const ops_stencil s = a.stencil;
-    for (int d = 0; d < dim && d < a.dat->block->dims; d++) {
-      const int st = d < s->dims ? s->stride[d] : 1;
-      const int m = (s->type != 0 && d < s->dims) ? s->mgrid_stride[d] : 1;
-      int lo_p = 0, hi_p = 0;
-      for (int p = 0; d < s->dims && p < s->points; p++) {
-        const int v = s->stencil[p * s->dims + d];
-        lo_p = (p == 0 || v < lo_p) ? v : lo_p;
-        hi_p = (p == 0 || v > hi_p) ? v : hi_p;
-      }
-      const long shift = -(long)a.dat->base[d] - a.dat->d_m[d];
-      long first = (st ? (long)range[2 * d] * st : 0) + lo_p + shift;
-      long last = (st ? (long)(range[2 * d + 1] - 1) * st : 0) + hi_p + shift;
-      if (s->type == 2) {  // restrict: index * m
-        first = (long)range[2 * d] * st * m + lo_p + shift;
-        last = (long)(range[2 * d + 1] - 1) * st * m + hi_p + shift;
-      } else if (s->type == 1) {  // prolong: index / m
-        first = (long)(range[2 * d] / m) * st + lo_p + shift;
-        last = (long)((range[2 * d + 1] - 1) / m) * st + hi_p + shift;
-      }
-      if (first < 0 || last >= a.dat->size[d])
-        fail(OPS_INVALID_ARGUMENT, std::string("ops_par_loop: dat ") + a.dat->name +
-                                       " addressed outside its allocation in dimension " +
-                                       std::to_string(d) + " (" + std::to_string(first) + ".." +
-                                       std::to_string(last) + " of size " +
-                                       std::to_string(a.dat->size[d]) + ", stencil " +
-                                       s->name + ")");
+// Extent of one dat argument (loop range plus stencil reach, per dimension): the message
+// for the first dimension that leaves the allocation goes to *why.
+static bool arg_in_bounds(int dim, const int *range, const ops_arg &a, std::string *why) {
+  if (a.argtype != OPS_ARG_DAT || !a.dat || !a.stencil) return true;
+  const ops_stencil s = a.stencil;
+  for (int d = 0; d < dim && d < a.dat->block->dims; d++) {
+    const int st = d < s->dims ? s->stride[d] : 1;
+    const int m = (s->type != 0 && d < s->dims) ? s->mgrid_stride[d] : 1;
+    int lo_p = 0, hi_p = 0;
+    for (int p = 0; d < s->dims && p < s->points; p++) {
+      const int v = s->stencil[p * s->dims + d];
+      lo_p = (p == 0 || v < lo_p) ? v : lo_p;
+      hi_p = (p == 0 || v > hi_p) ? v : hi_p;
+    }
+    const long shift = -(long)a.dat->base[d] - a.dat->d_m[d];
+    long first = (st ? (long)range[2 * d] * st : 0) + lo_p + shift;
+    long last = (st ? (long)(range[2 * d + 1] - 1) * st : 0) + hi_p + shift;
+    if (s->type == 2) {  // restrict: index * m
+      first = (long)range[2 * d] * st * m + lo_p + shift;
+      last = (long)(range[2 * d + 1] - 1) * st * m + hi_p + shift;
+    } else if (s->type == 1) {  // prolong: index / m
+      first = (long)(range[2 * d] / m) * st + lo_p + shift;
+      last = (long)((range[2 * d + 1] - 1) / m) * st + hi_p + shift;
+    }
+    if (first < 0 || last >= a.dat->size[d]) {
+      if (why)
+        *why = std::string("ops_par_loop: dat ") + a.dat->name +
+               " addressed outside its allocation in dimension " + std::to_string(d) + " (" +
+               std::to_string(first) + ".." + std::to_string(last) + " of size " +
+               std::to_string(a.dat->size[d]) + ", stencil " + s->name + ")";
+      return false;
     }
   }
+  return true;
+}
+extern "C" int ops_cpu_arg_in_bounds(int dim, const int *range, const ops_arg *a) {
+  return arg_in_bounds(dim, range, *a, nullptr) ? 1 : 0;
+}
+// Extent check of every active dat argument: an out-of-allocation access raises
+// OPS_INVALID_ARGUMENT, as the device runtime does.
+extern "C" void ops_cpu_check_bounds(int dim, const int *range, const ops_arg *args, int n) {
+  std::string why;
+  for (int i = 0; i < n; i++)
+    if (args[i].opt && !arg_in_bounds(dim, range, args[i], &why)) fail(OPS_INVALID_ARGUMENT, why);
 }
 extern "C" void ops_cpu_after_loop(ops_arg *args, int n) {
   for (int i = 0; i < n; i++)
-    if (args[i].argtype == OPS_ARG_DAT && args[i].opt && args[i].acc != OPS_READ) {
+    if (args[i].argtype == OPS_ARG_DAT && args[i].dat && args[i].acc != OPS_READ) {
+      // every dat argument, whatever its opt flag (ops_dummy_singlenode.cpp:140-148)
       args[i].dat->dirty_hd = 1;
-      if (args[i].dat->e_dat) args[i].dat->edat_acc = args[i].acc;  // ops_dummy_singlenode.cpp:204-213
+      if (args[i].opt && args[i].dat->e_dat) args[i].dat->edat_acc = args[i].acc;  // :204-213
     }
 }
 
 static void dump_all(const char *dir);
 
+OPS_instance *OPS_instance::getOPSInstance() {
+  if (!g) fail(OPS_RUNTIME_ERROR, "Error: OPS used before ops_init");
+  return g;
+}
+
 extern "C" void ops_init(const int argc, const char *const argv[], const int diags) {
-  (void)diags;
   if (g) ops_exit();
-  g = new OPS_instance();
+  g = new ops_cpu_instance();
+  g->OPS_diags = diags;
   const char *e = getenv("OPS_DUMP_DIR");
   if (e) g->dump_dir = e;
   for (int i = 1; i < argc; i++) {
@@ -190,7 +205,7 @@ extern "C" void ops_exit(void) {
   g = nullptr;
 }
 
-extern "C" void ops_set_soa(const int v) { g->soa = v; }
+extern "C" void ops_set_soa(const int v) { g->OPS_soa = v; }
 extern "C" void ops_partition(const char *) {}
 extern "C" int ops_is_root(void) { return 1; }
 extern "C" void ops_execute(OPS_instance *) {}
@@ -238,7 +253,7 @@ extern "C" ops_dat ops_decl_dat_char(ops_block block, int dim, int *bs, int *bas
   d->user_managed = 0;
   size_t off = 0, cum = 1;
   for (int n = 0; n < block->dims; n++) {
-    off += (size_t)(g->soa ? d->type_size : d->elem_size) * cum * (size_t)(-d->base[n] - d->d_m[n]);
+    off += (size_t)(g->OPS_soa ? d->type_size : d->elem_size) * cum * (size_t)(-d->base[n] - d->d_m[n]);
     cum *= (size_t)d->size[n];
   }
   d->base_offset = off;
@@ -539,7 +554,7 @@ extern "C" void ops_halo_transfer(ops_halo_group grp) {
                 for (int comp = 0; comp < d->dim; comp++) {
                   // SoA: component comp a whole allocation plane further on
                   // (ops_host_singlenode.cpp:124-160)
-                  char *dp = g->soa ? d->data + ((size_t)comp * cum + doff) * d->type_size
+                  char *dp = g->OPS_soa ? d->data + ((size_t)comp * cum + doff) * d->type_size
                                     : d->data + doff * d->elem_size + (size_t)comp * d->type_size;
                   char *bp = buf.data() + bo * (size_t)esz + (size_t)comp * (mixed ? st : d->type_size);
                   if (!mixed) {
@@ -644,7 +659,7 @@ static void slab(ops_dat d, char *buf, const int *range, int dir) {
               off += (size_t)(range[2 * a] + c[a] - d->d_m[a]) * cum;
               cum *= (size_t)d->size[a];
             }
-            if (g->soa && d->dim > 1) {
+            if (g->OPS_soa && d->dim > 1) {
               // component c of the point sits c planes further on; the buffer is packed
               // with components innermost (ops_util.cpp:378-400)
               size_t plane = 1;

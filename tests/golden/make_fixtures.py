@@ -149,7 +149,7 @@ def clover_big():
     clover("clover_bm16_short")
 
 
-def tea():
+def tea(*which):
     sys.path.insert(0, str(ROOT / "tests"))
     sys.path.insert(0, str(ROOT))
     from dumps import read_dump, read_reductions as rr
@@ -159,15 +159,30 @@ def tea():
         "tea_cg_200": tea_deck(200, 2),
         # apps/c/TeaLeaf/tea.in verbatim: Jacobi, 10x10, 10 steps (temp 157.55084183279294)
         "tea_jacobi_ref": Path("/root/reference/apps/c/TeaLeaf/tea.in").read_text(),
+        # the reference's other solvers (apps/c/TeaLeaf/tea_solve.cpp:200-300): Chebyshev
+        # after CG presteps, PPCG, and CG with the diagonal Jacobi preconditioner (its block
+        # Jacobi preconditioner stops with "block solvers are not supported")
+        "tea_cheby_64": tea_deck(64, 3, solver="tl_use_chebyshev"),
+        "tea_ppcg_64": tea_deck(64, 3, solver="tl_use_ppcg"),
+        "tea_cg_jacdiag_64": tea_deck(64, 3, extra=("tl_preconditioner_type=jac_diag",)),
     }
+    only = set(which or decks)
     for name, deck in decks.items():
+        if name not in only:
+            continue
         with tempfile.TemporaryDirectory() as t:
             out = run_ref("tealeaf_dev_seq", t, deck=deck, deck_name="tea.in")
             fx = _dump_fixture(read_dump(t), rr(t))
             text = (Path(t) / "tea.out").read_text()
-        iters = [int(l.split()[-1]) for l in text.splitlines() if "Iteration count" in l]
+        # CG, Jacobi and Chebyshev print "Iteration count <n>"; PPCG prints its CG presteps
+        # that way and then "PPCG Iteration count <n> (Total <m>)"
+        iters = [int(l.split()[-1].rstrip(")")) for l in text.splitlines()
+                 if "Iteration count" in l and "PPCG" not in l]
+        ppcg = [int(l.split()[3]) for l in text.splitlines() if "PPCG Iteration count" in l]
         fx.update({"source": "apps/c/TeaLeaf (reference dev_seq)", "deck": deck,
                    "iteration_counts_printed": iters, "tail": out.splitlines()[-4:]})
+        if ppcg:
+            fx["ppcg_iteration_counts_printed"] = ppcg
         (OUT / f"{name}.json").write_text(json.dumps(fx, indent=1))
         print("wrote", name, iters)
 
@@ -231,5 +246,7 @@ def clover3d(which=None):
 if __name__ == "__main__":
     which = sys.argv[1:] or ["laplace", "clover"]
     for w in which:
-        globals()[w]()
+        # "tea:tea_cheby_64,tea_ppcg_64" regenerates only the named fixtures of a group
+        group, _, names = w.partition(":")
+        globals()[group](*[n for n in names.split(",") if n])
         print("done", w)
