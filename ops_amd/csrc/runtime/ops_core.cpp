@@ -871,13 +871,15 @@ void ops_dump_all_dats(ops_runtime_instance *in, const char *dir) {
   for (ops_dat d : in->dats) {
     if (!d) continue;
     ops_get_data(d);
-    fprintf(fi, "%s %d %d %d", d->name, d->dim, d->type_size, d->block->dims);
+    std::string name(d->name);  // some applications pad names with blanks
+    while (!name.empty() && name.back() == ' ') name.pop_back();
+    fprintf(fi, "%s %d %d %d", name.c_str(), d->dim, d->type_size, d->block->dims);
     for (int i = 0; i < OPS_MAX_DIM; i++) fprintf(fi, " %d", d->size[i]);
     for (int i = 0; i < OPS_MAX_DIM; i++) fprintf(fi, " %d", d->d_m[i]);
     for (int i = 0; i < OPS_MAX_DIM; i++) fprintf(fi, " %d", d->d_p[i]);
     for (int i = 0; i < OPS_MAX_DIM; i++) fprintf(fi, " %d", d->base[i]);
     fprintf(fi, " %d %zu\n", d->x_pad, d->base_offset);
-    std::string fn = std::string(dir) + "/" + d->name + ".bin";
+    std::string fn = std::string(dir) + "/" + name + ".bin";
     FILE *fb = fopen(fn.c_str(), "wb");
     if (fb) {
       fwrite(d->data, 1, d->mem, fb);

@@ -5,7 +5,9 @@ them with the hip_gfx950 target and links them with libops_hip.so. Here:
   - (CPU, where the reference tree is present) all five translate, compile and link;
   - (GPU) the executables run the reference's decks, and every dat they dump at ops_exit
     equals the reference's own sequential build bit for bit (the committed dev_seq
-    fixtures), with the reduction handles' last values byte for byte. TeaLeaf's solves sum
+    fixtures), and the reduction handles' last values within 1e-12 (the GPU adds sums in
+    another order; integer, MIN and MAX results are exact). Laplace step7
+    evaluates sin and exp on the device: within 1e-13 of the reference. TeaLeaf's solves sum
     in another order on the GPU: its set-up dats are exact, iteration counts within one,
     and u, energy1 and the temperature within 1e-12 of the reference's sequential build
     (oracle/_ref/tealeaf_dev_seq, run on the same deck) — the reordering alone moves them
@@ -21,7 +23,7 @@ import numpy as np
 import pytest
 
 from dumps import read_dump, read_reductions
-from helpers import sha256
+from helpers import max_rel, sha256
 
 ROOT = Path(__file__).resolve().parents[1]
 DROP = ROOT / "ops_amd" / "_dropin"
@@ -67,10 +69,40 @@ def dump_mismatches(dump, fx):
     return bad
 
 
-def reduction_mismatches(dump, fx):
+def deck_cells(deck):
+    n = 1
+    for line in deck.splitlines():
+        t = line.replace("=", " ").split()
+        if len(t) >= 2 and t[0] in ("x_cells", "y_cells", "z_cells"):
+            n *= int(t[1])
+    return n
+
+
+def reduction_mismatches(dump, fx, cells=1, skip=()):
+    """Reduction handles whose last value differs from the reference's. Integer, MIN and
+    MAX results are order-free and must match exactly. Sums of doubles are added in
+    another order on the GPU: they are held to 1e-12 relative (the field_summary bar), or
+    to the rounding bound of the reference's own left-to-right sum over `cells` terms,
+    (cells - 1) * eps relative, where that is larger (its 921,600-cell volume sum of
+    clover_bm_short is 1.2e-11 below the exact 100 that the GPU's tree sum returns)."""
+    rel = max(1e-12, (cells - 1) * np.finfo(np.float64).eps)
     got = read_reductions(dump)
-    return [k for k, v in fx["reductions"].items()
-            if k not in got or got[k].tobytes().hex() != v["hex"]]
+    bad = []
+    for k, v in fx["reductions"].items():
+        if k in skip:
+            continue
+        if k not in got:
+            bad.append((k, "missing"))
+            continue
+        g = got[k]
+        r = np.frombuffer(bytes.fromhex(v["hex"]), dtype=g.dtype)
+        if g.dtype == np.float64:
+            ok = np.all(np.abs(g - r) <= rel * np.abs(r))
+        else:
+            ok = np.array_equal(g, r)
+        if not ok:
+            bad.append((k, g.tolist(), r.tolist()))
+    return bad
 
 
 # ---------------------------------------------------------------- build (CPU)
@@ -96,12 +128,29 @@ def test_makefile_lists_join_continuation_lines(tmp_path):
 # ---------------------------------------------------------------- GPU runs
 
 @pytest.mark.gpu
-def test_gpu_laplace_step7_equals_reference(golden, tmp_path):
+def test_gpu_laplace_step7_matches_reference(golden, tmp_path):
+    # The tutorial evaluates its boundary columns on the device, sin and exp of the loop
+    # index (apps/c/laplace2d_tutorial/step7/laplace_kernels.h:12-18): the GPU's math
+    # library and glibc may differ in the last bit there, so A and Anew are compared with
+    # the reference's sequential build at 1e-13 relative (the Laplace bar), and the
+    # reference run is checked bitwise against its committed fixture
     fx = golden("laplace_step7.json")
-    out, dump = run_gpu("laplace2d_step7", tmp_path)
+    gdir = tmp_path / "gpu"
+    gdir.mkdir()
+    out, dump = run_gpu("laplace2d_step7", gdir)
     assert "PASSED" in out, out[-500:]
-    assert dump_mismatches(dump, fx) == []
-    assert reduction_mismatches(dump, fx) == []
+    ref_exe = REF_BIN / "laplace2d_step7_dev_seq"
+    if not ref_exe.exists():
+        pytest.skip("reference dev_seq build absent: fields not compared")
+    rdir = tmp_path / "ref"
+    rdir.mkdir()
+    run(ref_exe, rdir, env={"OMP_NUM_THREADS": "1", "OPS_REF_DUMP_DIR": str(rdir)})
+    assert dump_mismatches(rdir, fx) == []
+    got, ref = read_dump(dump), read_dump(rdir)
+    for k in ("A", "Anew"):
+        assert max_rel(got[k]["data"], ref[k]["data"], floor=1e-300) < 1e-13, k
+    eg, er = read_reductions(dump)["error"][0], read_reductions(rdir)["error"][0]
+    assert abs(eg - er) <= 1e-13 * abs(er)
 
 
 @pytest.mark.gpu
@@ -111,7 +160,7 @@ def test_gpu_cloverleaf_equals_reference(golden, tmp_path, name):
     out, dump = run_gpu("cloverleaf", tmp_path, fx["deck"], "clover.in")
     assert (tmp_path / "clover.out").read_text().count(" Step ") == fx["steps"]
     assert dump_mismatches(dump, fx) == []
-    assert reduction_mismatches(dump, fx) == []
+    assert reduction_mismatches(dump, fx, deck_cells(fx["deck"])) == []
 
 
 @pytest.mark.gpu
@@ -123,7 +172,11 @@ def test_gpu_cloverleaf3d_equals_reference(golden, tmp_path, name):
     if name == "clover3d_bm":
         assert "This test is considered PASSED" in out, out[-500:]
     assert dump_mismatches(dump, fx) == []
-    assert reduction_mismatches(dump, fx) == []
+    # zl_pos is reduced into (OPS_INC) but never read with ops_reduction_result
+    # (apps/c/CloverLeaf_3D/calc_dt.cpp:89-100): its handle's value is not an API result
+    # (the reference's sequential path keeps the kernel's last assignment, a device backend
+    # keeps the running sum on the device), so it is not compared
+    assert reduction_mismatches(dump, fx, deck_cells(fx["deck"]), skip=("zl_pos",)) == []
 
 
 TEA_SETUP = ("density", "volume", "xarea", "yarea", "cellx", "celly", "celldx", "celldy",
