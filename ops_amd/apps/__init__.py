@@ -45,6 +45,7 @@ SIGNATURES = {
         "tealeaf_set_guess": (_i, [_i, _i]),
         "tealeaf_capture_next": (_i, [_i]),
         "tealeaf_guess_state": (_i, [_p]),
+        "tealeaf_set_start": (_i, [_p, ctypes.c_long]),
         "tea_unet_init": (_i, [ctypes.c_ulonglong]),
         "tea_unet_num_params": (ctypes.c_long, []),
         "tea_unet_get_params": (_i, [_p]),

@@ -83,9 +83,12 @@ def train(L, epochs, batch=4, lr=1e-3, seed=1, beta=0.0):
     return loss, time.perf_counter() - t0
 
 
-def solve_run(L, text, solver, steps, net_cells=0, fetch=(), mode=1):
+def solve_run(L, text, solver, steps, net_cells=0, fetch=(), mode=1, starts=None):
     """Run `steps` implicit steps of a deck, with the learned warm start when net_cells > 0
-    (mode 1: kept when its residual beats the standard start's, mode 2: always kept).
+    (mode 1: kept when its residual beats the standard start's, mode 2: always kept, mode 3:
+    always kept and the network's change also corrects the multigrid preconditioner, mode
+    4: that correction alone), or from given starting fields (starts(k) -> the field of
+    step k, halo included, in tealeaf_fetch's layout).
     Returns per-step iterations, standard / warm initial residuals, warm start kept, its
     Galerkin scaling and energy-norm drop, the final CG residual ratio, seconds, and the
     fields named in `fetch` (halo included) after the last step."""
@@ -99,7 +102,12 @@ def solve_run(L, text, solver, steps, net_cells=0, fetch=(), mode=1):
         out = {"iters": [], "initial": [], "warm": [], "kept": [], "alpha": [], "energy_drop": [], "rel": []}
         R.ops_hip_device_sync()
         t0 = time.perf_counter()
-        for _ in range(steps):
+        cnt = L.tealeaf_fetch(b"u", None)
+        for k in range(steps):
+            if starts is not None:
+                f = np.ascontiguousarray(starts(k), dtype=np.float64)
+                if L.tealeaf_set_start(f.ctypes.data_as(ctypes.c_void_p), cnt) != 0:
+                    raise RuntimeError("tealeaf_set_start failed")
             if L.tealeaf_step(1) != 0:
                 raise RuntimeError("tealeaf_step failed")
             L.tealeaf_state(st.ctypes.data_as(ctypes.c_void_p))
@@ -122,4 +130,46 @@ def solve_run(L, text, solver, steps, net_cells=0, fetch=(), mode=1):
         L.tealeaf_set_guess(0, 0)
         L.tealeaf_finalize()
         t.cleanup()
+    return out
+
+
+def best_case(L, text, solver, steps, errors=(0.1, 0.01, 0.001), seed=1):
+    """What any warm start could save: the solve of each step starts from its converged
+    solution (from a plain run of the same deck) with a smooth error of relative size e
+    (in the 2-norm of the step's change u - u0: a sum of six random low-frequency sine
+    products, the kind of error a network's smooth prediction leaves), for each e in
+    `errors`. Returns {"plain": per-step iterations, e: per-step iterations}."""
+    base = solve_run(L, text, solver, steps, fetch=())
+    us, u0s = [], []
+    # the converged u and the u0 of every step, from a second plain run that fetches both
+    import ops_amd
+    t = _run_deck(L, text, solver)
+    try:
+        cnt = L.tealeaf_fetch(b"u", None)
+        for _ in range(steps):
+            if L.tealeaf_step(1) != 0:
+                raise RuntimeError("tealeaf_step failed")
+            u, v = np.zeros(cnt), np.zeros(cnt)
+            L.tealeaf_fetch(b"u", u.ctypes.data_as(ctypes.c_void_p))
+            L.tealeaf_fetch(b"u0", v.ctypes.data_as(ctypes.c_void_p))
+            us.append(u)
+            u0s.append(v)
+    finally:
+        L.tealeaf_finalize()
+        t.cleanup()
+    m = int(round(np.sqrt(cnt)))
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(np.arange(m), np.arange(m))
+    out = {"plain": base["iters"]}
+    for e in errors:
+        def starts(k):
+            f = np.zeros((m, m))
+            for _ in range(6):
+                kx, ky = rng.integers(1, 6, 2)
+                ph = rng.uniform(0.0, 2.0 * np.pi, 2)
+                f += np.sin(np.pi * kx * X / m + ph[0]) * np.sin(np.pi * ky * Y / m + ph[1])
+            d = us[k] - u0s[k]
+            f = f.ravel() * (np.linalg.norm(d) / np.linalg.norm(f))
+            return us[k] + e * f
+        out[e] = solve_run(L, text, solver, steps, starts=starts)["iters"]
     return out

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "ops_hip_kernel.h"
@@ -60,7 +61,8 @@ struct Level {
 
 struct Rec {
   double rz, rzp, pw, alpha, beta, rr, err, thresh;
-  int stopped, iters, pad[14];
+  double dinv, dgam;  // correction direction: 1 / d^T A d, and its coefficient d^T r / d^T A d
+  int stopped, iters, pad[10];
 };
 
 __device__ __forceinline__ unsigned lin_block() {
@@ -335,6 +337,61 @@ __global__ __launch_bounds__(FT) void k_fold_rr(const double *part, unsigned n, 
   }
 }
 
+// ---------------------------------------------------------------- preconditioner correction
+// z = V(r) + d (d^T r) / (d^T A d) with d the given direction less its mean: the V-cycle
+// plus the A-orthogonal projection onto d, a symmetric positive semi-definite term, so the
+// preconditioner stays symmetric positive definite (a rank-one coarse space).
+
+// (d - m)^T A (d - m) partials (once per solve)
+__global__ __launch_bounds__(NT) void k_defl_dad(const double *__restrict__ Kx, const double *__restrict__ Ky,
+                                                 const double *__restrict__ d, double m, Fine g, double *part) {
+  long i;
+  double v = 0.0;
+  if (fcell(g, i)) {
+    const int s = g.s, n = g.n;
+    const int x = (int)(i % s), y = (int)(i / s);
+    // d outside the mesh is zero (not d - m): the operator's boundary faces see x = 0
+    auto dv = [&](long j, bool in) { return in ? d[j] - m : 0.0; };
+    const double dc = d[i] - m;
+    const double de = dv(i + 1, x < n), dw = dv(i - 1, x > 1), dn = dv(i + s, y < n), ds = dv(i - s, y > 1);
+    const double kx0 = Kx[i], kx1 = Kx[i + 1], ky0 = Ky[i], ky1 = Ky[i + s];
+    const double ad = (1.0 + g.ry * (ky1 + ky0) + g.rx * (kx1 + kx0)) * dc - g.ry * (ky1 * dn + ky0 * ds) -
+                      g.rx * (kx1 * de + kx0 * dw);
+    v = dc * ad;
+  }
+  write_partial(v, part);
+}
+
+__global__ __launch_bounds__(FT) void k_fold_dad(const double *part, unsigned n, Rec *S) {
+  const double dad = fold(part, n);
+  if (threadIdx.x == 0) S->dinv = dad > 0.0 ? 1.0 / dad : 0.0;
+}
+
+// (d - m)^T r partials
+__global__ __launch_bounds__(NT) void k_defl_dot(const double *__restrict__ d, double m,
+                                                 const double *__restrict__ r, Fine g, double *part,
+                                                 const Rec *S) {
+  if (S->stopped) return;
+  long i;
+  double v = 0.0;
+  if (fcell(g, i)) v = (d[i] - m) * r[i];
+  write_partial(v, part);
+}
+
+__global__ __launch_bounds__(FT) void k_fold_defl(const double *part, unsigned n, Rec *S) {
+  if (S->stopped) return;
+  const double dr = fold(part, n);
+  if (threadIdx.x == 0) S->dgam = dr * S->dinv;
+}
+
+// z += dgam (d - m)
+__global__ __launch_bounds__(NT) void k_defl_add(const double *__restrict__ d, double m, vreal *__restrict__ z,
+                                                 Fine g, const Rec *S) {
+  if (S->stopped) return;
+  long i;
+  if (fcell(g, i)) z[i] = (vreal)((double)z[i] + S->dgam * (d[i] - m));
+}
+
 // ---------------------------------------------------------------- host side
 
 struct MG {
@@ -345,7 +402,7 @@ struct MG {
   Rec *rec = nullptr, *host = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int graph_n = -1;
-  const void *graph_key[8] = {};
+  const void *graph_key[10] = {};
 };
 MG G;
 
@@ -458,6 +515,12 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
   const double thresh = P->eps * P->initial_residual;
   hipLaunchKernelGGL(k_start, gf, blk, 0, st, r64, G.lv[0].b, g, part);
   hipLaunchKernelGGL(k_fold_start, dim3(1), dim3(FT), 0, st, part, nb, G.rec, thresh);
+  const double *defl = P->deflation;
+  const double dmean = P->deflation_mean;
+  if (defl) {
+    hipLaunchKernelGGL(k_defl_dad, gf, blk, 0, st, kx, ky, defl, dmean, g, part);
+    hipLaunchKernelGGL(k_fold_dad, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
+  }
   ops_hip_check_launch("tealeaf multigrid set-up");
   // one PCG iteration with z in zbuf[par], the previous z in zbuf[1-par], p written to
   // the buffer of parity par
@@ -466,6 +529,11 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
     const double *pin = par ? pa : pb;
     double *pout = par ? pb : pa;
     vcycle(st, z);
+    if (defl) {
+      hipLaunchKernelGGL(k_defl_dot, gf, blk, 0, st, defl, dmean, r64, g, part, G.rec);
+      hipLaunchKernelGGL(k_fold_defl, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
+      hipLaunchKernelGGL(k_defl_add, gf, blk, 0, st, defl, dmean, z, g, G.rec);
+    }
     hipLaunchKernelGGL(k_dots, gf, blk, 0, st, r64, z, zp, g, part, G.rec);
     hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
     hipLaunchKernelGGL(k_pw, gf, blk, 0, st, kx, ky, z, pin, pout, w64, g, part, G.rec);
@@ -474,9 +542,14 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
     hipLaunchKernelGGL(k_fold_rr, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
   };
   // capture the two parities as graphs (rebuilt when the arrays move)
-  const void *key[8] = {G.mem, u64, r64, pa, w64, kx, ky, (const void *)(long)n};
+  // the captured iteration bakes in the arrays, the size and the correction direction with
+  // its mean
+  double dm_bits = dmean;
+  long dm_key;
+  memcpy(&dm_key, &dm_bits, sizeof dm_key);
+  const void *key[10] = {G.mem, u64, r64, pa, w64, kx, ky, (const void *)(long)n, defl, (const void *)dm_key};
   bool same = G.graph[0] != nullptr;
-  for (int k = 0; k < 8 && same; k++) same = key[k] == G.graph_key[k];
+  for (int k = 0; k < 10 && same; k++) same = key[k] == G.graph_key[k];
   // OPS_TEA_GRAPH=0: launch every iteration's kernels eagerly (the same kernels and order;
   // for tools that do not follow graph launches)
   const char *ge = getenv("OPS_TEA_GRAPH");
@@ -491,7 +564,7 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
       CK(hipGraphInstantiate(&G.graph[par], gr, nullptr, nullptr, 0));
       CK(hipGraphDestroy(gr));
     }
-    for (int k = 0; k < 8; k++) G.graph_key[k] = key[k];
+    for (int k = 0; k < 10; k++) G.graph_key[k] = key[k];
   }
   // iterations 1, 2, ...: odd iterations use parity 1 (p from pa into pb)
   int issued = 0;

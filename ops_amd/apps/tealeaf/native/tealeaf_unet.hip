@@ -1106,6 +1106,17 @@ void tea_unet_guess_dats(ops_dat rho, ops_dat u0, ops_dat r, ops_dat Kx, ops_dat
   }
 }
 
+const double *tea_unet_direction(double *mean) {
+  if (SS.pf == nullptr || SS.stats == nullptr)
+    throw OPSException(OPS_INVALID_ARGUMENT, "tea_unet_direction: no warm start evaluated");
+  if (mean) {
+    double h[6];
+    CK(hipMemcpy(h, SS.stats, sizeof h, hipMemcpyDeviceToHost));
+    *mean = h[4];
+  }
+  return SS.pf;
+}
+
 extern "C" int tea_unet_init(unsigned long long seed) {
   return guarded("tea_unet_init", [&] {
     if (!N.live) {

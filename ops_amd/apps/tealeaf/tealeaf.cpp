@@ -48,6 +48,7 @@ struct Tea {
   int solver = 0;
   int guess = 0, guess_cells = 0;  // learned warm start mode / network size
   int capture = -1;                // dataset sample the next solve is stored as, -1 none
+  std::vector<double> start;       // the next solve's starting field (halo included), if set
   int step = 0;
   double time = 0.0, dt = 0.0;
   // last solve
@@ -394,7 +395,19 @@ void solve() {
   T.last_warm = -1;
   T.last_guess_used = 0;
   if (T.capture >= 0) tea_unet_capture_features(T.capture, T.density, T.u0, T.r, grid.x_cells);
-  if (T.guess) {
+  if (!T.start.empty()) {
+    // a given starting field (tealeaf_set_start); the stop threshold keeps the residual of
+    // the standard start u = u0, as for the learned warm start
+    ops_dat d = T.u;
+    int range[4] = {d->d_m[0], d->size[0] + d->d_m[0], d->d_m[1], d->size[1] + d->d_m[1]};
+    ops_dat_set_data_slab_memspace(d, 0, (char *)T.start.data(), range, OPS_HOST);
+    T.start.clear();
+    set_fields(f, {FIELD_U});
+    update_halo(f, 1);
+    calc_residual(rx, ry);
+    T.last_warm = sqrt(fabs(norm2(T.r)));
+    T.last_guess_used = 1;
+  } else if (T.guess) {
     // learned warm start: u = u0 + alpha * the network's predicted change, alpha the Galerkin
     // scaling (energy-norm optimal along that direction). The stop threshold keeps
     // the residual of the standard start u = u0. Mode 1 keeps the warm start only when its
@@ -405,7 +418,7 @@ void solve() {
     update_halo(f, 1);
     calc_residual(rx, ry);
     T.last_warm = sqrt(fabs(norm2(T.r)));
-    if (T.guess == 2 || T.last_warm < initial) {
+    if (T.guess == 2 || T.guess == 3 || (T.guess == 1 && T.last_warm < initial)) {
       T.last_guess_used = 1;
     } else {
       ops_par_loop(tea_copy, "tea_leaf_warm_start_reset_kernel", T.blk, 2, in,
@@ -433,6 +446,11 @@ void solve() {
     P.eps = T.eps;
     P.max_iters = T.max_iters;
     P.initial_residual = initial;
+    P.deflation = nullptr;
+    P.deflation_mean = 0.0;
+    // modes 3 and 4: the network's change also corrects the multigrid preconditioner
+    if (T.guess >= 3 && (T.solver == 3 || T.solver == 4))
+      P.deflation = tea_unet_direction(&P.deflation_mean);
     tea_native_result R;
     if (T.solver == 3)
       tea_native_mgcg(T.u, T.u0, T.r, T.p, T.w, T.Kx, T.Ky, &P, &R);
@@ -597,13 +615,25 @@ extern "C" int tealeaf_set_solver(int solver) {
   return 0;
 }
 
-// learned warm start for the implicit solve: modes 1 and 2 evaluate the network
-// (tea_unet_*) on a net_cells^2 view of the mesh before each solve (1: kept only when its
-// residual beats the standard start's, 2: always kept), mode 0 turns it off
+// learned warm start for the implicit solve: modes 1-4 evaluate the network (tea_unet_*)
+// on a net_cells^2 view of the mesh before each solve. 1: its warm start kept only when its
+// residual beats the standard start's; 2: always kept; 3: always kept, and the network's
+// change also corrects the multigrid preconditioner (solvers 3 and 4: z = V(r) + the
+// A-orthogonal projection of r's error onto that direction); 4: the correction alone, from
+// the standard start. Mode 0 turns it off.
 extern "C" int tealeaf_set_guess(int mode, int net_cells) {
-  if (mode < 0 || mode > 2 || (mode && net_cells <= 0)) return 1;
+  if (mode < 0 || mode > 4 || (mode && net_cells <= 0)) return 1;
   T.guess = mode;
   T.guess_cells = net_cells;
+  return 0;
+}
+
+// the next solve starts from `field` (u with its halo, size[0] * size[1] values, the layout
+// of tealeaf_fetch) instead of u = u0; its stop threshold stays that of u = u0. Used to
+// measure how the iteration count depends on the accuracy of a starting guess.
+extern "C" int tealeaf_set_start(const double *field, long count) {
+  if (field == nullptr || T.u == nullptr || count != (long)T.u->size[0] * T.u->size[1]) return 1;
+  T.start.assign(field, field + count);
   return 0;
 }
 

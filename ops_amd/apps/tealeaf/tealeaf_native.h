@@ -27,6 +27,11 @@ typedef struct {
   double rx, ry, eps;
   int max_iters;
   double initial_residual;
+  // multigrid PCG only: a correction direction d (device, (nx + 2)^2 with a zero halo, its
+  // mean `deflation_mean` removed on use) added to the preconditioner,
+  // z = V(r) + d (d^T r) / (d^T A d); nullptr for none
+  const double *deflation;
+  double deflation_mean;
 } tea_native_solve_params;
 
 typedef struct {
@@ -60,5 +65,8 @@ void tea_unet_capture_features(int index, ops_dat rho, ops_dat u0, ops_dat r, in
 void tea_unet_capture_target(int index, ops_dat u0, ops_dat u, ops_dat Kx, ops_dat Ky, double rx, double ry, int n);
 void tea_unet_guess_dats(ops_dat rho, ops_dat u0, ops_dat r, ops_dat Kx, ops_dat Ky, double rx, double ry, ops_dat u,
                          int n, int net_cells, double *out2);
+// the network's change p of the last tea_unet_guess_dats on the mesh (device, (n + 2)^2,
+// zero halo) and its mean
+const double *tea_unet_direction(double *mean);
 
 #endif

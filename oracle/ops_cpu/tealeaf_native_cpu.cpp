@@ -359,6 +359,9 @@ void tea_unet_capture_target(int, ops_dat, ops_dat, ops_dat, ops_dat, double, do
 void tea_unet_guess_dats(ops_dat, ops_dat, ops_dat, ops_dat, ops_dat, double, double, ops_dat, int, int, double *) {
   throw OPSException(OPS_NOT_IMPLEMENTED, "tealeaf learned warm start: device build only");
 }
+const double *tea_unet_direction(double *) {
+  throw OPSException(OPS_NOT_IMPLEMENTED, "tealeaf learned warm start: device build only");
+}
 
 // solver 4 (the FP64 V-cycle) is a device measurement of what FP32 storage buys; the host
 // backend restates solver 3 only

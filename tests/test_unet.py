@@ -180,3 +180,21 @@ def test_warm_started_solves_reach_the_same_field(trained, solver):
         assert k == (1 if w < i else 0)
     assert all(r < 1e-10 for r in warm["rel"]) and all(r < 1e-10 for r in plain["rel"])
     assert np.max(np.abs(warm["u"] - plain["u"])) <= 1e-8 * np.max(np.abs(plain["u"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [3, 4])
+def test_corrected_preconditioner_reaches_the_same_field(trained, mode):
+    """The network's change as a rank-one A-orthogonal correction of the multigrid
+    preconditioner, with the warm start (mode 3) or from the plain start (mode 4): the
+    multigrid PCG still stops on the plain start's threshold and ends on the field of its
+    plain run."""
+    from ops_amd.apps.tealeaf import learned
+    from ops_amd.apps.tealeaf.decks import tea_deck
+    L, _ = trained
+    deck = tea_deck(SMALL, 3, seed=HELD_OUT + 1, eps=1e-10, dt=SMALL_DT, max_iters=20000)
+    plain = learned.solve_run(L, deck, 3, 3, fetch=("u",))
+    corr = learned.solve_run(L, deck, 3, 3, net_cells=SMALL, fetch=("u",), mode=mode)
+    assert all(k == (1 if mode == 3 else 0) for k in corr["kept"])
+    assert all(r < 1e-10 for r in corr["rel"])
+    assert np.max(np.abs(corr["u"] - plain["u"])) <= 1e-8 * np.max(np.abs(plain["u"]))
