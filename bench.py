@@ -290,10 +290,12 @@ def clover_gpu(args, R):
         if args.warmup > 0:
             log(f"[bench] warmup: {args.warmup} steps")
             assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+            L.cloverleaf_set_chain(args.chain)
             assert L.cloverleaf_step(args.warmup) == 0
             assert L.cloverleaf_finalize() == 0
         log("[bench] generating the deck on the device")
         assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+        L.cloverleaf_set_chain(args.chain)
         # measured HBM bandwidth: a plain 1 GiB device copy (also the counter calibration)
         checks_copy = R.ops_hip_copy_probe(1 << 30, 10)
         graph = not args.no_graph
@@ -322,6 +324,7 @@ def clover_gpu(args, R):
             # the same steps: the replayed graphs run the same kernels
             log(f"[bench] eager pass with per-kernel timing: {args.steps} steps")
             assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+            L.cloverleaf_set_chain(args.chain)
             R.ops_hip_device_sync()
             R.ops_hip_reset_stats()
             R.ops_hip_set_timing(1)
@@ -333,7 +336,8 @@ def clover_gpu(args, R):
             assert L.cloverleaf_finalize() == 0
         stats = ops_amd.kernel_stats()
     secs = t1 - t0
-    checks = {"ke": s[3], "hbm_copy_GBps": round(checks_copy, 1), "captured_steps": graph}
+    checks = {"ke": s[3], "hbm_copy_GBps": round(checks_copy, 1), "captured_steps": graph,
+              "loop_chain_bands": args.chain}
     if eager_s is not None:
         checks["eager_ms_per_step_with_kernel_timing"] = round(eager_s * 1e3 / args.steps, 3)
     if args.steps == 87 and n == 3840:
@@ -707,6 +711,9 @@ def main():
                     help="keep the runtime's default workgroup shapes (no OPS_AUTOTUNE)")
     ap.add_argument("--workload", default="cloverleaf", choices=["cloverleaf"])
     ap.add_argument("--cells", type=int, default=3840)
+    ap.add_argument("--chain", type=int, default=0,
+                    help="CloverLeaf: run the loops between halo exchanges as loop chains of "
+                         "this many bands (0: one launch per loop)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10,
                     help="CloverLeaf 2D steps per timed CPU-baseline run (median of 3)")

@@ -60,6 +60,10 @@ typedef struct ops_hip_loop {
   double bytes;  /* ops_compute_transfer bytes of this launch */
   void (*batch)(void); /* set by the stub: the launcher of its pending loop batch (NULL
                           when the loop cannot join one) */
+  /* set by the stub: calls the stub again for a sub-range (a loop chain runs a recorded
+     loop band by band through it); NULL when the loop cannot be recorded */
+  void (*relaunch)(const char *name, ops_block block, int dim, int *range, ops_arg *args);
+  int deferred;  /* set by ops_hip_loop_begin: 1 when the loop was recorded in a chain */
   /* loops with reductions: the launch is a capped 1-D grid of nblocks workgroups that walk
      the `tiles` tiles of the tile grid tgrid[0] x tgrid[1] x tgrid[2] grid-stride */
   unsigned tiles;
@@ -128,6 +132,17 @@ void ops_hip_batch_end(void);
    stub's pending batch (it also identifies the batch: a launch of another stub launches
    the pending one first), and is called by the runtime before any other operation */
 int ops_hip_batch_join(int kernel_id, void (*flush)(void));
+/* Loop chains, the GPU form of the reference's lazy loop tiling (ops/c/src/core/
+   ops_lazy.cpp:186-1401; there the queue and its tiling are driven by OPS_TILING). Between
+   ops_hip_chain_begin(tiles) and ops_hip_chain_end the loops that can be tiled (no
+   reductions, plain stencils with unit stride along the last loop dimension, two or three
+   dimensions) are recorded; the chain then runs band by band along the last dimension,
+   `tiles` bands with per-loop skewed boundaries derived from the loops' stencils, so that
+   data one loop writes is re-read from the caches by the next. No promise is asked of the
+   caller: the dependencies are computed, results are bitwise those of the loops run in
+   order, and any other runtime operation runs the recorded chain first. */
+void ops_hip_chain_begin(int tiles);
+void ops_hip_chain_end(void);
 int ops_hip_graph_begin(void);
 void *ops_hip_graph_end(void);
 int ops_hip_graph_replay(void *graph, int times);

@@ -15,6 +15,7 @@ SIGNATURES = {
         "cloverleaf_init": (_i, [_i, _pp, ctypes.c_char_p]),
         "cloverleaf_step": (_i, [_i]),
         "cloverleaf_set_graph": (_i, [_i]),
+        "cloverleaf_set_chain": (_i, [_i]),
         "cloverleaf_capture": (_i, []),
         "cloverleaf_run": (_i, [_i]),
         "cloverleaf_summary": (_i, [_p]),
@@ -66,6 +67,7 @@ SIGNATURES = {
         "api_random": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
         "api_random_normal": (_i, [_i, _pp, ctypes.c_uint, _i, _i, _p]),
         "api_batch": (_i, [_i, _pp, _i, _i, _p]),
+        "api_chain": (_i, [_i, _pp, _i, _i, _i, _i, _p]),
         "api_tridiag2": (_i, [_i, _pp, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p]),
     },
     "multicomp": {

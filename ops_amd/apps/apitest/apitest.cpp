@@ -13,6 +13,7 @@
 //                  its halos
 //   api_tridiag2   two solves in a row (the second on a smaller box) through the queueing
 //                  entry points or the allocation-free one
+//   api_chain      loop chains (ops_hip_chain_begin/end): dependent loops run band by band
 //   api_batch      loop batches (ops_hip_batch_begin/end): many independent launches of one
 //                  kernel of mixed extents, two stubs of that kernel interleaved, a
 //                  tridiagonal solve between launches, eagerly and replayed from a graph
@@ -441,6 +442,73 @@ extern "C" int api_batch(int argc, const char **argv, int nloops, int mode, doub
     return 0;
   } catch (std::exception &e) {
     return fail("api_batch", e);
+  }
+}
+
+// Loop chains. Dats A, B, C, D of nx x ny cells with two halo layers: A and B set from
+// their indices, then five loops (api_ch_l1..l5) over different row ranges that read along
+// y up to two rows away what earlier loops wrote (read after write) and overwrite what
+// earlier loops read two rows away (write after read), once more for C (write after
+// write). mode bit 0: the five loops run between ops_hip_chain_begin(tiles) and
+// ops_hip_chain_end; bit 1: a sum reduction of D runs between l3 and l4 (a loop a chain
+// cannot hold: the chain runs first, recording resumes after it); bit 2: the weight l1
+// reads through ops_arg_gbl is changed by the caller after the call (a recorded loop keeps
+// the value of its call). out: A, B, C, D interiors (x fastest), then the sum.
+extern "C" int api_chain(int argc, const char **argv, int nx, int ny, int tiles, int mode,
+                         double *out) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block blk = ops_decl_block(2, "chain");
+    int size[2] = {nx, ny}, base[2] = {0, 0}, dm[2] = {-2, -2}, dp[2] = {2, 2};
+    double *none = nullptr;
+    ops_dat A = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "A");
+    ops_dat B = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "B");
+    ops_dat C = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "C");
+    ops_dat D = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "D");
+    int s00[] = {0, 0}, s_y1[] = {0, 0, 0, -1, 0, 1, 1, 0}, s_y2[] = {0, 0, 0, 2, -1, 0};
+    int s_a2[] = {0, -2, 0, 1}, s_c1[] = {0, -1}, s_b2[] = {0, 2, 0, -2}, s_d1[] = {0, 1, 0, -1};
+    ops_stencil S00 = ops_decl_stencil(2, 1, s00, "00");
+    ops_stencil SY1 = ops_decl_stencil(2, 4, s_y1, "a_l1");
+    ops_stencil SY2 = ops_decl_stencil(2, 3, s_y2, "c_l2");
+    ops_stencil SA2 = ops_decl_stencil(2, 2, s_a2, "a_l3");
+    ops_stencil SC1 = ops_decl_stencil(2, 1, s_c1, "c_l3");
+    ops_stencil SB2 = ops_decl_stencil(2, 2, s_b2, "b_l5");
+    ops_stencil SD1 = ops_decl_stencil(2, 2, s_d1, "d_l4");
+    ops_reduction sum = ops_decl_reduction_handle(sizeof(double), "double", "sum");
+    int full[4] = {-2, nx + 2, -2, ny + 2};
+    ops_par_loop(api_ch_init, "api_ch_init", blk, 2, full, ops_arg_dat(A, 1, S00, "double", OPS_WRITE),
+                 ops_arg_dat(B, 1, S00, "double", OPS_WRITE), ops_arg_dat(C, 1, S00, "double", OPS_WRITE),
+                 ops_arg_dat(D, 1, S00, "double", OPS_WRITE), ops_arg_idx());
+    double w = 2.0, total = 0.0;
+    if (mode & 1) ops_hip_chain_begin(tiles);
+    int r1[4] = {0, nx, 0, ny}, r2[4] = {0, nx, 1, ny}, r3[4] = {0, nx, 0, ny - 1};
+    int r4[4] = {0, nx, 2, ny}, r5[4] = {0, nx, 0, ny};
+    ops_par_loop(api_ch_l1, "api_ch_l1", blk, 2, r1, ops_arg_dat(C, 1, S00, "double", OPS_WRITE),
+                 ops_arg_dat(A, 1, SY1, "double", OPS_READ), ops_arg_dat(B, 1, S00, "double", OPS_READ),
+                 ops_arg_gbl(&w, 1, "double", OPS_READ));
+    if (mode & 4) w = -7.0;
+    ops_par_loop(api_ch_l2, "api_ch_l2", blk, 2, r2, ops_arg_dat(A, 1, S00, "double", OPS_WRITE),
+                 ops_arg_dat(C, 1, SY2, "double", OPS_READ));
+    ops_par_loop(api_ch_l3, "api_ch_l3", blk, 2, r3, ops_arg_dat(D, 1, S00, "double", OPS_WRITE),
+                 ops_arg_dat(A, 1, SA2, "double", OPS_READ), ops_arg_dat(C, 1, SC1, "double", OPS_READ));
+    if (mode & 2)
+      ops_par_loop(api_ch_sum, "api_ch_sum", blk, 2, r1, ops_arg_dat(D, 1, S00, "double", OPS_READ),
+                   ops_arg_reduce(sum, 1, "double", OPS_INC));
+    ops_par_loop(api_ch_l4, "api_ch_l4", blk, 2, r4, ops_arg_dat(B, 1, S00, "double", OPS_RW),
+                 ops_arg_dat(D, 1, SD1, "double", OPS_READ));
+    ops_par_loop(api_ch_l5, "api_ch_l5", blk, 2, r5, ops_arg_dat(C, 1, S00, "double", OPS_WRITE),
+                 ops_arg_dat(D, 1, S00, "double", OPS_READ), ops_arg_dat(B, 1, SB2, "double", OPS_READ));
+    if (mode & 1) ops_hip_chain_end();
+    if (mode & 2) ops_reduction_result(sum, &total);
+    int in[4] = {0, nx, 0, ny};
+    const size_t m = (size_t)nx * ny;
+    ops_dat all[4] = {A, B, C, D};
+    for (int k = 0; k < 4; k++) ops_dat_fetch_data_slab_memspace(all[k], 0, (char *)(out + k * m), in, OPS_HOST);
+    out[4 * m] = total;
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_chain", e);
   }
 }
 

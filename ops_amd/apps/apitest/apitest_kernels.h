@@ -31,4 +31,26 @@ void api_bset(ACC<double> &o, const ACC<double> &i, const int *idx, const double
   o(0, 0) = c[0] * i(0, 0) + (double)idx[0] + 1000.0 * (double)idx[1];
 }
 
+// loop chains (api_chain): loops that read, along y (the tiled dimension) and x, what the
+// loops before them wrote, and overwrite what earlier loops read
+void api_ch_init(ACC<double> &a, ACC<double> &b, ACC<double> &c, ACC<double> &d, const int *idx) {
+  const double x = (double)idx[0], y = (double)idx[1];
+  a(0, 0) = 1.0 + 0.01 * x - 0.003 * y * y + 0.25 * ((idx[0] + 3 * idx[1]) % 7);
+  b(0, 0) = 0.5 - 0.02 * x * y + 0.125 * ((2 * idx[0] + idx[1]) % 5);
+  c(0, 0) = 0.0;
+  d(0, 0) = 0.0;
+}
+void api_ch_l1(ACC<double> &c, const ACC<double> &a, const ACC<double> &b, const double *w) {
+  c(0, 0) = a(0, -1) + w[0] * a(0, 1) + b(0, 0) + 0.25 * a(1, 0);
+}
+void api_ch_l2(ACC<double> &a, const ACC<double> &c) { a(0, 0) = 0.5 * c(0, 0) + c(0, 2) - 0.125 * c(-1, 0); }
+void api_ch_l3(ACC<double> &d, const ACC<double> &a, const ACC<double> &c) {
+  d(0, 0) = a(0, -2) - a(0, 1) + c(0, -1);
+}
+void api_ch_sum(const ACC<double> &d, double *s) { *s = *s + d(0, 0); }
+void api_ch_l4(ACC<double> &b, const ACC<double> &d) { b(0, 0) = b(0, 0) + d(0, 1) - d(0, -1); }
+void api_ch_l5(ACC<double> &c, const ACC<double> &d, const ACC<double> &b) {
+  c(0, 0) = d(0, 0) + b(0, 2) - b(0, -2);
+}
+
 #endif

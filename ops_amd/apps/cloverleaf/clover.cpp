@@ -14,6 +14,7 @@
 //   cloverleaf_init(argc, argv, deck)   parse the deck, build and generate the problem
 //   cloverleaf_step(n)                  advance n hydro steps
 //   cloverleaf_set_graph(on)            replay steps from captured hipGraphs (dt on the GPU)
+//   cloverleaf_set_chain(tiles)         run the loops between halo exchanges as loop chains
 //   cloverleaf_summary(out[5])          field summary: volume, mass, ie, ke, pressure
 //   cloverleaf_state(out[4])            step, time, dt, end_step
 //   cloverleaf_finalize()
@@ -61,6 +62,8 @@ void *ops_hip_reduction_device_ptr(ops_reduction h);
 void ops_hip_reduction_discard(ops_reduction h);
 void ops_hip_batch_begin(void);
 void ops_hip_batch_end(void);
+void ops_hip_chain_begin(int tiles);
+void ops_hip_chain_end(void);
 }
 
 namespace {
@@ -76,6 +79,9 @@ struct Sim {
   // captured steps: one graph per sweep order (advect_x), the device dt state
   int graph = 0;
   void *step_graph[2] = {nullptr, nullptr};
+  // loop chains: the loops between two halo exchanges run band by band in `chain` bands
+  // (0 or 1: launched one by one)
+  int chain = 0;
   void *dt_state = nullptr;
   // OPS objects
   ops_block blk;
@@ -545,6 +551,13 @@ void timestep(bool device = false) {
   S.dtold = dt;
 }
 
+void chain_on() {
+  if (S.chain > 1) ops_hip_chain_begin(S.chain);
+}
+void chain_off() {
+  if (S.chain > 1) ops_hip_chain_end();
+}
+
 void revert() {
   int in[] = {field.x_min, field.x_max, field.y_min, field.y_max};
   ops_par_loop(cl_revert, "revert_kernel", S.blk, 2, in,
@@ -557,6 +570,7 @@ void revert() {
 void pdv(int predict) {
   int in[] = {field.x_min, field.x_max, field.y_min, field.y_max};
   if (predict) {
+    chain_on();
     ops_par_loop(cl_pdv_predict, "PdV_kernel_predict", S.blk, 2, in,
                  ops_arg_dat(S.xarea, 1, S.P10_P01_P11, "double", OPS_READ),
                  ops_arg_dat(S.xvel0, 1, S.P10_P01_P11, "double", OPS_READ),
@@ -589,9 +603,13 @@ void pdv(int predict) {
   }
   if (predict) {
     ideal_gas(1);
+    chain_off();
     int f[NUM_FIELDS];
     set_fields(f, {FIELD_PRESSURE});
     update_halo(f, 1);
+    // revert, accelerate, the full PdV and the fluxes form the next chain (the caller
+    // closes it after flux_calc)
+    chain_on();
     revert();
   }
 }
@@ -797,19 +815,25 @@ void advection() {
   int dir = S.advect_x ? XDIR : YDIR;
   set_fields(f, {FIELD_DENSITY1, FIELD_ENERGY1, FIELD_VOL_FLUX_X, FIELD_VOL_FLUX_Y});
   update_halo(f, 2);
+  chain_on();
   advec_cell(1, dir);
+  chain_off();
   if (dir == XDIR)
     set_fields(f, {FIELD_DENSITY1, FIELD_ENERGY1, FIELD_XVEL1, FIELD_YVEL1, FIELD_MASS_FLUX_X});
   else
     set_fields(f, {FIELD_DENSITY1, FIELD_ENERGY1, FIELD_XVEL1, FIELD_YVEL1, FIELD_MASS_FLUX_Y});
   update_halo(f, 2);
+  chain_on();
   advec_mom(1, 1, dir);
   advec_mom(2, 1, dir);
   dir = S.advect_x ? YDIR : XDIR;
   advec_cell(2, dir);
+  chain_off();
   set_fields(f, {FIELD_DENSITY1, FIELD_ENERGY1, FIELD_XVEL1, FIELD_YVEL1, FIELD_MASS_FLUX_X,
                  FIELD_MASS_FLUX_Y});
   update_halo(f, 2);
+  // the last momentum sweeps and reset_field form one chain (the caller closes it)
+  chain_on();
   advec_mom(1, 2, dir);
   advec_mom(2, 2, dir);
 }
@@ -890,8 +914,10 @@ void hydro_step() {
   accelerate();
   pdv(0);
   flux_calc();
+  chain_off();
   advection();
   reset_field();
+  chain_off();
   S.advect_x = !S.advect_x;
   S.time = S.time + dt;
   // clover_leaf.cpp:206-208: a field summary every summary_frequency steps
@@ -905,8 +931,10 @@ void device_step() {
   accelerate();
   pdv(0);
   flux_calc();
+  chain_off();
   advection();
   reset_field();
+  chain_off();
 }
 
 // n steps replayed from two captured graphs (one per sweep order); the results are those
@@ -985,6 +1013,15 @@ extern "C" int cloverleaf_init(int argc, const char **argv, const char *deck) {
     fprintf(stderr, "cloverleaf_init: %s\n", e.what());
     return 1;
   }
+}
+
+// loop chains: the loops between halo exchanges run band by band in `tiles` bands (0 or 1:
+// one launch per loop); returns the previous setting. Captured step graphs are rebuilt.
+extern "C" int cloverleaf_set_chain(int tiles) {
+  const int old = S.chain;
+  if (tiles != old) free_graphs();
+  S.chain = tiles;
+  return old;
 }
 
 // captured steps on (1) or off (0) for cloverleaf_step; returns the previous setting

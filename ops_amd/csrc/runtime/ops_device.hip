@@ -212,7 +212,12 @@ void (*g_batch_fn)(void) = nullptr;
 bool g_batch_keep = false;  // loop_begin of the pending stub: do not flush it
 }  // namespace
 
+static void chain_flush();
+static void chain_off();
+
 void ops_loop_batch_flush() {
+  // a recorded loop chain was issued before anything that flushes: it runs first
+  chain_flush();
   if (!g_batch_fn || g_batch_keep) return;
   void (*f)(void) = g_batch_fn;
   g_batch_fn = nullptr;
@@ -225,6 +230,7 @@ static void loop_batch_reset() {
   g_batch_keep = false;
   ops_loop_batch_flush();
   g_batch_on = false;
+  chain_off();
 }
 
 extern "C" void ops_hip_batch_begin(void) {
@@ -242,6 +248,180 @@ extern "C" int ops_hip_batch_join(int kernel_id, void (*flush)(void)) {
   if (g_batch_fn && g_batch_fn != flush) ops_loop_batch_flush();
   g_batch_fn = flush;
   return 1;
+}
+
+// ------------------------------------------------------------------ loop chains
+// Between ops_hip_chain_begin(tiles) and ops_hip_chain_end(), loops that can be tiled are
+// recorded instead of launched. When the chain ends (or anything else needs the device
+// first) it runs tile by tile along the loops' last dimension: every loop's range is cut
+// into bands and the bands run band-major (band 0 of each loop in program order, then band
+// 1, ...), so that what one loop writes is read by the next loops while it is still in the
+// caches (a band of a 3840^2 CloverLeaf advection chain fits the 256 MB Infinity Cache).
+// Band boundaries are skewed per loop, the dependency rule of the reference's lazy tiling
+// (ops/c/src/core/ops_lazy.cpp:380-1186): loop j's boundaries sit s_j rows behind the
+// common ones, s_j >= s_i for every earlier loop i sharing a dat with it, plus
+//   - the forward reach of j's read of a dat i writes (read after write), and
+//   - the backward reach of i's read of a dat j writes (write after read).
+// Each point of each loop then runs once, with the arithmetic of the untiled launch, after
+// everything it depends on and before anything that overwrites what it reads: the results
+// are bitwise those of the loops launched in order. Loops with reductions, multigrid or
+// strided stencils along the tiled dimension, low-dim reductions or fewer than two
+// dimensions are not recorded (the chain runs first, then they run as usual).
+namespace {
+struct ChainLoop {
+  void (*relaunch)(const char *, ops_block, int, int *, ops_arg *);
+  const char *name;
+  ops_block block;
+  int dim;
+  int range[6];
+  std::vector<ops_arg> args;
+  std::vector<std::vector<char>> gbl;  // the READ gbl values (the caller's may not outlive it)
+};
+struct Chain {
+  bool on = false, running = false;
+  int tiles = 1;
+  std::vector<ChainLoop> loops;
+} g_chain;
+}  // namespace
+
+static bool lowdim_reduction(const ops_arg &a);
+
+static bool chain_eligible(const ops_hip_loop *lp, int dim, const ops_arg *args, int nargs) {
+  if (lp->relaunch == nullptr || dim < 2 || dim > 3) return false;
+  const int ax = dim - 1;
+  for (int i = 0; i < nargs; i++) {
+    const ops_arg &a = args[i];
+    if (a.argtype == OPS_ARG_GBL && a.acc != OPS_READ) return false;
+    if (a.argtype != OPS_ARG_DAT || a.dat == nullptr || a.stencil == nullptr) continue;
+    if (a.stencil->type != 0 || lowdim_reduction(a)) return false;
+    const int st = ax < a.stencil->dims ? a.stencil->stride[ax] : 1;
+    if (st != 1 && !(st == 0 && a.acc == OPS_READ)) return false;
+    if (a.acc != OPS_READ)  // writes only at the point itself along the tiled dimension
+      for (int p = 0; p < a.stencil->points; p++)
+        if (ax < a.stencil->dims && a.stencil->stencil[p * a.stencil->dims + ax] != 0) return false;
+  }
+  return true;
+}
+
+// reach of a dat argument's stencil along the tiled dimension
+static void chain_reach(const ops_arg &a, int ax, int &lo, int &hi) {
+  lo = hi = 0;
+  if (ax >= a.stencil->dims) return;
+  for (int p = 0; p < a.stencil->points; p++) {
+    const int v = a.stencil->stencil[p * a.stencil->dims + ax];
+    if (p == 0 || v < lo) lo = v;
+    if (p == 0 || v > hi) hi = v;
+  }
+}
+
+static void chain_record(const ops_hip_loop *lp, const char *name, ops_block block, int dim,
+                         const int *range, const ops_arg *args, int nargs) {
+  int start[OPS_MAX_DIM] = {0, 0, 0, 0, 0}, end[OPS_MAX_DIM] = {1, 1, 1, 1, 1};
+  for (int d = 0; d < dim; d++) {
+    start[d] = range[2 * d];
+    end[d] = range[2 * d + 1];
+  }
+  bool empty = false;
+  for (int d = 0; d < dim; d++) empty = empty || end[d] <= start[d];
+  if (!empty) ops_check_loop_bounds(name, dim, start, end, args, nargs);  // report at the call
+  ChainLoop c;
+  c.relaunch = lp->relaunch;
+  c.name = name;
+  c.block = block;
+  c.dim = dim;
+  for (int d = 0; d < 6; d++) c.range[d] = d < 2 * dim ? range[d] : 0;
+  c.args.assign(args, args + nargs);
+  c.gbl.resize(nargs);
+  for (int i = 0; i < nargs; i++)
+    if (args[i].argtype == OPS_ARG_GBL && args[i].data != nullptr) {
+      c.gbl[i].assign(args[i].data, args[i].data + (size_t)args[i].dim * args[i].elem_size);
+      c.args[i].data = c.gbl[i].data();
+    }
+  g_chain.loops.push_back(std::move(c));
+}
+
+static void chain_flush() {
+  if (g_chain.loops.empty() || g_chain.running) return;
+  g_chain.running = true;
+  std::vector<ChainLoop> loops;
+  loops.swap(g_chain.loops);
+  try {
+    const int n = (int)loops.size();
+    const int ax = loops[0].dim - 1;
+    bool tile = g_chain.tiles > 1;
+    for (const ChainLoop &c : loops) tile = tile && c.dim == loops[0].dim;
+    // skews
+    std::vector<int> shift(n, 0);
+    for (int j = 0; j < n && tile; j++)
+      for (int i = 0; i < j && tile; i++)
+        for (const ops_arg &aj : loops[j].args) {
+          if (aj.argtype != OPS_ARG_DAT || !aj.dat || !aj.stencil) continue;
+          for (const ops_arg &ai : loops[i].args) {
+            if (ai.argtype != OPS_ARG_DAT || ai.dat != aj.dat || !ai.stencil) continue;
+            const bool wi = ai.acc != OPS_READ, wj = aj.acc != OPS_READ;
+            const bool ri = ai.acc != OPS_WRITE, rj = aj.acc != OPS_WRITE;
+            if (!wi && !wj) continue;
+            // a read that does not move along the tiled dimension sees every band
+            const int sti = ax < ai.stencil->dims ? ai.stencil->stride[ax] : 1;
+            const int stj = ax < aj.stencil->dims ? aj.stencil->stride[ax] : 1;
+            if (sti == 0 || stj == 0) {
+              tile = false;
+              break;
+            }
+            int ilo, ihi, jlo, jhi;
+            chain_reach(ai, ax, ilo, ihi);
+            chain_reach(aj, ax, jlo, jhi);
+            int need = 0;
+            if (wi && rj) need = std::max(need, jhi);   // read after write
+            if (wj && ri) need = std::max(need, -ilo);  // write after read
+            shift[j] = std::max(shift[j], shift[i] + need);
+          }
+          if (!tile) break;
+        }
+    if (!tile) {
+      for (ChainLoop &c : loops) c.relaunch(c.name, c.block, c.dim, c.range, c.args.data());
+    } else {
+      long lo = LONG_MAX, hi = LONG_MIN;
+      for (const ChainLoop &c : loops)
+        if (c.range[2 * ax + 1] > c.range[2 * ax]) {
+          lo = std::min(lo, (long)c.range[2 * ax]);
+          hi = std::max(hi, (long)c.range[2 * ax + 1]);
+        }
+      const int T = g_chain.tiles;
+      for (int t = 0; t < T; t++) {
+        const long b0 = lo + (hi - lo) * t / T, b1 = lo + (hi - lo) * (t + 1) / T;
+        for (int j = 0; j < n; j++) {
+          ChainLoop &c = loops[j];
+          const long l0 = c.range[2 * ax], l1 = c.range[2 * ax + 1];
+          const long s0 = t == 0 ? l0 : std::max(l0, b0 - shift[j]);
+          const long s1 = t == T - 1 ? l1 : std::min(l1, b1 - shift[j]);
+          if (s1 <= s0) continue;
+          int sub[6];
+          for (int d = 0; d < 6; d++) sub[d] = c.range[d];
+          sub[2 * ax] = (int)s0;
+          sub[2 * ax + 1] = (int)s1;
+          c.relaunch(c.name, c.block, c.dim, sub, c.args.data());
+        }
+      }
+    }
+  } catch (...) {
+    g_chain.running = false;
+    throw;
+  }
+  g_chain.running = false;
+}
+
+static void chain_off() { g_chain.on = false; }
+
+extern "C" void ops_hip_chain_begin(int tiles) {
+  ops_trid_flush();
+  g_chain.on = true;
+  g_chain.tiles = tiles < 1 ? 1 : tiles;
+}
+
+extern "C" void ops_hip_chain_end(void) {
+  g_chain.on = false;
+  chain_flush();
 }
 
 // ------------------------------------------------------------------ timing
@@ -431,6 +611,16 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
                                   ops_block block, int dim, const int *range, ops_arg *args,
                                   int nargs) {
   ops_runtime_instance *in = ops_instance_get();
+  lp->deferred = 0;
+  if (g_chain.on && !g_chain.running) {
+    if (chain_eligible(lp, dim, args, nargs)) {
+      chain_record(lp, name, block, dim, range, args, nargs);
+      lp->deferred = 1;
+      lp->nblocks = 0;
+      return 0;
+    }
+    chain_flush();  // a loop the chain cannot hold runs after it
+  }
   // another launch of the stub whose batch is pending may join that batch: everything
   // else pending runs first
   g_batch_keep = g_batch_on && lp->batch != nullptr && lp->batch == g_batch_fn;
@@ -765,6 +955,7 @@ static void finish_reductions(const ops_hip_loop *lp, const ops_arg *args, int n
 }
 
 extern "C" void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs) {
+  if (lp->deferred) return;  // recorded in a loop chain: its launches do this
   if (lp->nblocks > 0) finish_reductions(lp, args, nargs);
   if (lp->timed && lp->tune_cand >= 0) {
     ops_tune_end(lp->kernel_id, lp->ndim >= 2 ? 5 : 3, 2);

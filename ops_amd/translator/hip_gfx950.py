@@ -559,8 +559,14 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
               f"(hipStream_t)ops_batch_stream_{k}, ops_batch_{k});",
               f"  ops_hip_check_launch(\"{k} (batch)\");",
               "}", ""]
-    L += [f"void ops_par_loop_{k}(char const *name, ops_block block, int dim, int *range"
-          + (f", {sig}" if n else "") + ") {"]
+    # the stub again for a sub-range, with the recorded arguments (loop chains)
+    proto = (f"void ops_par_loop_{k}(char const *name, ops_block block, int dim, int *range"
+             + (f", {sig}" if n else "") + ")")
+    L += [proto + ";",
+          f"static void ops_relaunch_{k}(const char *name, ops_block block, int dim, int *range, ops_arg *a) {{",
+          f"  ops_par_loop_{k}(name, block, dim, range" + "".join(f", a[{i}]" for i in range(n)) + ");",
+          "}", ""]
+    L += [proto + " {"]
     L.append(f"  static int kid = ops_hip_register_kernel(\"{loop.kernel}\");")
     L.append(f"  ops_arg args[{max(n, 1)}] = {{{', '.join(f'arg{i}' for i in range(n)) or 'ops_arg_idx()'}}};")
     L.append("  ops_hip_loop lp;")
@@ -568,6 +574,7 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     mask = sum(1 << i for i, a in enumerate(loop.args) if a.kind == "dat" and a.prolong)
     L.append(f"  lp.prolong_mask = {mask}u;")
     L.append(f"  lp.batch = {'ops_batch_flush_' + k if bat else 'nullptr'};")
+    L.append(f"  lp.relaunch = ops_relaunch_{k};")
     L.append(f"  if (ops_hip_loop_begin(&lp, kid, name, block, dim, range, args, {n})) {{")
     L.append(f"    {ns}::ops_kp_{k} p;")
     L.append("    p.cnt0 = lp.count[0]; p.cnt1 = lp.count[1]; p.cnt2 = lp.count[2];")

@@ -814,6 +814,10 @@ extern "C" void ops_cpu_set_threads(int n) {
 // it comes
 extern "C" void ops_hip_batch_begin(void) {}
 extern "C" void ops_hip_batch_end(void) {}
+// loop chains only change the order in which a device runs the points of chained loops: the
+// host backend runs every loop whole, in program order
+extern "C" void ops_hip_chain_begin(int) {}
+extern "C" void ops_hip_chain_end(void) {}
 extern "C" void ops_hip_device_sync(void) {}
 extern "C" int ops_hip_graph_begin(void) { return 1; }
 extern "C" void *ops_hip_graph_end(void) { return nullptr; }

@@ -34,8 +34,9 @@ def lib(kind):
     return _cpu
 
 
-def run(kind, deck_text, steps=None, dump_dir=None, fetch=()):
-    """Run to the deck's end (or `steps` steps); returns (steps, summary[5], fetched dats)."""
+def run(kind, deck_text, steps=None, dump_dir=None, fetch=(), chain=0):
+    """Run to the deck's end (or `steps` steps); returns (steps, summary[5], fetched dats).
+    chain > 1: the loops between halo exchanges run as loop chains of that many bands."""
     import ops_amd
     L = lib(kind)
     with tempfile.TemporaryDirectory() as t:
@@ -44,6 +45,7 @@ def run(kind, deck_text, steps=None, dump_dir=None, fetch=()):
         toks = [f"OPS_DUMP_DIR={dump_dir}"] if dump_dir else []
         argc, argv = ops_amd.argv(*toks)
         assert L.cloverleaf_init(argc, argv, str(deck).encode()) == 0
+        L.cloverleaf_set_chain(chain)
         n = L.cloverleaf_step(steps) if steps is not None else L.cloverleaf_run(1 << 30)
         if steps is not None:
             assert n == 0

@@ -506,3 +506,72 @@ def test_gpu_batch_sequence_equals_oracle(mode, monkeypatch):
         gpu = batch_run("gpu", n, mode)
         for a, b in zip(cpu, gpu):
             assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- loop chains
+
+def chain_run(kind, nx, ny, tiles, mode):
+    L = lib("apitest", kind)
+    out = np.zeros(4 * nx * ny + 1)
+    argc, av = argv()
+    assert L.api_chain(argc, av, nx, ny, tiles, mode, p(out)) == 0
+    return out
+
+
+def chain_numpy(nx, ny, mode):
+    """The api_chain sequence in program order (apps/apitest: api_ch_init, l1..l5), on
+    [y][x] arrays with two halo layers."""
+    h = 2
+    y, x = np.mgrid[-h:ny + h, -h:nx + h]
+    # C's % truncates toward zero (negative in the halo): np.fmod, not Python's %
+    A = 1.0 + 0.01 * x - 0.003 * y * y + 0.25 * np.fmod(x + 3 * y, 7)
+    B = 0.5 - 0.02 * x * y + 0.125 * np.fmod(2 * x + y, 5)
+    C = np.zeros_like(A)
+    D = np.zeros_like(A)
+
+    def at(F, dx, dy, y0, y1):  # F(x + dx, y + dy) over rows y0..y1-1 and every x
+        return F[h + y0 + dy:h + y1 + dy, h + dx:h + nx + dx]
+
+    def put(F, v, y0, y1):
+        F[h + y0:h + y1, h:h + nx] = v
+
+    w = 2.0
+    put(C, at(A, 0, -1, 0, ny) + w * at(A, 0, 1, 0, ny) + at(B, 0, 0, 0, ny) + 0.25 * at(A, 1, 0, 0, ny), 0, ny)
+    put(A, 0.5 * at(C, 0, 0, 1, ny) + at(C, 0, 2, 1, ny) - 0.125 * at(C, -1, 0, 1, ny), 1, ny)
+    put(D, at(A, 0, -2, 0, ny - 1) - at(A, 0, 1, 0, ny - 1) + at(C, 0, -1, 0, ny - 1), 0, ny - 1)
+    total = float(np.sum(at(D, 0, 0, 0, ny))) if mode & 2 else 0.0
+    put(B, at(B, 0, 0, 2, ny) + at(D, 0, 1, 2, ny) - at(D, 0, -1, 2, ny), 2, ny)
+    put(C, at(D, 0, 0, 0, ny) + at(B, 0, 2, 0, ny) - at(B, 0, -2, 0, ny), 0, ny)
+    inner = [F[h:h + ny, h:h + nx].ravel() for F in (A, B, C, D)]
+    return np.concatenate(inner + [np.array([total])])
+
+
+@pytest.mark.parametrize("mode", [0, 7])
+def test_oracle_chain_sequence_equals_numpy(mode):
+    # the host backend runs chained loops whole and in order: the numpy restatement of the
+    # sequence, operation for operation (bitwise), the sum to rounding
+    nx, ny = 23, 19
+    got = chain_run("cpu", nx, ny, 4, mode)
+    want = chain_numpy(nx, ny, mode)
+    m = 4 * nx * ny
+    assert np.array_equal(got[:m].view(np.uint64), want[:m].view(np.uint64))
+    assert abs(got[m] - want[m]) <= 1e-12 * max(1.0, abs(want[m]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(37, 29), (300, 257)])
+@pytest.mark.parametrize("tiles", [2, 3, 7, 16, 40])
+@pytest.mark.parametrize("mode", [1, 3, 7])
+def test_gpu_chain_bitwise_equals_loops_in_order(shape, tiles, mode):
+    # chained loops run band by band with skewed boundaries: every field bitwise the loops
+    # run in order (the host backend, and the GPU without the chain); the reduction between
+    # l3 and l4 is not chained and sums in the GPU's order either way
+    nx, ny = shape
+    want = chain_run("cpu", nx, ny, tiles, mode)
+    eager = chain_run("gpu", nx, ny, tiles, mode & ~1)
+    got = chain_run("gpu", nx, ny, tiles, mode)
+    m = 4 * nx * ny
+    assert np.array_equal(eager[:m].view(np.uint64), want[:m].view(np.uint64))
+    assert np.array_equal(got[:m].view(np.uint64), want[:m].view(np.uint64))
+    assert got[m] == eager[m]
+    assert abs(got[m] - want[m]) <= 1e-12 * max(1.0, abs(want[m]))

@@ -112,6 +112,19 @@ def test_gpu_bitwise_equals_reference(golden, tmp_path, name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name,chain", [("clover_small", 3), ("clover_small", 16),
+                                        ("clover_default", 4), ("clover_bm_short", 8)])
+def test_gpu_loop_chains_bitwise_equal_reference(golden, tmp_path, name, chain):
+    # the loops between halo exchanges run as loop chains (band by band, skewed bands):
+    # every dat stays bitwise the reference's
+    fx = golden(f"{name}.json")
+    n, s, got = cu.run("gpu", fx["deck"], dump_dir=str(tmp_path), fetch=cu.SUMMARY_FIELDS, chain=chain)
+    assert n == fx["steps"]
+    assert check_dump(tmp_path, fx) == []
+    check_summary(s, got, fx, *DECK_SIZE[name])
+
+
+@pytest.mark.gpu
 def test_gpu_matches_oracle_midrun(golden):
     # stop mid-run (not at the deck's end) and compare fields and summary with the oracle
     fx = golden("clover_small.json")
