@@ -275,7 +275,7 @@ constexpr int PXP = PP * PM + 1;  // tile pitch (odd: the 16 lanes reading one r
                                   // hit 16 different banks)
 
 template <typename T, bool INC, bool XT, bool PCRR>
-__global__ __launch_bounds__(256) void trid_part(const Batch<T> B) {
+__global__ __launch_bounds__(256, 3) void trid_part(const Batch<T> B) {
   // reduced rows (sub-diagonal, diagonal, super-diagonal, rhs), two buffers for the PCR
   // steps; for x-lines they share the memory of the row tile (never live at the same time)
   constexpr size_t RED_BYTES = 2 * sizeof(T) * PL * 2 * PP * 4;
@@ -300,14 +300,16 @@ __global__ __launch_bounds__(256) void trid_part(const Batch<T> B) {
   T *d = S.d, *u = S.u;
   const bool act = live && c > 0;
   T ra[PM], rb[PM], rc[PM], rd[PM];
+  // x-lines: the line origins of the workgroup, kept in LDS rather than in 32 registers
+  // per lane (the kernel's register count sets how many wavefronts a SIMD holds)
+  __shared__ long long lo[XT ? PL : 1];
   if (XT) {
     // coalesced loads of the 16 lines' rows into the tile, one array at a time
     // thread r moves row r of every line (n <= 256): the 16 loads of an array are all in
     // flight before the first LDS write
     const long long nl = S.nlines - (g0 - S.line0);  // live lines of the workgroup
-    long long lo[PL];
-#pragma unroll
-    for (int q = 0; q < PL; q++) lo[q] = q < nl ? line_origin(S, 0, g0 - S.line0 + q) : 0;
+    if (tid < PL) lo[tid] = tid < nl ? line_origin(S, 0, g0 - S.line0 + tid) : 0;
+    __syncthreads();
     // software-pipelined: the next array's loads are in flight while the lanes read the
     // current one from the tile
     T v[PL];
@@ -539,8 +541,7 @@ __global__ __launch_bounds__(256) void trid_part(const Batch<T> B) {
 #pragma unroll
     for (int q = 0; q < PL; q++)
       if (q < nl && tid < n) {
-        const long long lo = line_origin(S, 0, g0 - S.line0 + q);
-        if (INC) u[lo + tid] += tile[q][tid]; else d[lo + tid] = tile[q][tid];
+        if (INC) u[lo[q] + tid] += tile[q][tid]; else d[lo[q] + tid] = tile[q][tid];
       }
   }
 }
