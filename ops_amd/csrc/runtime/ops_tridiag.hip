@@ -274,6 +274,11 @@ constexpr int PM = 16;  // rows per partition at most: n <= PP * PM
 constexpr int PXP = PP * PM + 1;  // tile pitch (odd: the 16 lanes reading one row of 16 lines
                                   // hit 16 different banks)
 
+// Three 256-lane workgroups per CU (three wavefronts per SIMD): the x-line variant needs ~240
+// registers per lane unconstrained (two wavefronts per SIMD) and spills a few rows to
+// scratch at 168; the extra wavefront in flight hides more HBM latency than the spills cost
+// (ADI's 48-block step 13.9 -> 12.4 ms; at four wavefronts, 128 registers, the spills
+// dominate: 19 ms). The y/z variants fit in 166 registers without spilling.
 template <typename T, bool INC, bool XT, bool PCRR>
 __global__ __launch_bounds__(256, 3) void trid_part(const Batch<T> B) {
   // reduced rows (sub-diagonal, diagonal, super-diagonal, rhs), two buffers for the PCR
