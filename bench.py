@@ -597,7 +597,9 @@ def clover3d_gpu(args, R):
     from ops_amd.apps.cloverleaf3d.decks import GOLD_KE_BM, bm_deck
     L = ops_amd.app("cloverleaf3d")
     out = {}
-    for tag, n, steps in (("deck_96", 96, 87), ("large", args.c3_cells, args.c3_steps)):
+    decks = [d for d in (("deck_96", 96, 87), ("large", args.c3_cells, args.c3_steps))
+             if d[0] in args.c3_decks.split(",")]
+    for tag, n, steps in decks:
         with tempfile.TemporaryDirectory() as t:
             deck = Path(t) / "clover.in"
             deck.write_text(bm_deck(n, steps))
@@ -742,6 +744,8 @@ def main():
     ap.add_argument("--no-clover3d", action="store_true")
     ap.add_argument("--c3-cells", type=int, default=320)
     ap.add_argument("--c3-steps", type=int, default=20)
+    ap.add_argument("--c3-decks", default="deck_96,large",
+                    help="CloverLeaf 3D meshes to run: deck_96 (the reference's deck), large")
     ap.add_argument("--tea-cells", type=int, default=4096)
     ap.add_argument("--tea-steps", type=int, default=10)
     ap.add_argument("--ingest-pmc", metavar="DIR", default=None,
