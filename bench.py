@@ -87,6 +87,29 @@ def traffic_from_profile(kernel, algorithmic=None):
         return None
 
 
+def kernel_short_name(name):
+    """A demangled kernel name without its return type, namespaces, ops_kern_ prefix and
+    parameter list; template arguments kept (trid_part<double, false, true, false>)."""
+    name = name.replace("(anonymous namespace)::", "")
+    if name.startswith("void "):
+        name = name[5:]
+    depth, cut = 0, len(name)
+    for i, ch in enumerate(name):  # the parameter list: the first '(' outside <...>
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    short = name[:cut]
+    head, _, tail = short.partition("<")
+    short = head.split("::")[-1] + (("<" + tail) if tail else "")
+    if short.startswith("ops_kern_"):
+        short = short[len("ops_kern_"):]
+    return short
+
+
 def ingest_pmc(d):
     """rocprofv3 --pmc CSVs (one run with FETCH_SIZE, one with WRITE_SIZE; values in KiB per
     dispatch) -> profiles/pmc_traffic.json. Each counter is scaled by what it reads for the
@@ -99,10 +122,7 @@ def ingest_pmc(d):
             rows += list(csv.DictReader(fh))
     per = {}
     for r in rows:
-        name = r["Kernel_Name"]
-        short = name.split("(")[0].split("::")[-1]
-        if short.startswith("ops_kern_"):
-            short = short[len("ops_kern_"):]
+        short = kernel_short_name(r["Kernel_Name"])
         per.setdefault((short, r["Counter_Name"]), []).append(float(r["Counter_Value"]) * 1024.0)
     probe_bytes = float(1 << 30)
     out = {"source": str(d), "unit": "bytes per dispatch", "kernels": {}}

@@ -27,6 +27,8 @@
 // so they are written and re-read coalesced.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -70,6 +72,10 @@ __device__ __forceinline__ long long line_origin(const Sys<T> &S, int sd, long l
   const long long p = l % S.n[0], q = l / S.n[0];
   return p + q * S.s1;
 }
+
+// y/z-line partitioned solves: workgroups per row of x positions (16 lines each, starting
+// at an aligned x that may lie up to 15 before the row's first position)
+__host__ __device__ inline long long trid_row_groups(int n0) { return (n0 + 30) / 16; }
 
 template <typename T> __device__ __forceinline__ int find_sys(const Batch<T> &B, long long g) {
   int k = 0;
@@ -292,15 +298,34 @@ __global__ __launch_bounds__(256, 3) void trid_part(const Batch<T> B) {
   const long long g0 = (long long)blockIdx.x * PL;
   const int k = find_sys(B, g0);
   const Sys<T> &S = B.sys[k];
-  const long long l = g0 - S.line0 + li;
-  const bool live = l < S.nlines;
   const int sd = XT ? 0 : B.solvedim;
   const int n = S.n[sd];
   const int m = (n + PP - 1) / PP;
   int c = n - p * m;
   c = c < 0 ? 0 : (c > m ? m : c);
   const long long st = sd == 0 ? 1 : (sd == 1 ? S.s1 : S.s2);
-  const long long o = live ? line_origin(S, sd, l) + (long long)p * m * st : 0;
+  bool live;
+  long long o = 0;
+  if (XT) {
+    const long long l = g0 - S.line0 + li;
+    live = l < S.nlines;
+    if (live) o = line_origin(S, 0, l);
+  } else {
+    // y- and z-lines: the workgroup takes 16 consecutive x positions of one row of the box
+    // (a z-plane row for y-lines, a y-row for z-lines) starting at a 128-byte boundary of
+    // the arrays (they share one layout), so that a row of its 16 lines is one whole cache
+    // line rather than parts of two that neighbouring workgroups fetch again; lanes outside
+    // the box idle, and trid_row_groups workgroups per row cover any alignment
+    const long long G = trid_row_groups(S.n[0]);
+    const long long w = (g0 - S.line0) / PL;
+    const long long r = w / G;
+    const long long ro = r * (sd == 1 ? S.s2 : S.s1);
+    const int mis = (int)(((uintptr_t)(S.d + ro) / sizeof(T)) % PL);
+    const long long x = PL * (w - r * G) - mis + li;
+    live = x >= 0 && x < S.n[0];
+    if (live) o = ro + x;
+  }
+  if (live) o += (long long)p * m * st;
   const T *__restrict__ a = S.a, *__restrict__ b = S.b, *__restrict__ cc_in = S.c;
   T *d = S.d, *u = S.u;
   const bool act = live && c > 0;
@@ -636,7 +661,10 @@ void launch_chunk(const Call *calls, size_t first, size_t count, T *fixed = null
     S.nlines = total / S.n[sd];
     S.line0 = line;
     S.scr = scr;
-    line += ((S.nlines + pad - 1) / pad) * pad;
+    if (part && sd != 0)  // rows of aligned 16-line groups (trid_part)
+      line += (S.nlines / S.n[0]) * trid_row_groups(S.n[0]) * PL;
+    else
+      line += ((S.nlines + pad - 1) / pad) * pad;
     if (!part) scr += 2 * total;
     B.sys[B.nsys++] = S;
   }
