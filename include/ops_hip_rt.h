@@ -64,6 +64,9 @@ typedef struct ops_hip_loop {
      loop band by band through it); NULL when the loop cannot be recorded */
   void (*relaunch)(const char *name, ops_block block, int dim, int *range, ops_arg *args);
   int deferred;  /* set by ops_hip_loop_begin: 1 when the loop was recorded in a chain */
+  /* timed launches (timed != 0): the stub launches with hipExtLaunchKernelGGL between these
+     two events (hipEvent_t), which then bracket the kernel's own execution */
+  void *ev_start, *ev_stop;
   /* loops with reductions: the launch is a capped 1-D grid of nblocks workgroups that walk
      the `tiles` tiles of the tile grid tgrid[0] x tgrid[1] x tgrid[2] grid-stride */
   unsigned tiles;

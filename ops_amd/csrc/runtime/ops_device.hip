@@ -443,16 +443,29 @@ int ops_tune_next(int kid, int ncand, int reps) {
   (void)reps;
 }
 
-void ops_tune_begin(int kid, int cand, double pts) {
+// a measured launch: events recorded on the stream around it (ops_tune_begin/_end), or
+// handed to the launch itself (record = false: the generated stubs pass them to
+// hipExtLaunchKernelGGL, so that they bracket the kernel and not the queue around it)
+static void tune_begin(int kid, int cand, double pts, bool record, hipEvent_t *ev) {
   hipEvent_t a = get_event(), b = get_event();
-  HIPCHECK(hipEventRecord(a, g_dev.stream));
+  if (record) HIPCHECK(hipEventRecord(a, g_dev.stream));
   g_dev.pending.push_back({kid, a, b, cand, pts});
+  if (ev) {
+    ev[0] = a;
+    ev[1] = b;
+  }
 }
 
-void ops_tune_end(int kid, int ncand, int reps) {
+void ops_tune_begin(int kid, int cand, double pts) { tune_begin(kid, cand, pts, true, nullptr); }
+
+static void tune_end(int kid, int ncand, int reps, bool record);
+
+void ops_tune_end(int kid, int ncand, int reps) { tune_end(kid, ncand, reps, true); }
+
+static void tune_end(int kid, int ncand, int reps, bool record) {
   auto &kt = ops_kernel_table();
   ops_kernel_entry &e = kt[kid];
-  HIPCHECK(hipEventRecord(g_dev.pending.back().b, g_dev.stream));
+  if (record) HIPCHECK(hipEventRecord(g_dev.pending.back().b, g_dev.stream));
   if (++e.tune_calls < ncand * reps) return;
   ops_dev_timing_flush();
   int best = 0;
@@ -871,16 +884,18 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
       lp->red_value[i] = h->data_d;
       lp->red_init[i] = h->dirty_d ? 0 : 1;
     }
-  // a timed launch runs alone: the pending batch goes first, outside its time
+  // a timed launch runs alone: the pending batch goes first, outside its time. The stub
+  // launches it between the two events (hipExtLaunchKernelGGL), so the time is the kernel's
+  // own, without the queue around it or the reduction finisher after it
   if (lp->tune_cand >= 0 || g_timing) ops_loop_batch_flush();
-  if (lp->tune_cand >= 0) {
-    ops_tune_begin(kernel_id, lp->tune_cand,
-                   (double)lp->count[0] * (double)lp->count[1] * (double)lp->count[2]);
-    lp->timed = 1;
-  } else if (g_timing) {
-    hipEvent_t a = get_event(), b = get_event();
-    HIPCHECK(hipEventRecord(a, g_dev.stream));
-    g_dev.pending.push_back({kernel_id, a, b, -1, 0.0});
+  lp->ev_start = lp->ev_stop = nullptr;
+  if (lp->tune_cand >= 0 || g_timing) {
+    hipEvent_t ev[2];
+    tune_begin(kernel_id, lp->tune_cand,
+               lp->tune_cand >= 0 ? (double)lp->count[0] * (double)lp->count[1] * (double)lp->count[2] : 0.0,
+               false, ev);
+    lp->ev_start = (void *)ev[0];
+    lp->ev_stop = (void *)ev[1];
     lp->timed = 1;
   }
   return 1;
@@ -958,9 +973,8 @@ extern "C" void ops_hip_loop_end(ops_hip_loop *lp, ops_arg *args, int nargs) {
   if (lp->deferred) return;  // recorded in a loop chain: its launches do this
   if (lp->nblocks > 0) finish_reductions(lp, args, nargs);
   if (lp->timed && lp->tune_cand >= 0) {
-    ops_tune_end(lp->kernel_id, lp->ndim >= 2 ? 5 : 3, 2);
+    tune_end(lp->kernel_id, lp->ndim >= 2 ? 5 : 3, 2, false);
   } else if (lp->timed) {
-    HIPCHECK(hipEventRecord(g_dev.pending.back().b, g_dev.stream));
     if (g_dev.pending.size() >= 8192) ops_dev_timing_flush();
   }
   auto &kt = ops_kernel_table();

@@ -594,8 +594,12 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
         elif a.kind == "reduce":
             t = CTYPES.get(a.typ, a.typ)
             L.append(f"    p.rp{i} = ({t} *)lp.red_part[{i}];")
-    launch = ("hipLaunchKernelGGL({}, dim3(lp.grid[0], lp.grid[1], lp.grid[2]), "
-              "dim3(lp.block[0], lp.block[1], lp.block[2]), 0, (hipStream_t)lp.stream, p{});")
+    # timed launches go between the runtime's two events (they bracket the kernel itself)
+    launch = ("if (lp.timed) hipExtLaunchKernelGGL({0}, dim3(lp.grid[0], lp.grid[1], lp.grid[2]), "
+              "dim3(lp.block[0], lp.block[1], lp.block[2]), 0, (hipStream_t)lp.stream, "
+              "(hipEvent_t)lp.ev_start, (hipEvent_t)lp.ev_stop, 0, p{1}); "
+              "else hipLaunchKernelGGL({0}, dim3(lp.grid[0], lp.grid[1], lp.grid[2]), "
+              "dim3(lp.block[0], lp.block[1], lp.block[2]), 0, (hipStream_t)lp.stream, p{1});")
     if bat:
         # join the pending batch (an untimed launch while batching is on), else launch alone
         # after whatever this kernel has pending
@@ -618,9 +622,11 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     if noalias_variant(loop):
         ptrs = "".join(f", p.p{i}" for i, a in enumerate(loop.args) if a.kind == "dat")
         L.append("    if (lp.alias)")
+        L.append("    {")
         L.append("      " + launch.format(f"{ns}::ops_kern_{k}_alias", ""))
-        L.append("    else")
+        L.append("    } else {")
         L.append("      " + launch.format(f"{ns}::ops_kern_{k}", ptrs))
+        L.append("    }")
     else:
         L.append("    " + launch.format(f"{ns}::ops_kern_{k}", ""))
     L.append("    ops_hip_check_launch(name);")
@@ -719,6 +725,7 @@ def translate_app(app_name: str, sources: List[str], out_dir: str,
          "// Do not edit: regenerate from the application sources.",
          f"#define OPS_{ndim}D",
          "#include <hip/hip_runtime.h>",
+         "#include <hip/hip_ext.h>",
          "#include <string.h>",
          "#include <math.h>",
          "#include \"ops_lib_core.h\"",
