@@ -71,6 +71,13 @@ typedef struct ops_hip_loop {
      the `tiles` tiles of the tile grid tgrid[0] x tgrid[1] x tgrid[2] grid-stride */
   unsigned tiles;
   unsigned tgrid[3];
+  /* set by the stub: the dat argument whose rows the x tiles start on 128-byte lines of
+     (-1: none). ops_hip_loop_begin sets align to the elements per 128-byte line of that
+     argument when the launch uses aligned tiles (a loop without reductions, one point per
+     lane, wider than a wavefront, the argument contiguous along x), else to 0; the grid then
+     has room for the shift of up to align - 1 points at the start of each row */
+  int align_arg;
+  int align;
 } ops_hip_loop;
 
 /* Registers a generated loop name once; returns its id for timing and tuning tables. */

@@ -48,12 +48,14 @@ APP_SOURCES = {
 APP_SHARED_NATIVE = {
     "cloverleaf3d": [("cloverleaf", "native/clover_dt.hip")],
 }
-# translator options per application. ADI's preproc writes the ten coefficient and
-# right-hand-side arrays that the tridiagonal solves read next: with non-temporal stores
-# its step measured 4 % slower (18.1 against 17.4 ms), while the CloverLeaf steps gained
-# 0.5-3 %, so ADI keeps plain stores
+# translator options per application. ADI's preproc writes ten coefficient and right-hand-
+# side arrays of one layout: with x tiles on 128-byte lines of their rows and non-temporal
+# stores its 48-block step measured 11.0 ms, against 11.8 with neither and 11.6 with aligned
+# tiles alone (plain stores into split lines had measured slower than plain stores, 18.1
+# against 17.4 ms with the round-one solver); CloverLeaf keeps unaligned tiles (7 % slower
+# aligned, its dats have several row pitches)
 APP_TRANSLATE = {
-    "adi": {"ntstore": False},
+    "adi": {"ntstore": True, "align": True},
 }
 # builds of an application's sources with extra defines: name -> (application, defines)
 APP_VARIANTS = {

@@ -824,22 +824,38 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
               "and at most 256 threads");
   // The generated kernels address a dat argument from the workgroup's origin (64-bit,
   // wave-uniform) plus the lane's offset inside the workgroup, tx*sx + ty*sy + tz*sz,
-  // taken as 24-bit products along x and y (full-rate multiplies): the x and y steps must
-  // be below 2^24 elements and the lane offset must fit 32 bits.
+  // taken as 24-bit products along x and y (full-rate multiplies): the x step must be below
+  // 2^23 elements (signed product: the lane's x offset is negative in aligned tiles), the y
+  // step below 2^24, and the lane offset must fit a signed 32-bit sum.
   for (int i = 0; i < nargs; i++) {
     if (lp->ptr[i] == nullptr || args[i].argtype != OPS_ARG_DAT) continue;
     const long long lane_max = (long long)(bx - 1) * lp->step[i][0] +
                                (long long)(by - 1) * lp->step[i][1] +
                                (long long)(bz - 1) * lp->step[i][2];
-    if (lp->step[i][0] >= (1 << 24) || lp->step[i][1] >= (1 << 24) || lane_max > UINT_MAX)
+    if (lp->step[i][0] >= (1 << 23) || lp->step[i][1] >= (1 << 24) || lane_max > INT_MAX)
       ops_throw(OPS_INVALID_ARGUMENT,
                 std::string("ops_par_loop ") + (name ? name : "") + ": argument " +
-                    std::to_string(i) + " has an x or y step of 2^24 elements or more");
+                    std::to_string(i) + " has an x step of 2^23 or a y step of 2^24 elements or more");
+  }
+  // x tiles on 128-byte lines of the anchor argument's rows: each wavefront's row segment
+  // is then whole cache lines (a row of a dat starts wherever its pitch puts it), so stores
+  // fill lines rather than splitting them between workgroups. Loops with reductions keep
+  // their tiles (the tiling sets the order of the partial sums).
+  bool red_loop = false;
+  for (int i = 0; i < nargs; i++)
+    if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) red_loop = true;
+  lp->align = 0;
+  const int aa = lp->align_arg;
+  if (aa >= 0 && aa < nargs && !red_loop && !narrow && vx == 1 && args[aa].argtype == OPS_ARG_DAT &&
+      lp->ptr[aa] != nullptr && lp->step[aa][0] == 1) {
+    const int esz = args[aa].dat->type_size;
+    const int per_line = (esz == 4 || esz == 8) ? 128 / esz : 0;
+    if (per_line > 0 && bx % (unsigned)per_line == 0) lp->align = per_line;
   }
   lp->block[0] = bx;
   lp->block[1] = by;
   lp->block[2] = bz;
-  lp->grid[0] = (lp->count[0] + bx * vx - 1) / (bx * vx);
+  lp->grid[0] = (lp->count[0] + (lp->align ? lp->align - 1 : 0) + bx * vx - 1) / (bx * vx);
   lp->grid[1] = (lp->count[1] + by - 1) / by;
   lp->grid[2] = (lp->count[2] + bz - 1) / bz;
   lp->nblocks = lp->grid[0] * lp->grid[1] * lp->grid[2];

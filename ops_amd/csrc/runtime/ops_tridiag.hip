@@ -57,6 +57,7 @@ template <typename T> struct Batch {
   Sys<T> sys[MAXSYS];
   int nsys;
   int solvedim;
+  unsigned group;  // partitioned solve: workgroups per XCD run (trid_group)
 };
 
 // line l of system S: its origin (elements from the box origin) and stride
@@ -77,9 +78,13 @@ __device__ __forceinline__ long long line_origin(const Sys<T> &S, int sd, long l
 // at an aligned x that may lie up to 15 before the row's first position)
 __host__ __device__ inline long long trid_row_groups(int n0) { return (n0 + 30) / 16; }
 
+// the system holding global line g: the count of later systems whose first line is at or
+// before g (line0 increases with the system index). Every comparison reads its own line0,
+// so the loads are issued together rather than one after another as a search would.
 template <typename T> __device__ __forceinline__ int find_sys(const Batch<T> &B, long long g) {
   int k = 0;
-  while (k + 1 < B.nsys && g >= B.sys[k + 1].line0) k++;
+#pragma unroll
+  for (int i = 1; i < MAXSYS; i++) k += (i < B.nsys && g >= B.sys[i].line0) ? 1 : 0;
   return k;
 }
 
@@ -271,6 +276,18 @@ __global__ __launch_bounds__(64) void thomas_x(const Batch<T> B, T *__restrict__
 // the sequential sweep also writes and re-reads c', d'. The result differs from the
 // sequential sweep's in the last bits (the eliminations are ordered differently).
 constexpr int PL = 16;  // lines per workgroup
+// Line groups of consecutive workgroups run on the same XCD in runs of nb (workgroup b
+// runs on XCD b % 8; within each span of 8 nb workgroups, XCD x takes groups nb x ..
+// nb x + nb - 1). Neighbouring groups of 16 y- or z-lines share the cache lines that
+// straddle them (a row of a group is 128-byte aligned only where the row pitch allows),
+// and on one XCD the second reader finds them in that XCD's L2; the XCDs still work on
+// neighbouring spans at any moment. A partial last span keeps its order.
+__device__ __forceinline__ unsigned trid_group(unsigned b, unsigned n, unsigned nb) {
+  const unsigned S = 8 * nb;
+  if (nb <= 1 || b >= n - n % S) return b;
+  const unsigned q = b / S, r = b % S;
+  return q * S + (r % 8) * nb + r / 8;
+}
 constexpr int PP = 16;  // partitions (lanes) per line
 constexpr int PM = 16;  // rows per partition at most: n <= PP * PM
 
@@ -295,7 +312,7 @@ __global__ __launch_bounds__(256, 3) void trid_part(const Batch<T> B) {
   T(*red)[PL][2 * PP][4] = (T(*)[PL][2 * PP][4])smem;
   T(*tile)[PXP] = (T(*)[PXP])smem;
   const int tid = threadIdx.x, li = tid % PL, p = tid / PL;
-  const long long g0 = (long long)blockIdx.x * PL;
+  const long long g0 = (long long)trid_group(blockIdx.x, gridDim.x, B.group) * PL;
   const int k = find_sys(B, g0);
   const Sys<T> &S = B.sys[k];
   const int sd = XT ? 0 : B.solvedim;
@@ -628,6 +645,9 @@ void launch_chunk(const Call *calls, size_t first, size_t count, T *fixed = null
   Batch<T> B;
   B.nsys = 0;
   B.solvedim = calls[first].solvedim;
+  // runs of 4 neighbouring line groups per XCD: ADI's 48-block step 12.2 -> 11.8 ms (runs of
+  // 2 and 8 measure the same; one XCD band per launch, ops_xcd_remap, 12.5 ms)
+  B.group = 4;
   const int sd = B.solvedim;
   bool part = fixed == nullptr && !use_thomas();
   for (size_t k = first; k < first + count; k++) {

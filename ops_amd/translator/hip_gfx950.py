@@ -290,6 +290,29 @@ def loop_vx(loop: Loop) -> int:
     return VX
 
 
+# x tiles that start on 128-byte lines of a written argument's rows (OPS_GEN_ALIGN=1, or the
+# per-application option): whole-line stores for kernels that write many arrays of one
+# layout (ADI's preproc: 12.2 -> 11.6 ms per step, more with non-temporal stores). Off by
+# default: when a loop's dats have different row pitches the other arguments stay split,
+# and the CloverLeaf 2D step measured 7 % slower with them
+ALIGN = os.environ.get("OPS_GEN_ALIGN", "0") == "1"
+_align_on = ALIGN  # the current application's setting (translate_app)
+
+
+def align_anchor(loop: Loop) -> int:
+    """The dat argument whose rows the x tiles of a launch start on 128-byte lines of (the
+    first written one-component dat; -1 for loops with reductions, several points per lane
+    or no such argument, and with aligned tiles off). The runtime decides per launch
+    (ops_hip_loop.align)."""
+    if not _align_on or any(a.kind == "reduce" for a in loop.args) or loop_vx(loop) != 1:
+        return -1
+    for i, a in enumerate(loop.args):
+        if (a.kind == "dat" and a.acc in ("OPS_WRITE", "OPS_RW", "OPS_INC") and not a.prolong and
+                cparse.int_literal(a.dim) == 1):
+            return i
+    return -1
+
+
 def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
                pointwise: Optional[set] = None, written_once: Optional[set] = None) -> str:
     k = loop.stub
@@ -299,6 +322,8 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
     L.append("  int cnt0, cnt1, cnt2, st0, st1, st2;")
     if any(a.kind == "reduce" for a in loop.args):
         L.append("  unsigned ntiles, tg0, tg1;  // tile count and tile grid (grid-stride walk)")
+    if align_anchor(loop) >= 0:
+        L.append("  int al;  // elements per 128-byte line of the anchor argument (aligned x tiles), or 0")
     for i, a in enumerate(loop.args):
         if a.kind == "dat":
             t = params[i].ctype
@@ -390,18 +415,24 @@ def noalias_variant(loop: Loop) -> bool:
     return len(dats) >= 2 and any(a.acc != "OPS_READ" for a in dats)
 
 
-def dat_offset(i: int, ndim: int) -> str:
+def dat_offset(i: int, ndim: int, aligned: bool = False) -> str:
     """Element offset of dat argument i at point (x, y, z): the workgroup origin's part in
     64 bits, then the lane's part (unsigned, 24-bit products along x and y). Dimensions
-    beyond the application's are 0 inside the kernel's bounds test, so they are left out."""
+    beyond the application's are 0 inside the kernel's bounds test, so they are left out.
+    aligned: the lane's x offset is txm = tx - mis, which can be negative: its product is a
+    signed 24-bit multiply and the lane's part a signed sum (the runtime keeps the x step
+    below 2^23 and the lane's part below 2^31)."""
     uni = [f"(long long)x0 * p.sx{i}"]
-    lane = [f"(unsigned)__umul24(tx, (unsigned)p.sx{i})"]
+    if aligned:
+        lane = [f"__mul24(txm, p.sx{i})"]
+    else:
+        lane = [f"(unsigned)__umul24(tx, (unsigned)p.sx{i})"]
     if ndim >= 2:
         uni.append(f"(long long)y0 * p.sy{i}")
-        lane.append(f"(unsigned)__umul24(ty, (unsigned)p.sy{i})")
+        lane.append(f"(int)__umul24(ty, (unsigned)p.sy{i})" if aligned else f"(unsigned)__umul24(ty, (unsigned)p.sy{i})")
     if ndim >= 3:
         uni.append(f"(long long)z0 * p.sz{i}")
-        lane.append(f"tz * (unsigned)p.sz{i}")
+        lane.append(f"(int)(tz * (unsigned)p.sz{i})" if aligned else f"tz * (unsigned)p.sz{i}")
     return f"({' + '.join(uni)}) + ({' + '.join(lane)})"
 
 
@@ -445,6 +476,18 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
     L.append("  const int y = y0 + (int)ty;")
     L.append("  const int z = z0 + (int)tz;")
     vx = loop_vx(loop)
+    anc = align_anchor(loop)
+    if anc >= 0:
+        # x tiles start on 128-byte lines of the anchor argument's row (y, z): the lane's x
+        # moves back by that row's offset into its first line (0 without alignment)
+        row = f"p.p{anc}"
+        if ndim >= 2:
+            row += f" + (long long)y * p.sy{anc}"
+        if ndim >= 3:
+            row += f" + (long long)z * p.sz{anc}"
+        L.append(f"  const int mis = p.al ? (int)(((uintptr_t)({row}) / sizeof(*p.p{anc})) & "
+                 "(unsigned)(p.al - 1)) : 0;")
+        L.append("  const int txm = (int)tx - mis;")
     if vx > 1:
         # vx points per lane, blockDim.x apart: every pass of the wavefront stays contiguous
         L.append("#pragma unroll")
@@ -452,8 +495,12 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
         L.append(f"  const int x0 = (int)((bx * {vx} + v) * blockDim.x);")
     else:
         L.append("  const int x0 = (int)(bx * blockDim.x);")
-    L.append("  const int x = x0 + (int)tx;")
-    L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
+    if anc >= 0:
+        L.append("  const int x = x0 + txm;")
+        L.append("  if (x >= 0 && x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
+    else:
+        L.append("  const int x = x0 + (int)tx;")
+        L.append("  if (x < p.cnt0 && y < p.cnt1 && z < p.cnt2) {")
     if v1:
         # sizes behind OPS_ACC<n>(...) for the dat arguments (xdim<n> = size[0], ...)
         L.append("    ops_v1_shape v1s;")
@@ -471,7 +518,7 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
                                      "must be ACC<T>& (or a pointer addressed with OPS_ACC<n>) "
                                      "for an ops_arg_dat argument")
             t = pr.ctype
-            off = dat_offset(i, ndim)
+            off = dat_offset(i, ndim, anc >= 0)
             if a.prolong:
                 # coarse index = global loop index / mgrid stride (C division, as the
                 # reference's generated loops: templates/cpp/mpi_openmp/loop_host.cpp.j2:234-237)
@@ -571,6 +618,7 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     L.append(f"  ops_arg args[{max(n, 1)}] = {{{', '.join(f'arg{i}' for i in range(n)) or 'ops_arg_idx()'}}};")
     L.append("  ops_hip_loop lp;")
     L.append(f"  lp.vx = {loop_vx(loop)};")
+    L.append(f"  lp.align_arg = {align_anchor(loop)};")
     mask = sum(1 << i for i, a in enumerate(loop.args) if a.kind == "dat" and a.prolong)
     L.append(f"  lp.prolong_mask = {mask}u;")
     L.append(f"  lp.batch = {'ops_batch_flush_' + k if bat else 'nullptr'};")
@@ -579,6 +627,8 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     L.append(f"    {ns}::ops_kp_{k} p;")
     L.append("    p.cnt0 = lp.count[0]; p.cnt1 = lp.count[1]; p.cnt2 = lp.count[2];")
     L.append("    p.st0 = lp.start[0]; p.st1 = lp.start[1]; p.st2 = lp.start[2];")
+    if align_anchor(loop) >= 0:
+        L.append("    p.al = lp.align;")
     if any(a.kind == "reduce" for a in loop.args):
         L.append("    p.ntiles = lp.tiles; p.tg0 = lp.tgrid[0]; p.tg1 = lp.tgrid[1];")
     for i, a in enumerate(loop.args):
@@ -647,9 +697,13 @@ def const_decl(c: Const) -> Tuple[str, str]:
 
 
 def translate_app(app_name: str, sources: List[str], out_dir: str,
-                  app_dir: Optional[str] = None, ntstore: Optional[bool] = None) -> Dict[str, object]:
+                  app_dir: Optional[str] = None, ntstore: Optional[bool] = None,
+                  align: Optional[bool] = None) -> Dict[str, object]:
     """Translate the application made of `sources`; returns the generated file list.
-    `ntstore` overrides OPS_GEN_NTSTORE (non-temporal stores of write-once arguments)."""
+    `ntstore` overrides OPS_GEN_NTSTORE (non-temporal stores of write-once arguments),
+    `align` OPS_GEN_ALIGN (x tiles on 128-byte lines of a written argument's rows)."""
+    global _align_on
+    _align_on = ALIGN if align is None else align
     use_nt = NTSTORE if ntstore is None else ntstore
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
