@@ -15,16 +15,19 @@
 // decoder convolutions read both sources directly; nothing is copied).
 //
 // Storage: activations NHWC in BF16, weights [Cout][3][3][Cin] in BF16 (an FP32 master copy
-// is updated by Adam), gradients FP32. Convolutions are implicit GEMMs on MFMA
-// (v_mfma_f32_16x16x16bf16_1k, FP32 accumulation): M = pixels, N = output channels,
-// K = 9 taps x input channels. The lane layout of the instruction (A: lane l holds
-// A[l % 16][4 (l / 16) + j], B: B[4 (l / 16) + j][l % 16], D: D[4 (l / 16) + j][l % 16],
-// j = 0..3) was checked on the device against a host product.
-//   forward / input gradient: a 256-lane workgroup covers 64 pixels of one row, each
-//     wavefront a 16-pixel M tile and all output channels (up to 7 N tiles: the input
-//     gradient of the first decoder layer has 112 channels);
-//   weight gradient: one wavefront per (tap, band of rows), all (Cout, Cin) tiles in
-//     registers, K = pixels; bands are summed by a second kernel in a fixed order.
+// is updated by Adam), gradients FP32. Convolutions are implicit GEMMs on gfx950's
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulation). Its lane layout (A: lane l holds
+// A[l % 16][8 (l / 16) + j], B: B[8 (l / 16) + j][l % 16], j = 0..7; D: D[4 (l / 16) + j]
+// [l % 16], j = 0..3) was checked on the device against a host product.
+//   forward / input gradient: M = pixels, N = output channels, K = (tap, input channel)
+//     flattened; a wavefront covers 64 consecutive pixels (four M tiles) and all output
+//     channels (up to 7 N tiles: the input gradient of the first decoder layer has 112), so
+//     each weight fragment feeds four products and each input fragment up to seven;
+//   weight gradient: M = output channels, N = input channels, K = 32 pixels of a row, from
+//     row-chunked copies of dy and of the layer input ([b][y][x / 8][c][8]: a lane's 8
+//     pixels of one channel are one 16-byte load); thin layers run all nine taps in one
+//     wavefront per band of rows, the others one wavefront per tap with the nine of a band
+//     in one workgroup; bands are summed by a second kernel in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -42,9 +45,11 @@
 
 namespace {
 
-typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));   // 8 BF16 values, one 16-byte load
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));    // the MFMA operand type
 typedef uint16_t bf16;
+__device__ __forceinline__ bf8 as_bf8(u4 v) { return __builtin_bit_cast(bf8, v); }
 
 __host__ __device__ inline bf16 f2bf(float f) {
   // round to nearest even
@@ -62,7 +67,6 @@ __host__ __device__ inline float bf2f(bf16 h) {
 }
 
 constexpr int NL = 8;  // convolution layers
-constexpr int WG_BANDS = 1024;  // row bands of the weight gradient (partial sums per band)
 // layer table: input channels (source 1 = upsampled, source 2 = same resolution), output
 // channels, resolution level
 struct LayerSpec {
@@ -92,68 +96,96 @@ struct ConvArgs {
   int relu;
 };
 
-// forward convolution (and the input-gradient convolution with transformed weights)
+// forward convolution (and the input-gradient convolution with transformed weights) as an
+// implicit GEMM on v_mfma_f32_16x16x32_bf16: M = pixels, N = output channels, K = (tap,
+// input channel) flattened, k = 9-tap index * cin + channel, in chunks of 32 (the tail
+// chunk zero-padded). cin is a multiple of 16, so a lane's 8 consecutive k (one 16-byte
+// load) are 8 channels of one pixel at one tap. A wavefront computes four 16-pixel M tiles
+// (64 consecutive pixels of the image, row-major; any width) by NT 16-channel N tiles, so
+// every weight fragment it loads feeds four products and every input fragment NT.
+template <int NT>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
-  const int y = blockIdx.y, b = blockIdx.z;
-  const int H = a.H, W = a.W, cin = a.c1 + a.c2;
-  const int xm = blockIdx.x * 64 + wv * 16 + m;  // this lane's A-row pixel
-  const int nt = a.cout >> 4;  // <= 7, checked by the host
-  f4 acc[7];
+  const int b = blockIdx.y;
+  const int H = a.H, W = a.W, c1 = a.c1, c2 = a.c2, cin = c1 + c2;
+  const int K = 9 * cin, nkc = (K + 31) / 32;
+  const int npix = H * W;
+  const int q0 = (blockIdx.x * 4 + wv) * 64;  // the wavefront's first pixel
+  int py[4], px[4];
+  bool pv[4];
 #pragma unroll
-  for (int t = 0; t < 7; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < 4; t++) {
+    const int q = q0 + 16 * t + m;
+    pv[t] = q < npix;
+    py[t] = q / W;
+    px[t] = q - py[t] * W;
+  }
+  f4 acc[4][NT];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int u = 0; u < NT; u++) acc[t][u] = f4{0.f, 0.f, 0.f, 0.f};
   const int Hc = H >> 1, Wc = W >> 1;
-  for (int tap = 0; tap < 9; tap++) {
+  const bf16 *wrow = a.w + (long)m * K;  // weight row of output channel m (+ 16 u)
+  for (int kc = 0; kc < nkc; kc++) {
+    const int k = kc * 32 + 8 * kq;
+    const bool kv = k < K;
+    const int tap = kv ? k / cin : 0, ci = k - tap * cin;
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int xs = xm + dx, ys = y + dy;
-    const bool inb = xs >= 0 && xs < W && ys >= 0 && ys < H;
-    for (int c0 = 0; c0 < cin; c0 += 16) {
-      const int c = c0 + 4 * kq;
-      s4 av = s4{0, 0, 0, 0};
-      if (inb) {
-        const bf16 *p = (c < a.c1)
-                            ? a.src1 + (((long)b * Hc + (ys >> 1)) * Wc + (xs >> 1)) * a.c1 + c
-                            : a.src2 + (((long)b * H + ys) * W + xs) * a.c2 + (c - a.c1);
-        av = *(const s4 *)p;
-      }
+    u4 av[4], bv[NT];
 #pragma unroll
-      for (int t = 0; t < 7; t++) {
-        if (t < nt) {
-          const s4 bv = *(const s4 *)(a.w + ((long)(16 * t + m) * 9 + tap) * cin + c);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
-        }
+    for (int t = 0; t < 4; t++) {
+      const int ys = py[t] + dy, xs = px[t] + dx;
+      av[t] = u4{0u, 0u, 0u, 0u};
+      if (kv && pv[t] && ys >= 0 && ys < H && xs >= 0 && xs < W) {
+        const bf16 *src = ci < c1 ? a.src1 + (((long)b * Hc + (ys >> 1)) * Wc + (xs >> 1)) * c1 + ci
+                                  : a.src2 + (((long)b * H + ys) * W + xs) * c2 + (ci - c1);
+        av[t] = *(const u4 *)src;
       }
     }
-  }
-  // D[pixel 4 kq + j][channel m] of each N tile
 #pragma unroll
-  for (int t = 0; t < 7; t++) {
-    if (t >= nt) continue;
-    const int co = 16 * t + m;
+    for (int u = 0; u < NT; u++) bv[u] = kv ? *(const u4 *)(wrow + (long)16 * u * K + k) : u4{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int u = 0; u < NT; u++)
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[t]), as_bf8(bv[u]), acc[t][u], 0, 0, 0);
+  }
+  // D[pixel 4 kq + j of the tile][channel m of the N tile]
+  const int cout = a.cout;
+#pragma unroll
+  for (int u = 0; u < NT; u++) {
+    const int co = 16 * u + m;
     const float bia = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int px = blockIdx.x * 64 + wv * 16 + 4 * kq + j;
-      if (px >= W) continue;
-      float v = acc[t][j] + bia;
-      if (a.relu && v < 0.f) v = 0.f;
-      const long o = (((long)b * H + y) * W + px) * a.cout + co;
-      if (a.out) a.out[o] = f2bf(v);
-      if (a.out32) a.out32[o] = v;
-    }
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int q = q0 + 16 * t + 4 * kq + j;
+        if (q >= npix) continue;
+        float v = acc[t][u][j] + bia;
+        if (a.relu && v < 0.f) v = 0.f;
+        const long o = ((long)b * npix + q) * cout + co;
+        if (a.out) a.out[o] = f2bf(v);
+        if (a.out32) a.out32[o] = v;
+      }
   }
 }
 
-// channel-major copy of a layer input for the weight gradient: dst[b][c][y][x] for the
-// concatenation [up(src1), src2] (src1 read at (y/2, x/2)); a workgroup moves a 64-pixel
-// piece of one row through LDS so that both the NHWC reads and the CHW writes are contiguous
+// row-chunked copy of a layer input (or of the masked output gradient) for the weight
+// gradient: dst[b][y][xc][c][8] holds pixels 8 xc .. 8 xc + 7 of channel c of row y, so a
+// lane's 8 consecutive pixels of one channel are one 16-byte load and the 16 channels of an
+// operand tile are 256 contiguous bytes; rows are padded with zeros to whole chunks
+// (Wb = ceil(W / 8)). Input: the concatenation [up(src1), src2] (src1 read at (y/2, x/2)).
+// A workgroup moves 64 pixels of one row through LDS, so both the NHWC reads and the
+// row-chunked writes are contiguous.
 constexpr int TP = 64;
-__global__ __launch_bounds__(256) void k_to_chw(const bf16 *src1, const bf16 *src2, int c1, int c2, int H, int W,
-                                                bf16 *dst) {
+__global__ __launch_bounds__(256) void k_to_rc(const bf16 *src1, const bf16 *src2, int c1, int c2, int H, int W,
+                                               bf16 *dst) {
   __shared__ bf16 t[112][TP + 2];
   const int C = c1 + c2, x0 = blockIdx.x * TP, y = blockIdx.y, b = blockIdx.z;
-  const int Hc = H >> 1, Wc = W >> 1;
+  const int Hc = H >> 1, Wc = W >> 1, Wb = (W + 7) >> 3;
   for (int e = threadIdx.x; e < TP * C; e += 256) {
     const int px = e / C, c = e % C, x = x0 + px;
     bf16 v = 0;
@@ -163,122 +195,224 @@ __global__ __launch_bounds__(256) void k_to_chw(const bf16 *src1, const bf16 *sr
     t[c][px] = v;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < TP * C; e += 256) {
-    const int c = e / TP, px = e % TP, x = x0 + px;
-    if (x < W) dst[(((long)b * C + c) * H + y) * W + x] = t[c][px];
+  // chunks x0 / 8 .. of this row: one contiguous block of nch * C * 8 values
+  const int xc0 = x0 >> 3, nch = min(TP / 8, Wb - xc0);
+  bf16 *out = dst + (((long)b * H + y) * Wb + xc0) * C * 8;
+  for (int e = threadIdx.x; e < nch * C * 8; e += 256) {
+    const int j = e & 7, c = (e >> 3) % C, xl = (e >> 3) / C;
+    out[e] = t[c][xl * 8 + j];
   }
 }
 
 struct WgradArgs {
-  const bf16 *dyT;   // masked output gradient, channel-major [B][cout][H][W]
-  const bf16 *xT;    // layer input, channel-major [B][cin][H][W]
+  const bf16 *dyT;   // masked output gradient, row-chunked [B][H][Wb][cout][8]
+  const bf16 *xT;    // layer input, row-chunked [B][H][Wb][cin][8]
   int cin, cout, H, W, rows, band;  // rows = B*H, band = rows per wavefront
   float *part;       // [nbands][9][cout][cin]
 };
 
-// 4 consecutive values of a channel-major row at pixels p, p+1, p+2, p+3, zero outside
-// [0, W): aligned 8-byte loads where the row allows them
-__device__ __forceinline__ s4 row4(const bf16 *row, int p, int W) {
-  if ((W & 3) == 0 && p >= 0 && (p & 3) == 0 && p + 3 < W) return *(const s4 *)(row + p);
-  if ((W & 3) == 0 && p >= 3 && ((p + 1) & 3) == 0 && p + 4 < W) {  // p = 4k - 1: reads p-3 .. p+4
-    const s4 lo = *(const s4 *)(row + p - 3), hi = *(const s4 *)(row + p + 1);
-    return s4{lo[3], hi[0], hi[1], hi[2]};
+// the 8 values of one channel of a row-chunked row at pixels 8 xc + dx .. 8 xc + dx + 7
+// (dx = -1, 0, 1), zero outside the row's Wb chunks (the chunks' own padding is zero):
+// `row` points at chunk 0 of the channel, chunks are cs values apart. A shift loads the
+// neighbouring chunk's edge pair and moves the values by one BF16 across the 32-bit words
+// (v_alignbyte_b32: the low word of {hi, lo} >> 16).
+__device__ __forceinline__ u4 rc8(const bf16 *row, int xc, int dx, int Wb, int cs) {
+  if (xc >= Wb) return u4{0u, 0u, 0u, 0u};  // past the row: the other operand is zero there
+  const u4 h = *(const u4 *)(row + (long)xc * cs);
+  if (dx == 0) return h;
+  if (dx < 0) {
+    const unsigned l = xc > 0 ? *(const unsigned *)(row + (long)(xc - 1) * cs + 6) : 0u;  // pixels -2, -1
+    return u4{__builtin_amdgcn_alignbyte(h[0], l, 2), __builtin_amdgcn_alignbyte(h[1], h[0], 2),
+              __builtin_amdgcn_alignbyte(h[2], h[1], 2), __builtin_amdgcn_alignbyte(h[3], h[2], 2)};
   }
-  if ((W & 3) == 0 && p >= 1 && ((p - 1) & 3) == 0 && p + 3 < W) {  // p = 4k + 1: reads p-1 .. p+3
-    const s4 lo = *(const s4 *)(row + p - 1);
-    const short nx = (short)row[p + 3];
-    return s4{lo[1], lo[2], lo[3], nx};
-  }
-  s4 v = s4{0, 0, 0, 0};
-#pragma unroll
-  for (int j = 0; j < 4; j++)
-    if (p + j >= 0 && p + j < W) v[j] = (short)row[p + j];
-  return v;
+  const unsigned n = xc + 1 < Wb ? *(const unsigned *)(row + (long)(xc + 1) * cs) : 0u;  // pixels 8, 9
+  return u4{__builtin_amdgcn_alignbyte(h[1], h[0], 2), __builtin_amdgcn_alignbyte(h[2], h[1], 2),
+            __builtin_amdgcn_alignbyte(h[3], h[2], 2), __builtin_amdgcn_alignbyte(n, h[3], 2)};
 }
 
-// partial weight gradient of one band of rows for one tap:
+// partial weight gradient of one band of rows for one tap, on v_mfma_f32_16x16x32_bf16:
 // dW[co][tap][ci] += sum over pixels p of dy[p][co] x[p + offset(tap)][ci]
-__global__ __launch_bounds__(64) void k_wgrad(WgradArgs a) {
-  const int lane = threadIdx.x, m = lane & 15, kq = lane >> 4;
-  const int band = blockIdx.x, tap = blockIdx.y;
+// (M = output channels, NTC tiles; N = input channels, NTN tiles; K = 32 pixels of a row per
+// product). A workgroup is nine wavefronts, one per tap, over the same band of rows: they
+// read the same rows of dy and the same three rows of x (shifted), so all but the first
+// reader find them in the CU's L1. The loads of U consecutive 32-pixel chunks are issued
+// before their products, so a wavefront waits for memory once per U chunks.
+template <int NTC, int NTN, int U>
+__global__ __launch_bounds__(576) void k_wgrad(WgradArgs a) {
+  const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+  const int band = blockIdx.x, tap = threadIdx.x >> 6;
   const int dy = tap / 3 - 1, dx = tap % 3 - 1;
   const int H = a.H, W = a.W, cin = a.cin;
-  const int ntc = a.cout >> 4, ntn = cin >> 4;  // ntc <= 4, ntn <= 7
-  f4 acc[4][7];
+  f4 acc[NTC][NTN];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < NTC; i++)
 #pragma unroll
-    for (int j = 0; j < 7; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < NTN; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int Wb = (W + 7) >> 3, cso = 8 * a.cout, csi = 8 * cin;  // chunk strides
   const int r0 = band * a.band, r1 = min(a.rows, r0 + a.band);
   for (int r = r0; r < r1; r++) {
     const int b = r / H, y = r % H, ys = y + dy;
     if (ys < 0 || ys >= H) continue;  // the shifted row is zero padding
-    for (int x0 = 0; x0 < W; x0 += 16) {
-      const int p = x0 + 4 * kq;
-      // A = dy^T (cout x pixels): lane holds channel 16 tc + m, pixels p .. p + 3
-      s4 av[4];
+    const bf16 *dyrow = a.dyT + ((long)b * H + y) * Wb * cso + 8 * m;
+    const bf16 *xrow = a.xT + ((long)b * H + ys) * Wb * csi + 8 * m;
+    for (int x0 = 0; x0 < 8 * Wb; x0 += 32 * U) {
+      // A = dy^T (cout x pixels): lane holds channel 16 tc + m, pixels p .. p + 7;
+      // B = shifted input (pixels x cin): lane holds channel 16 tn + m, pixels p + dx ..
+      u4 av[U][NTC], bv[U][NTN];
 #pragma unroll
-      for (int tc = 0; tc < 4; tc++)
-        av[tc] = tc < ntc ? row4(a.dyT + (((long)b * a.cout + 16 * tc + m) * H + y) * W, p, W) : s4{0, 0, 0, 0};
+      for (int v = 0; v < U; v++) {
+        const int xc = (x0 >> 3) + 4 * v + kq;
 #pragma unroll
-      for (int tn = 0; tn < 7; tn++) {
-        if (tn >= ntn) continue;
-        // B = shifted input (pixels x cin): lane holds channel 16 tn + m, pixels p + dx ..;
-        // pixels past the row end carry no gradient (A is zero there), so only the
-        // shifted read itself needs the bounds
-        const s4 bv = row4(a.xT + (((long)b * cin + 16 * tn + m) * H + ys) * W, p + dx, W);
+        for (int tc = 0; tc < NTC; tc++) av[v][tc] = rc8(dyrow + 128 * tc, xc, 0, Wb, cso);
 #pragma unroll
-        for (int tc = 0; tc < 4; tc++)
-          if (tc < ntc) acc[tc][tn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[tc], bv, acc[tc][tn], 0, 0, 0);
+        for (int tn = 0; tn < NTN; tn++) bv[v][tn] = rc8(xrow + 128 * tn, xc, dx, Wb, csi);
       }
+#pragma unroll
+      for (int v = 0; v < U; v++)
+#pragma unroll
+        for (int tn = 0; tn < NTN; tn++)
+#pragma unroll
+          for (int tc = 0; tc < NTC; tc++)
+            acc[tc][tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[v][tc]), as_bf8(bv[v][tn]), acc[tc][tn], 0, 0, 0);
     }
   }
   // D[co = 16 tc + 4 kq + j][ci = 16 tn + m]
   float *out = a.part + ((long)band * 9 + tap) * a.cout * cin;
 #pragma unroll
-  for (int tc = 0; tc < 4; tc++)
+  for (int tc = 0; tc < NTC; tc++)
 #pragma unroll
-    for (int tn = 0; tn < 7; tn++)
-      if (tc < ntc && tn < ntn)
+    for (int tn = 0; tn < NTN; tn++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) out[(long)(16 * tc + 4 * kq + j) * cin + 16 * tn + m] = acc[tc][tn][j];
+      for (int j = 0; j < 4; j++) out[(long)(16 * tc + 4 * kq + j) * cin + 16 * tn + m] = acc[tc][tn][j];
 }
 
-// dW[co][tap][ci] = sum over bands (fixed order)
+// the same partial weight gradient for layers with few (output, input) channel tiles
+// (NTC * NTN <= 3: the full-resolution layers), all nine taps in one wavefront: the dy
+// chunk it loads feeds nine taps, and each of the three shifted x rows three, so a wavefront
+// issues about one load per product instead of two or more. A workgroup is four wavefronts,
+// each with its own band of rows (the partials are per band, as above).
+template <int NTC, int NTN, int U>
+__global__ __launch_bounds__(256) void k_wgrad9(WgradArgs a, int nbands) {
+  const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+  const int band = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (band >= nbands) return;
+  const int H = a.H, W = a.W, cin = a.cin;
+  f4 acc[9][NTC][NTN];
+#pragma unroll
+  for (int t = 0; t < 9; t++)
+#pragma unroll
+    for (int i = 0; i < NTC; i++)
+#pragma unroll
+      for (int j = 0; j < NTN; j++) acc[t][i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int Wb = (W + 7) >> 3, cso = 8 * a.cout, csi = 8 * cin;
+  const int r0 = band * a.band, r1 = min(a.rows, r0 + a.band);
+  for (int r = r0; r < r1; r++) {
+    const int b = r / H, y = r % H;
+    const bf16 *dyrow = a.dyT + ((long)b * H + y) * Wb * cso + 8 * m;
+    for (int x0 = 0; x0 < 8 * Wb; x0 += 32 * U) {
+      u4 av[U][NTC];
+#pragma unroll
+      for (int v = 0; v < U; v++)
+#pragma unroll
+        for (int tc = 0; tc < NTC; tc++) av[v][tc] = rc8(dyrow + 128 * tc, (x0 >> 3) + 4 * v + kq, 0, Wb, cso);
+#pragma unroll
+      for (int dyi = 0; dyi < 3; dyi++) {
+        const int ys = y + dyi - 1;
+        if (ys < 0 || ys >= H) continue;  // zero padding: no contribution
+        const bf16 *xrow = a.xT + ((long)b * H + ys) * Wb * csi + 8 * m;
+        u4 h[U][NTN];
+        unsigned lo[U][NTN], hi[U][NTN];
+#pragma unroll
+        for (int v = 0; v < U; v++)
+#pragma unroll
+          for (int tn = 0; tn < NTN; tn++) {
+            const int xc = (x0 >> 3) + 4 * v + kq;
+            const bf16 *c0 = xrow + 128 * tn;
+            const bool in = xc < Wb;
+            h[v][tn] = in ? *(const u4 *)(c0 + (long)xc * csi) : u4{0u, 0u, 0u, 0u};
+            lo[v][tn] = (in && xc > 0) ? *(const unsigned *)(c0 + (long)(xc - 1) * csi + 6) : 0u;
+            hi[v][tn] = (xc + 1 < Wb) ? *(const unsigned *)(c0 + (long)(xc + 1) * csi) : 0u;
+          }
+#pragma unroll
+        for (int v = 0; v < U; v++)
+#pragma unroll
+          for (int tn = 0; tn < NTN; tn++) {
+            const u4 q = h[v][tn];
+            const u4 bm = u4{__builtin_amdgcn_alignbyte(q[0], lo[v][tn], 2), __builtin_amdgcn_alignbyte(q[1], q[0], 2),
+                             __builtin_amdgcn_alignbyte(q[2], q[1], 2), __builtin_amdgcn_alignbyte(q[3], q[2], 2)};
+            const u4 bp = u4{__builtin_amdgcn_alignbyte(q[1], q[0], 2), __builtin_amdgcn_alignbyte(q[2], q[1], 2),
+                             __builtin_amdgcn_alignbyte(q[3], q[2], 2), __builtin_amdgcn_alignbyte(hi[v][tn], q[3], 2)};
+#pragma unroll
+            for (int tc = 0; tc < NTC; tc++) {
+              acc[3 * dyi][tc][tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[v][tc]), as_bf8(bm), acc[3 * dyi][tc][tn], 0, 0, 0);
+              acc[3 * dyi + 1][tc][tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[v][tc]), as_bf8(q), acc[3 * dyi + 1][tc][tn], 0, 0, 0);
+              acc[3 * dyi + 2][tc][tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[v][tc]), as_bf8(bp), acc[3 * dyi + 2][tc][tn], 0, 0, 0);
+            }
+          }
+      }
+    }
+  }
+  // D[co = 16 tc + 4 kq + j][ci = 16 tn + m] of every tap
+#pragma unroll
+  for (int t = 0; t < 9; t++) {
+    float *out = a.part + ((long)band * 9 + t) * a.cout * cin;
+#pragma unroll
+    for (int tc = 0; tc < NTC; tc++)
+#pragma unroll
+      for (int tn = 0; tn < NTN; tn++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[(long)(16 * tc + 4 * kq + j) * cin + 16 * tn + m] = acc[t][tc][tn][j];
+  }
+}
+
+// dW[co][tap][ci] = sum over bands, in a fixed order. Pass one (seg > 0): workgroup
+// (x, s) sums bands s L .. s L + L - 1 (L = ceil(nbands / nseg)) of 64 consecutive
+// (tap, co, ci) entries into segment partial s (four groups of threads take every fourth
+// band, then the four group sums are added in order); pass two sums the nseg segment
+// partials the same way and writes dw in its [co][tap][ci] layout.
 __global__ __launch_bounds__(256) void k_wgrad_sum(const float *part, int nbands, int cout, int cin,
-                                                   float *dw) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;  // over [9][cout][cin]
+                                                   float *seg, float *dw) {
+  __shared__ float lds[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const long i = (long)blockIdx.x * 64 + tx;  // over [9][cout][cin]
   const long n = 9L * cout * cin;
-  if (i >= n) return;
+  const int nseg = gridDim.y, L = (nbands + nseg - 1) / nseg;
+  const int k0 = blockIdx.y * L, k1 = min(nbands, k0 + L);
   float s = 0.f;
-  for (int k = 0; k < nbands; k++) s += part[(long)k * n + i];
+  if (i < n)
+    for (int k = k0 + ty; k < k1; k += 4) s += part[(long)k * n + i];
+  lds[ty][tx] = s;
+  __syncthreads();
+  if (ty != 0 || i >= n) return;
+  s = ((lds[0][tx] + lds[1][tx]) + lds[2][tx]) + lds[3][tx];
+  if (seg) {
+    seg[(long)blockIdx.y * n + i] = s;
+    return;
+  }
   const int tap = (int)(i / ((long)cout * cin)), rem = (int)(i % ((long)cout * cin));
   const int co = rem / cin, ci = rem % cin;
   dw[((long)co * 9 + tap) * cin + ci] = s;
 }
 
-// masked gradient in BF16 for the MFMA operands, and per-workgroup partial bias gradient
-__global__ __launch_bounds__(256) void k_mask(const float *g, const bf16 *y, int relu, long n,
-                                              int cout, bf16 *gm, float *gmf) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  float v = g[i];
-  if (relu && bf2f(y[i]) <= 0.f) v = 0.f;
-  gm[i] = f2bf(v);
-  gmf[i] = v;
-  (void)cout;
-}
-
-// bias gradient, first pass: each workgroup sums the masked gradient of its pixel slice per
-// channel (thread t owns channel t % cout and every (256 / cout)-th pixel), bpart[wg][co]
-constexpr int BIAS_WG = 512;
-__global__ __launch_bounds__(256) void k_bias_part(const float *gmf, long npix, int cout, float *bpart) {
+// masked gradient in BF16 for the MFMA operands and the first pass of the bias gradient:
+// each of BIAS_WG workgroups walks its slice of pixels (thread t owns channel t % cout and
+// every (256 / cout)-th pixel, so each pass of the workgroup covers consecutive elements)
+// and writes its per-channel sums, bpart[wg][co]
+constexpr int BIAS_WG = 2048;
+__global__ __launch_bounds__(256) void k_mask_bias(const float *g, const bf16 *y, int relu, long npix,
+                                                   int cout, bf16 *gm, float *bpart) {
   __shared__ float lds[256];
   const int lanes = 256 / cout, t = threadIdx.x, c = t % cout, q = t / cout;
   const long per = (npix + BIAS_WG - 1) / BIAS_WG, p0 = blockIdx.x * per, p1 = min(npix, p0 + per);
   float s = 0.f;
   if (q < lanes)
-    for (long p = p0 + q; p < p1; p += lanes) s += gmf[p * cout + c];
+    for (long p = p0 + q; p < p1; p += lanes) {
+      const long i = p * cout + c;
+      float v = g[i];
+      if (relu && bf2f(y[i]) <= 0.f) v = 0.f;
+      gm[i] = f2bf(v);
+      s += v;
+    }
   lds[t] = s;
   __syncthreads();
   if (t < cout) {
@@ -288,13 +422,15 @@ __global__ __launch_bounds__(256) void k_bias_part(const float *gmf, long npix, 
   }
 }
 
-// second pass: db[co] = sum over workgroups (fixed order)
-__global__ __launch_bounds__(64) void k_bias_fin(const float *bpart, int cout, float *db) {
-  const int c = threadIdx.x;
-  if (c >= cout) return;
+// second pass: db[co] = sum over the workgroups' partials, one workgroup per channel (each
+// thread a strided share in order, then the workgroup tree)
+__global__ __launch_bounds__(256) void k_bias_fin(const float *bpart, int cout, float *db) {
+  __shared__ float lds[8];
+  const int c = blockIdx.x;
   float v = 0.f;
-  for (int k = 0; k < BIAS_WG; k++) v += bpart[k * cout + c];
-  db[c] = v;
+  for (int k = threadIdx.x; k < BIAS_WG; k += 256) v += bpart[k * cout + c];
+  v = ops_block_reduce<ops_red_inc<float>>(v, lds);
+  if (threadIdx.x == 0) db[c] = v;
 }
 
 // weights for the input gradient: wt[ci][8 - tap][co] = w[co][tap][ci]
@@ -440,11 +576,11 @@ struct Net {
   float *gin = nullptr; // input gradient of a layer (largest)
   float *gpool = nullptr;
   bf16 *gm = nullptr;   // masked output gradient, BF16
-  bf16 *gmT = nullptr;  // the same, channel-major (weight gradient)
-  bf16 *xT = nullptr;   // a layer's input, channel-major (weight gradient)
-  float *gmf = nullptr; // same, FP32
+  bf16 *gmT = nullptr;  // the same, row-chunked (weight gradient)
+  bf16 *xT = nullptr;   // a layer's input, row-chunked (weight gradient)
   float *wpart = nullptr;
   size_t wpart_cap = 0;
+  float *wseg = nullptr;  // segment partials of the two-pass band sum (at most 2048 x 64 values)
   float *lpart = nullptr;
   float *bpart = nullptr;  // bias-gradient partials [BIAS_WG][64]
   char *pinned = nullptr;
@@ -483,7 +619,6 @@ void free_acts() {
   dfree(N.gm);
   dfree(N.gmT);
   dfree(N.xT);
-  dfree(N.gmf);
   dfree(N.lpart);
   dfree(N.bpart);
   N.B = N.H = N.W = 0;
@@ -500,11 +635,14 @@ void ensure_acts(int B, int H, int W) {
   N.H = H;
   N.W = W;
   auto pix = [&](int level) { return (long)B * (H >> level) * (W >> level); };
-  long maxg = 0;
+  long maxg = 0, maxrc = 0;
   for (int l = 0; l < NL; l++) {
     N.act[l] = dalloc<bf16>(pix(SPEC[l].level) * SPEC[l].cout);
     N.grad[l] = dalloc<float>(pix(SPEC[l].level) * SPEC[l].cout);
     maxg = std::max(maxg, pix(SPEC[l].level) * (long)std::max(cin_of(l), SPEC[l].cout));
+    // row-chunked copies: rows padded to whole 8-pixel chunks
+    const long rows = (long)B * (H >> SPEC[l].level), wb = ((W >> SPEC[l].level) + 7) / 8;
+    maxrc = std::max(maxrc, rows * wb * 8 * (long)std::max(cin_of(l), SPEC[l].cout));
   }
   for (int lv = 1; lv < 4; lv++) N.pooled[lv] = dalloc<bf16>(pix(lv) * SPEC[lv].c2);
   N.in = dalloc<bf16>(pix(0) * 16);
@@ -515,13 +653,37 @@ void ensure_acts(int B, int H, int W) {
   N.gin = dalloc<float>(maxg);
   N.gpool = dalloc<float>(maxg);
   N.gm = dalloc<bf16>(maxg);
-  N.gmT = dalloc<bf16>(maxg);
-  N.xT = dalloc<bf16>(maxg);
-  N.gmf = dalloc<float>(maxg);
+  N.gmT = dalloc<bf16>(maxrc);
+  N.xT = dalloc<bf16>(maxrc);
   N.lpart = dalloc<float>(g1d(pix(0)) + 1);
   N.bpart = dalloc<float>((size_t)BIAS_WG * 64);
 }
 
+
+// the weight-gradient kernel for the layer's tile counts (one instance per layer shape of
+// the network)
+void launch_wgrad(const WgradArgs &a, int nbands) {
+  const int ntc = a.cout / 16, ntn = a.cin / 16;
+  if (a.cout % 16 || a.cin % 16 || ntc < 1 || ntc > 4 || ntn < 1 || ntn > 7)
+    throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: weight-gradient shape outside the kernel's limits");
+  const dim3 grid((unsigned)nbands), blk(576);
+  hipStream_t st = stream();
+  const int key = ntc * 10 + ntn;
+  const dim3 grid9((unsigned)((nbands + 3) / 4));
+  switch (key) {
+    case 11: hipLaunchKernelGGL((k_wgrad9<1, 1, 4>), grid9, dim3(256), 0, st, a, nbands); return;
+    case 21: hipLaunchKernelGGL((k_wgrad9<2, 1, 2>), grid9, dim3(256), 0, st, a, nbands); return;
+    case 13: hipLaunchKernelGGL((k_wgrad9<1, 3, 2>), grid9, dim3(256), 0, st, a, nbands); return;
+    default: break;
+  }
+  switch (key) {
+    case 32: hipLaunchKernelGGL((k_wgrad<3, 2, 2>), grid, blk, 0, st, a); break;
+    case 43: hipLaunchKernelGGL((k_wgrad<4, 3, 2>), grid, blk, 0, st, a); break;
+    case 37: hipLaunchKernelGGL((k_wgrad<3, 7, 1>), grid, blk, 0, st, a); break;
+    case 25: hipLaunchKernelGGL((k_wgrad<2, 5, 2>), grid, blk, 0, st, a); break;
+    default: throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: no weight-gradient kernel for this layer shape");
+  }
+}
 
 // k_conv assumes channel counts in multiples of 16, at most 7 output tiles, and src1 present
 // exactly when c1 > 0
@@ -529,7 +691,17 @@ void launch_conv(const ConvArgs &a, int B) {
   if (a.c1 % 16 || a.c2 % 16 || a.c2 <= 0 || a.cout % 16 || a.cout <= 0 || a.cout > 112 ||
       (a.c1 > 0) != (a.src1 != nullptr) || !a.src2 || !a.w || a.H <= 0 || a.W <= 0 || (a.c1 && (a.H % 2 || a.W % 2)))
     throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: convolution shape outside the kernel's limits");
-  hipLaunchKernelGGL(k_conv, dim3((unsigned)((a.W + 63) / 64), (unsigned)a.H, (unsigned)B), dim3(256), 0, stream(), a);
+  // four wavefronts of 64 pixels per workgroup, pixels of one image row-major
+  const dim3 grid((unsigned)(((long)a.H * a.W + 255) / 256), (unsigned)B);
+  switch (a.cout / 16) {
+    case 1: hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), 0, stream(), a); break;
+    case 2: hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, stream(), a); break;
+    case 3: hipLaunchKernelGGL(k_conv<3>, grid, dim3(256), 0, stream(), a); break;
+    case 4: hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, stream(), a); break;
+    case 5: hipLaunchKernelGGL(k_conv<5>, grid, dim3(256), 0, stream(), a); break;
+    case 6: hipLaunchKernelGGL(k_conv<6>, grid, dim3(256), 0, stream(), a); break;
+    default: hipLaunchKernelGGL(k_conv<7>, grid, dim3(256), 0, stream(), a); break;
+  }
 }
 
 // sources of layer l
@@ -587,16 +759,15 @@ void backward() {
     const int lv = SPEC[l].level, h = H >> lv, w = W >> lv, cout = SPEC[l].cout, cin = cin_of(l);
     const long npix = (long)B * h * w, n = npix * cout;
     // masked gradient
-    hipLaunchKernelGGL(k_mask, dim3(g1d(n)), dim3(256), 0, st, N.grad[l], N.act[l], SPEC[l].relu, n, cout,
-                       N.gm, N.gmf);
-    hipLaunchKernelGGL(k_bias_part, dim3(BIAS_WG), dim3(256), 0, st, N.gmf, npix, cout, N.bpart);
-    hipLaunchKernelGGL(k_bias_fin, dim3(1), dim3(64), 0, st, N.bpart, cout, N.g + N.boff[l]);
-    // weight gradient from channel-major copies of the masked gradient and the layer input
+    hipLaunchKernelGGL(k_mask_bias, dim3(BIAS_WG), dim3(256), 0, st, N.grad[l], N.act[l], SPEC[l].relu, npix,
+                       cout, N.gm, N.bpart);
+    hipLaunchKernelGGL(k_bias_fin, dim3((unsigned)cout), dim3(256), 0, st, N.bpart, cout, N.g + N.boff[l]);
+    // weight gradient from row-chunked copies of the masked gradient and the layer input
     const bf16 *s1, *s2;
     sources(l, s1, s2);
     const dim3 tgrid((unsigned)((w + TP - 1) / TP), (unsigned)h, (unsigned)B);
-    hipLaunchKernelGGL(k_to_chw, tgrid, dim3(256), 0, st, nullptr, N.gm, 0, cout, h, w, N.gmT);
-    hipLaunchKernelGGL(k_to_chw, tgrid, dim3(256), 0, st, s1, s2, SPEC[l].c1, SPEC[l].c2, h, w, N.xT);
+    hipLaunchKernelGGL(k_to_rc, tgrid, dim3(256), 0, st, nullptr, N.gm, 0, cout, h, w, N.gmT);
+    hipLaunchKernelGGL(k_to_rc, tgrid, dim3(256), 0, st, s1, s2, SPEC[l].c1, SPEC[l].c2, h, w, N.xT);
     WgradArgs wa;
     wa.dyT = N.gmT;
     wa.xT = N.xT;
@@ -605,9 +776,15 @@ void backward() {
     wa.H = h;
     wa.W = w;
     wa.rows = B * h;
-    wa.band = std::max(1, (B * h + WG_BANDS - 1) / WG_BANDS);
+    // bands of rows: partial sums of at most ~16 MB (~64 MB for the nine-tap kernel, whose
+    // wavefronts each take a band), at least 64 and at most 1024 (4096) bands
+    const long per_band = 9L * cout * cin;
+    const bool nine = (cout / 16) * (cin / 16) <= 3;
+    const long cap = nine ? 4096L : 1024L;
+    const int want = (int)std::min(cap, std::max(64L, (nine ? (16L << 20) : (4L << 20)) / per_band));
+    wa.band = std::max(1, (B * h + want - 1) / want);
     const int nbands = (B * h + wa.band - 1) / wa.band;
-    const size_t need = (size_t)nbands * 9 * cout * cin;
+    const size_t need = (size_t)nbands * per_band;
     if (need > N.wpart_cap) {
       sync();
       dfree(N.wpart);
@@ -615,9 +792,19 @@ void backward() {
       N.wpart_cap = need;
     }
     wa.part = N.wpart;
-    hipLaunchKernelGGL(k_wgrad, dim3((unsigned)nbands, 9), dim3(64), 0, st, wa);
-    hipLaunchKernelGGL(k_wgrad_sum, dim3(g1d(9L * cout * cin)), dim3(256), 0, st, N.wpart, nbands, cout, cin,
-                       N.g + N.woff[l]);
+    launch_wgrad(wa, nbands);
+    // band sums in two passes when the bands are many: enough workgroups to fill the GPU
+    const unsigned gx = (unsigned)((per_band + 63) / 64);
+    const int nseg = std::max(1, std::min(nbands / 16, (int)(2048 / gx)));
+    if (nseg > 1) {
+      hipLaunchKernelGGL(k_wgrad_sum, dim3(gx, (unsigned)nseg), dim3(256), 0, st, N.wpart, nbands, cout, cin,
+                         N.wseg, nullptr);
+      hipLaunchKernelGGL(k_wgrad_sum, dim3(gx, 1), dim3(256), 0, st, N.wseg, nseg, cout, cin, nullptr,
+                         N.g + N.woff[l]);
+    } else {
+      hipLaunchKernelGGL(k_wgrad_sum, dim3(gx, 1), dim3(256), 0, st, N.wpart, nbands, cout, cin, nullptr,
+                         N.g + N.woff[l]);
+    }
     if (l == 0) break;  // no gradient w.r.t. the network input
     // input gradient: convolution of the masked gradient with the transformed weights
     hipLaunchKernelGGL(k_wtrans, dim3(g1d(9L * cout * cin)), dim3(256), 0, st, N.wb + N.woff[l], cout, cin, N.wt);
@@ -1136,6 +1323,7 @@ extern "C" int tea_unet_init(unsigned long long seed) {
       N.m2 = dalloc<float>(off);
       N.wb = dalloc<bf16>(off);
       N.wt = dalloc<bf16>(maxw);
+      N.wseg = dalloc<float>(2048L * 64);
       N.live = true;
     }
     // He initialisation from splitmix64 (uniform with the normal's variance), zero biases
@@ -1314,6 +1502,7 @@ extern "C" int tea_unet_free(void) {
     dfree(N.wt);
     dfree(N.wpart);
     N.wpart_cap = 0;
+    dfree(N.wseg);
     N.live = false;
   });
 }
