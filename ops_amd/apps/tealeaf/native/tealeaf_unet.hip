@@ -398,19 +398,29 @@ __global__ __launch_bounds__(256) void k_wgrad_sum(const float *part, int nbands
 // each of BIAS_WG workgroups walks its slice of pixels (thread t owns channel t % cout and
 // every (256 / cout)-th pixel, so each pass of the workgroup covers consecutive elements)
 // and writes its per-channel sums, bpart[wg][co]
+// The masked gradient is written twice: NHWC (gm, the input-gradient convolution's operand)
+// and row-chunked (gmT, the weight gradient's: [b][y][x / 8][c][8], rows of H x W images
+// padded with zeros to whole chunks).
 constexpr int BIAS_WG = 2048;
 __global__ __launch_bounds__(256) void k_mask_bias(const float *g, const bf16 *y, int relu, long npix,
-                                                   int cout, bf16 *gm, float *bpart) {
+                                                   int cout, int W, bf16 *gm, bf16 *gmT, float *bpart) {
   __shared__ float lds[256];
   const int lanes = 256 / cout, t = threadIdx.x, c = t % cout, q = t / cout;
   const long per = (npix + BIAS_WG - 1) / BIAS_WG, p0 = blockIdx.x * per, p1 = min(npix, p0 + per);
+  const int Wb = (W + 7) >> 3;
   float s = 0.f;
   if (q < lanes)
     for (long p = p0 + q; p < p1; p += lanes) {
       const long i = p * cout + c;
       float v = g[i];
       if (relu && bf2f(y[i]) <= 0.f) v = 0.f;
-      gm[i] = f2bf(v);
+      const bf16 hv = f2bf(v);
+      gm[i] = hv;
+      const long row = p / W;  // b * H + y
+      const int x = (int)(p - row * W);
+      gmT[((row * Wb + (x >> 3)) * cout + c) * 8 + (x & 7)] = hv;
+      if (x == W - 1)  // the row's padding (the buffer is shared by layers of other widths)
+        for (int xp = W; xp < 8 * Wb; xp++) gmT[((row * Wb + (xp >> 3)) * cout + c) * 8 + (xp & 7)] = 0;
       s += v;
     }
   lds[t] = s;
@@ -760,13 +770,12 @@ void backward() {
     const long npix = (long)B * h * w, n = npix * cout;
     // masked gradient
     hipLaunchKernelGGL(k_mask_bias, dim3(BIAS_WG), dim3(256), 0, st, N.grad[l], N.act[l], SPEC[l].relu, npix,
-                       cout, N.gm, N.bpart);
+                       cout, w, N.gm, N.gmT, N.bpart);
     hipLaunchKernelGGL(k_bias_fin, dim3((unsigned)cout), dim3(256), 0, st, N.bpart, cout, N.g + N.boff[l]);
     // weight gradient from row-chunked copies of the masked gradient and the layer input
     const bf16 *s1, *s2;
     sources(l, s1, s2);
     const dim3 tgrid((unsigned)((w + TP - 1) / TP), (unsigned)h, (unsigned)B);
-    hipLaunchKernelGGL(k_to_rc, tgrid, dim3(256), 0, st, nullptr, N.gm, 0, cout, h, w, N.gmT);
     hipLaunchKernelGGL(k_to_rc, tgrid, dim3(256), 0, st, s1, s2, SPEC[l].c1, SPEC[l].c2, h, w, N.xT);
     WgradArgs wa;
     wa.dyT = N.gmT;
