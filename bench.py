@@ -113,7 +113,8 @@ def kernel_short_name(name):
 def ingest_pmc(d):
     """rocprofv3 --pmc CSVs (one run with FETCH_SIZE, one with WRITE_SIZE; values in KiB per
     dispatch) -> profiles/pmc_traffic.json. Each counter is scaled by what it reads for the
-    runtime's copy probe kernel, whose true traffic is known (n bytes read, n written)."""
+    runtime's copy probe kernel, whose true traffic is known (n bytes read, n written).
+    Kernels measured here replace their entries; the file's other kernels are kept."""
     import csv
     import glob
     rows = []
@@ -144,6 +145,13 @@ def ingest_pmc(d):
         out["kernels"][k] = {"fetch_bytes": fm / fs, "write_bytes": wm / ws,
                              "bytes_per_launch": fm / fs + wm / ws, "dispatches": len(f)}
     p = ROOT / "profiles" / "pmc_traffic.json"
+    if p.exists():
+        old = json.loads(p.read_text())
+        merged = dict(old.get("kernels", {}))
+        merged.update(out["kernels"])
+        srcs = old.get("sources", [old["source"]] if "source" in old else [])
+        out = {"sources": srcs + [str(d)], "unit": out["unit"], "calibration": out["calibration"],
+               "kernels": dict(sorted(merged.items()))}
     p.write_text(json.dumps(out, indent=1) + "\n")
     print(f"wrote {p} ({len(out['kernels'])} kernels, calibration {cal})")
 
@@ -483,6 +491,35 @@ def tealeaf_cpu(args, cpu):
                       f"iteration"}
 
 
+def unet_training_gpu(args):
+    """U-Net training throughput alone (no TeaLeaf solves): --unet-synthetic random samples
+    at 1024^2 (4 features uniform on [-1, 1], targets normal), Adam with batch 4, one
+    warm-up epoch, then two timed epochs; TFLOP/s by the network's training flop count.
+    For profiling the training kernels in a short run."""
+    import ctypes
+    import numpy as np
+    import ops_amd
+    from ops_amd.apps.tealeaf import learned
+    L = ops_amd.app("tealeaf")
+    net, cnt = 1024, args.unet_synthetic
+    assert L.tea_unet_init(7) == 0
+    assert L.tea_unet_dataset_reserve(cnt, net, net) == 0
+    rng = np.random.default_rng(1)
+    for i in range(cnt):
+        x = rng.uniform(-1, 1, (net, net, 4)).astype(np.float32)
+        t = rng.normal(0, 1, (net, net)).astype(np.float32)
+        assert L.tea_unet_dataset_set(i, x.ctypes.data_as(ctypes.c_void_p),
+                                      t.ctypes.data_as(ctypes.c_void_p)) == 0
+    learned.train(L, 1, batch=4)
+    loss, s = learned.train(L, 2, batch=4)
+    assert L.tea_unet_free() == 0
+    flops = learned.training_flops_per_pixel() * net * net * cnt * 2
+    return {"workload": f"U-Net training, {cnt} synthetic samples at {net}^2, batch 4, 2 epochs",
+            "s": round(s, 4), "ms_per_sample": round(s * 1e3 / (2 * cnt), 3),
+            "TFLOPs": round(flops / s / 1e12, 2),
+            "frac_of_bf16_mfma_peak": round(flops / s / 1e12 / BF16_MFMA_PEAK_TFLOPS, 5)}
+
+
 def tealeaf_learned_gpu(args):
     """Learned warm start and preconditioner correction: training pairs at 1024^2 from
     seeded rectangle decks (seeds 20261203 + k for k >= 1, so the benchmark's own layout,
@@ -754,6 +791,8 @@ def main():
     ap.add_argument("--no-adi", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--unet-pairs", type=int, default=2000)
+    ap.add_argument("--unet-synthetic", type=int, default=0,
+                    help="also time U-Net training alone on this many random 1024^2 samples")
     ap.add_argument("--unet-epochs", type=int, default=4)
     ap.add_argument("--unet-steps-per-deck", type=int, default=10)
     ap.add_argument("--adi-steps", type=int, default=100)
@@ -830,6 +869,8 @@ def main():
                 tc["time_to_solution_s_extrapolated"] = round(tc["s_per_cg_iteration"] * plain, 2)
     if not args.no_learned:
         out["tealeaf_learned"] = tealeaf_learned_gpu(args)
+    if args.unet_synthetic > 0:
+        out["unet_training"] = unet_training_gpu(args)
     if not args.no_adi:
         out["adi"] = adi_gpu(args)
         if cpu:

@@ -173,6 +173,86 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   }
 }
 
+// The same convolution for layers with few input channels (cin <= 32) and output tiles
+// (NT <= 2): a workgroup computes a tile of CT_ROWS rows x 64 pixels and first copies the
+// input rows y0 - 1 .. y0 + CT_ROWS, pixels x0 - 1 .. x0 + 64 (zero outside the image) into
+// LDS, one 16-byte load per 8 channels, so the nine taps read each input value from LDS
+// instead of nine times through the vector-memory path (the bottleneck of these layers);
+// wavefront w computes row y0 + w. With more input channels the tile's LDS leaves room for
+// too few workgroups per CU, and the direct kernel above measured faster.
+constexpr int CT_ROWS = 4;
+constexpr int CT_PX = 66;                         // 64 pixels and the two neighbours
+__host__ __device__ inline int ct_pitch(int cin) { return cin + 8; }  // LDS pixel pitch (bf16)
+inline size_t ct_lds_bytes(int cin) { return (size_t)(CT_ROWS + 2) * CT_PX * ct_pitch(cin) * 2; }
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
+  extern __shared__ __align__(16) bf16 tile[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int b = blockIdx.z, y0 = blockIdx.y * CT_ROWS, x0 = blockIdx.x * 64;
+  const int H = a.H, W = a.W, c1 = a.c1, c2 = a.c2, cin = c1 + c2, cp = ct_pitch(cin);
+  const int K = 9 * cin, nkc = (K + 31) / 32;
+  const int Hc = H >> 1, Wc = W >> 1;
+  // the input tile: (row r, pixel px, channels 8 c8 .. 8 c8 + 7) per 16-byte unit
+  const int c8n = cin >> 3, units = (CT_ROWS + 2) * CT_PX * c8n;
+  for (int e = threadIdx.x; e < units; e += 256) {
+    const int c8 = e % c8n, rp = e / c8n, px = rp % CT_PX, r = rp / CT_PX;
+    const int ys = y0 - 1 + r, xs = x0 - 1 + px, ci = 8 * c8;
+    u4 v = u4{0u, 0u, 0u, 0u};
+    if (ys >= 0 && ys < H && xs >= 0 && xs < W)
+      v = ci < c1 ? *(const u4 *)(a.src1 + (((long)b * Hc + (ys >> 1)) * Wc + (xs >> 1)) * c1 + ci)
+                  : *(const u4 *)(a.src2 + (((long)b * H + ys) * W + xs) * c2 + (ci - c1));
+    *(u4 *)(tile + (r * CT_PX + px) * cp + ci) = v;
+  }
+  __syncthreads();
+  f4 acc[4][NT];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int u = 0; u < NT; u++) acc[t][u] = f4{0.f, 0.f, 0.f, 0.f};
+  const bf16 *wrow = a.w + (long)m * K;  // weight row of output channel m (+ 16 u)
+  for (int kc = 0; kc < nkc; kc++) {
+    const int k = kc * 32 + 8 * kq;
+    const bool kv = k < K;
+    const int tap = kv ? k / cin : 0, ci = k - tap * cin;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    u4 av[4], bv[NT];
+    // tile row wv + 1 + dy, pixel 16 t + m + 1 + dx: always inside the tile
+    const bf16 *arow = tile + ((wv + 1 + dy) * CT_PX + m + 1 + dx) * cp + ci;
+#pragma unroll
+    for (int t = 0; t < 4; t++) av[t] = kv ? *(const u4 *)(arow + 16 * t * cp) : u4{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int u = 0; u < NT; u++) bv[u] = kv ? *(const u4 *)(wrow + (long)16 * u * K + k) : u4{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int u = 0; u < NT; u++)
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[t]), as_bf8(bv[u]), acc[t][u], 0, 0, 0);
+  }
+  // D[pixel 4 kq + j of the tile][channel m of the N tile]
+  const int y = y0 + wv;
+  if (y >= H) return;
+  const int cout = a.cout;
+#pragma unroll
+  for (int u = 0; u < NT; u++) {
+    const int co = 16 * u + m;
+    const float bia = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int x = x0 + 16 * t + 4 * kq + j;
+        if (x >= W) continue;
+        float v = acc[t][u][j] + bia;
+        if (a.relu && v < 0.f) v = 0.f;
+        const long o = (((long)b * H + y) * W + x) * cout + co;
+        if (a.out) a.out[o] = f2bf(v);
+        if (a.out32) a.out32[o] = v;
+      }
+  }
+}
+
 // row-chunked copy of a layer input (or of the masked output gradient) for the weight
 // gradient: dst[b][y][xc][c][8] holds pixels 8 xc .. 8 xc + 7 of channel c of row y, so a
 // lane's 8 consecutive pixels of one channel are one 16-byte load and the 16 channels of an
@@ -701,6 +781,15 @@ void launch_conv(const ConvArgs &a, int B) {
   if (a.c1 % 16 || a.c2 % 16 || a.c2 <= 0 || a.cout % 16 || a.cout <= 0 || a.cout > 112 ||
       (a.c1 > 0) != (a.src1 != nullptr) || !a.src2 || !a.w || a.H <= 0 || a.W <= 0 || (a.c1 && (a.H % 2 || a.W % 2)))
     throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: convolution shape outside the kernel's limits");
+  const int cin = a.c1 + a.c2;
+  if (cin <= 32 && a.cout <= 32) {
+    // thin layers: the LDS-tiled kernel (at most 19 KB of LDS per workgroup)
+    const dim3 g((unsigned)((a.W + 63) / 64), (unsigned)((a.H + CT_ROWS - 1) / CT_ROWS), (unsigned)B);
+    const size_t lds = ct_lds_bytes(cin);
+    if (a.cout == 16) hipLaunchKernelGGL(k_conv_lds<1>, g, dim3(256), lds, stream(), a);
+    else hipLaunchKernelGGL(k_conv_lds<2>, g, dim3(256), lds, stream(), a);
+    return;
+  }
   // four wavefronts of 64 pixels per workgroup, pixels of one image row-major
   const dim3 grid((unsigned)(((long)a.H * a.W + 255) / 256), (unsigned)B);
   switch (a.cout / 16) {
