@@ -175,16 +175,6 @@ __device__ __forceinline__ void prolong_cell(const Level &F, const Level &C, int
   F.x[x + (long)y * F.s] += ALPHA_C * C.x[(x + 1) / 2 + (long)((y + 1) / 2) * C.s];
 }
 
-__global__ __launch_bounds__(NT) void k_smooth(Level L, int color, int init, const Rec *S) {
-  if (S->stopped) return;
-  unsigned bx, by, bz;
-  ops_block_coords(bx, by, bz);
-  const int t = (int)(bx * BX + threadIdx.x);
-  const int y = 1 + (int)(by * BY + threadIdx.y);
-  if (y > L.n) return;
-  smooth_pair(L, color, init, t, y);
-}
-
 __global__ __launch_bounds__(NT) void k_restrict(Level F, Level C, const Rec *S) {
   if (S->stopped) return;
   int X, Y;
@@ -197,6 +187,55 @@ __global__ __launch_bounds__(NT) void k_prolong(Level F, Level C, const Rec *S) 
   int x, y;
   if (!cell2(F.n, F.n, x, y)) return;
   prolong_cell(F, C, x, y);
+}
+
+// Both half sweeps of one Gauss-Seidel step in one launch: colour `first`, then the other
+// colour, over a tile of SW x SH cells per workgroup, from the iterate xin into L.x (a
+// different array: a neighbouring workgroup may still read xin's cells that this one
+// updates). Phase one computes colour `first` on the tile and a one-cell ring around it
+// (the ring's cells are also computed by the neighbouring workgroups, with the same inputs
+// and arithmetic, so the values agree), keeps them in LDS and stores the tile's own; phase
+// two computes the other colour of the tile from those LDS values. The result is the two
+// separate half sweeps' (with init, xin is not read: the first colour starts from zero and
+// the other colour's old values are never used); each array is read about once per step
+// instead of twice.
+constexpr int SW = 64, SH = 16;
+__global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict__ xin, int first, int init,
+                                                const Rec *S) {
+  if (S->stopped) return;
+  __shared__ vreal R[SH + 2][SW + 2];
+  unsigned bx, by, bz;
+  ops_block_coords(bx, by, bz);
+  const int x0 = 1 + (int)bx * SW, y0 = 1 + (int)by * SH;
+  const int tid = threadIdx.x + BX * threadIdx.y;
+  const int s = L.s, n = L.n;
+  for (int e = tid; e < (SH + 2) * (SW + 2); e += NT) {
+    const int lx = e % (SW + 2), ly = e / (SW + 2);
+    const int x = x0 - 1 + lx, y = y0 - 1 + ly;
+    vreal r = vreal(0);
+    if (x >= 1 && x <= n && y >= 1 && y <= n && (x + y) % 2 == first) {
+      const long i = x + (long)y * s;
+      const vreal c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
+      const vreal diag = L.m + ((c0 + c1) + (c2 + c3));
+      vreal v = L.b[i];
+      if (!init) v += (c0 * xin[i - 1] + c1 * xin[i + 1]) + (c2 * xin[i - s] + c3 * xin[i + s]);
+      r = v / diag;
+      if (lx >= 1 && lx <= SW && ly >= 1 && ly <= SH) L.x[i] = r;
+    }
+    R[ly][lx] = r;
+  }
+  __syncthreads();
+  for (int e = tid; e < SH * SW; e += NT) {
+    const int lx = 1 + e % SW, ly = 1 + e / SW;
+    const int x = x0 - 1 + lx, y = y0 - 1 + ly;
+    if (x > n || y > n || (x + y) % 2 == first) continue;
+    const long i = x + (long)y * s;
+    const vreal c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
+    const vreal diag = L.m + ((c0 + c1) + (c2 + c3));
+    vreal v = L.b[i];
+    v += (c0 * R[ly][lx - 1] + c1 * R[ly][lx + 1]) + (c2 * R[ly - 1][lx] + c3 * R[ly + 1][lx]);
+    L.x[i] = v / diag;
+  }
 }
 
 // The levels of at most FOLD_N cells per side, the coarsest included, in one workgroup of
@@ -454,6 +493,7 @@ __global__ __launch_bounds__(NT) void k_defl_add(const double *__restrict__ d, d
 struct MG {
   int n = 0;
   std::vector<Level> lv;
+  std::vector<vreal *> xb;  // second iterate array of each level above the fold (k_smooth2)
   void *mem = nullptr;
   size_t cap = 0;
   Rec *rec = nullptr, *host = nullptr;
@@ -486,23 +526,29 @@ void vcycle(hipStream_t st, vreal *x0) {
   const int L = (int)G.lv.size();
   std::vector<Level> lv = G.lv;
   lv[0].x = x0;
-  auto sgrid = [](const Level &l) { return grid2((l.n + 1) / 2, l.n); };
   int l0 = 0;  // the first folded level
   while (l0 < L - 1 && lv[l0].n > FOLD_N) l0++;
   if (L - l0 > MAXFOLD) throw OPSException(OPS_INTERNAL_ERROR, "tealeaf multigrid: too many coarse levels");
+  auto s2grid = [](const Level &l) {
+    return dim3((unsigned)((l.n + SW - 1) / SW), (unsigned)((l.n + SH - 1) / SH));
+  };
+  // levels above the fold: the down-sweep's iterate in the level's second array xb[l], the
+  // up-sweep's result in its x (x0 on level 0)
   for (int l = 0; l < l0; l++) {
-    hipLaunchKernelGGL(k_smooth, sgrid(lv[l]), blk, 0, st, lv[l], 0, 1, G.rec);
-    hipLaunchKernelGGL(k_smooth, sgrid(lv[l]), blk, 0, st, lv[l], 1, 0, G.rec);
-    hipLaunchKernelGGL(k_restrict, grid2(lv[l + 1].n, lv[l + 1].n), blk, 0, st, lv[l], lv[l + 1], G.rec);
+    Level a = lv[l];
+    a.x = G.xb[l];
+    hipLaunchKernelGGL(k_smooth2, s2grid(a), blk, 0, st, a, (const vreal *)nullptr, 0, 1, G.rec);
+    hipLaunchKernelGGL(k_restrict, grid2(lv[l + 1].n, lv[l + 1].n), blk, 0, st, a, lv[l + 1], G.rec);
   }
   Folded f;
   f.n = L - l0;
   for (int k = 0; k < f.n; k++) f.lv[k] = lv[l0 + k];
   hipLaunchKernelGGL(k_coarse, dim3(1), dim3(CT), 0, st, f, G.rec);
   for (int l = l0 - 1; l >= 0; l--) {
-    hipLaunchKernelGGL(k_prolong, grid2(lv[l].n, lv[l].n), blk, 0, st, lv[l], lv[l + 1], G.rec);
-    hipLaunchKernelGGL(k_smooth, sgrid(lv[l]), blk, 0, st, lv[l], 1, 0, G.rec);
-    hipLaunchKernelGGL(k_smooth, sgrid(lv[l]), blk, 0, st, lv[l], 0, 0, G.rec);
+    Level a = lv[l];
+    a.x = G.xb[l];
+    hipLaunchKernelGGL(k_prolong, grid2(lv[l].n, lv[l].n), blk, 0, st, a, lv[l + 1], G.rec);
+    hipLaunchKernelGGL(k_smooth2, s2grid(lv[l]), blk, 0, st, lv[l], (const vreal *)G.xb[l], 1, 0, G.rec);
   }
 }
 
@@ -530,7 +576,8 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
   size_t bytes = 4 * (((size_t)nb * 2 * sizeof(double) + 255) & ~(size_t)255);
   const size_t fine64 = (((size_t)(n + 2) * (n + 2) * sizeof(double)) + 255) & ~(size_t)255;
   bytes += fine64;  // second p buffer
-  for (int s : sizes) bytes += 4 * ((((size_t)(s + 2) * (s + 2) * sizeof(vreal)) + 255) & ~(size_t)255);
+  for (int s : sizes)
+    bytes += (s > FOLD_N ? 5 : 4) * ((((size_t)(s + 2) * (s + 2) * sizeof(vreal)) + 255) & ~(size_t)255);
   bytes += (((size_t)(n + 2) * (n + 2) * sizeof(vreal)) + 255) & ~(size_t)255;  // second z buffer
   if (bytes > G.cap || G.n != n) {
     ops_hip_device_sync();
@@ -548,12 +595,14 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
   double *part = (double *)carve(q, (size_t)nb * 2 * sizeof(double));
   double *pb = (double *)carve(q, fine64);
   G.lv.clear();
+  G.xb.clear();
   vreal mass = 1;
   for (int s : sizes) {
     const size_t lb = (size_t)(s + 2) * (s + 2) * sizeof(vreal);
     Level L{s, s + 2, mass, (vreal *)carve(q, lb), (vreal *)carve(q, lb), (vreal *)carve(q, lb),
             (vreal *)carve(q, lb)};
     G.lv.push_back(L);
+    G.xb.push_back(s > FOLD_N ? (vreal *)carve(q, lb) : nullptr);
     mass *= 4.f;
   }
   vreal *z2 = (vreal *)carve(q, (size_t)(n + 2) * (n + 2) * sizeof(vreal));
