@@ -31,6 +31,29 @@ __device__ __forceinline__ unsigned ops_xcd_remap(unsigned b, unsigned n) {
   return first + local;
 }
 
+/* The XCD remap with each XCD's run of tiles walked from its end when rev is set: a launch
+   that follows another in the opposite direction starts on the tiles that the previous one
+   touched last, whose lines the 256 MB memory-side cache still holds. */
+__device__ __forceinline__ unsigned ops_xcd_remap_dir(unsigned b, unsigned n, int rev) {
+  unsigned q = n / OPS_XCDS, r = n % OPS_XCDS;
+  unsigned x = b % OPS_XCDS, local = b / OPS_XCDS;
+  unsigned first = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  unsigned len = q + (x < r ? 1u : 0u);
+  return first + (rev ? len - 1 - local : local);
+}
+
+/* Tile coordinates of this workgroup after the XCD remap, walked forwards or backwards. */
+__device__ __forceinline__ void ops_block_coords_dir(unsigned &bx, unsigned &by, unsigned &bz, int rev) {
+  unsigned gx = gridDim.x, gy = gridDim.y;
+  unsigned n = gx * gy * gridDim.z;
+  unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  unsigned t = ops_xcd_remap_dir(b, n, rev);
+  bx = t % gx;
+  t /= gx;
+  by = t % gy;
+  bz = t / gy;
+}
+
 /* Tile coordinates of this workgroup after the XCD remap. */
 __device__ __forceinline__ void ops_block_coords(unsigned &bx, unsigned &by, unsigned &bz) {
   unsigned gx = gridDim.x, gy = gridDim.y;

@@ -78,6 +78,9 @@ typedef struct ops_hip_loop {
      has room for the shift of up to align - 1 points at the start of each row */
   int align_arg;
   int align;
+  /* set by ops_hip_loop_begin for loops without reductions: 1 on every other launch, whose
+     workgroups then walk each XCD's run of tiles from its end (ops_block_coords_dir) */
+  int rev;
 } ops_hip_loop;
 
 /* Registers a generated loop name once; returns its id for timing and tuning tables. */

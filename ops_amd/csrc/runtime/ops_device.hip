@@ -844,6 +844,14 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
   bool red_loop = false;
   for (int i = 0; i < nargs; i++)
     if (args[i].argtype == OPS_ARG_GBL && args[i].acc != OPS_READ) red_loop = true;
+  // loops without reductions alternate their tile order launch by launch (a reduction's
+  // tile order sets the order of its partial sums, so those keep theirs)
+  static const bool alternate = [] {
+    const char *e = getenv("OPS_ALTERNATE");
+    return e == nullptr || atoi(e) != 0;
+  }();
+  static unsigned launches = 0;
+  lp->rev = (alternate && !red_loop) ? (int)(launches++ & 1u) : 0;
   lp->align = 0;
   const int aa = lp->align_arg;
   if (aa >= 0 && aa < nargs && !red_loop && !narrow && vx == 1 && args[aa].argtype == OPS_ARG_DAT &&

@@ -324,6 +324,8 @@ def gen_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool = False,
         L.append("  unsigned ntiles, tg0, tg1;  // tile count and tile grid (grid-stride walk)")
     if align_anchor(loop) >= 0:
         L.append("  int al;  // elements per 128-byte line of the anchor argument (aligned x tiles), or 0")
+    if not any(a.kind == "reduce" for a in loop.args):
+        L.append("  int rev;  // walk each XCD's run of tiles from its end (every other launch)")
     for i, a in enumerate(loop.args):
         if a.kind == "dat":
             t = params[i].ctype
@@ -389,7 +391,7 @@ def batch_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool, pointwise
     sig = f"void ops_kern_{k}_batch(const ops_kp_{k} p) {{"
     assert sig in body
     body = body.replace(sig, f"void ops_kern_{k}_batch(const ops_kb_{k} B) {{")
-    pro = ("  unsigned bx, by, bz;\n  ops_block_coords(bx, by, bz);\n"
+    pro = ("  unsigned bx, by, bz;\n  ops_block_coords_dir(bx, by, bz, p.rev);\n"
            "  const unsigned tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;")
     assert pro in body
     body = body.replace(pro, "\n".join([
@@ -466,7 +468,7 @@ def kernel_body(loop: Loop, params: List[Param], ndim: int, name: str, noalias: 
         L.append("  ops_tile_coords(vt, p.ntiles, p.tg0, p.tg1, bx, by, bz);")
     else:
         L.append("  unsigned bx, by, bz;")
-        L.append("  ops_block_coords(bx, by, bz);")
+        L.append("  ops_block_coords_dir(bx, by, bz, p.rev);")
     # the workgroup's origin (x0, y0, z0) is wave-uniform, so its part of every address is
     # scalar arithmetic; the lane's part uses 24-bit multiplies (the runtime checks that the
     # x and y steps fit), where a full 32-bit multiply per argument and dimension would be
@@ -629,6 +631,8 @@ def gen_stub(loop: Loop, params: List[Param], ns: str) -> str:
     L.append("    p.st0 = lp.start[0]; p.st1 = lp.start[1]; p.st2 = lp.start[2];")
     if align_anchor(loop) >= 0:
         L.append("    p.al = lp.align;")
+    if not any(a.kind == "reduce" for a in loop.args):
+        L.append("    p.rev = lp.rev;")
     if any(a.kind == "reduce" for a in loop.args):
         L.append("    p.ntiles = lp.tiles; p.tg0 = lp.tgrid[0]; p.tg1 = lp.tgrid[1];")
     for i, a in enumerate(loop.args):
