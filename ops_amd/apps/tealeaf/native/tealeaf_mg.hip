@@ -69,10 +69,17 @@ __device__ __forceinline__ unsigned lin_block() {
   return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 }
 
+// tile of this workgroup: the XCD remap, each XCD's run of tiles walked from its end when
+// rev is set (launches alternate, so each starts on what the previous one left in the
+// memory-side cache); partials are indexed by tile, so sums do not depend on the direction
+__device__ __forceinline__ unsigned tile_id(int rev) {
+  return ops_xcd_remap_dir(lin_block(), gridDim.x * gridDim.y * gridDim.z, rev);
+}
+
 // 2-D cell of this thread (1-based interior coordinates); false outside
-__device__ __forceinline__ bool cell2(int nxc, int nyc, int &x, int &y) {
+__device__ __forceinline__ bool cell2(int nxc, int nyc, int &x, int &y, int rev = 0) {
   unsigned bx, by, bz;
-  ops_block_coords(bx, by, bz);
+  ops_block_coords_dir(bx, by, bz, rev);
   x = 1 + (int)(bx * BX + threadIdx.x);
   y = 1 + (int)(by * BY + threadIdx.y);
   return x <= nxc && y <= nyc;
@@ -82,10 +89,10 @@ dim3 grid2(int nxc, int nyc) {
   return dim3((unsigned)((nxc + BX - 1) / BX), (unsigned)((nyc + BY - 1) / BY));
 }
 
-template <typename T> __device__ __forceinline__ void write_partial(T v, T *part) {
+template <typename T> __device__ __forceinline__ void write_partial(T v, T *part, int rev) {
   __shared__ T lds[NT / OPS_WAVE];
   T s = ops_block_reduce<ops_red_inc<T>>(v, lds);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[lin_block()] = s;
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[tile_id(rev)] = s;
 }
 
 // fixed-order sum of n partials by one FT-lane workgroup (valid in thread 0)
@@ -175,17 +182,17 @@ __device__ __forceinline__ void prolong_cell(const Level &F, const Level &C, int
   F.x[x + (long)y * F.s] += ALPHA_C * C.x[(x + 1) / 2 + (long)((y + 1) / 2) * C.s];
 }
 
-__global__ __launch_bounds__(NT) void k_restrict(Level F, Level C, const Rec *S) {
+__global__ __launch_bounds__(NT) void k_restrict(Level F, Level C, const Rec *S, int rev) {
   if (S->stopped) return;
   int X, Y;
-  if (!cell2(C.n, C.n, X, Y)) return;
+  if (!cell2(C.n, C.n, X, Y, rev)) return;
   restrict_cell(F, C, X, Y);
 }
 
-__global__ __launch_bounds__(NT) void k_prolong(Level F, Level C, const Rec *S) {
+__global__ __launch_bounds__(NT) void k_prolong(Level F, Level C, const Rec *S, int rev) {
   if (S->stopped) return;
   int x, y;
-  if (!cell2(F.n, F.n, x, y)) return;
+  if (!cell2(F.n, F.n, x, y, rev)) return;
   prolong_cell(F, C, x, y);
 }
 
@@ -201,11 +208,11 @@ __global__ __launch_bounds__(NT) void k_prolong(Level F, Level C, const Rec *S) 
 // instead of twice.
 constexpr int SW = 64, SH = 16;
 __global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict__ xin, int first, int init,
-                                                const Rec *S) {
+                                                const Rec *S, int rev) {
   if (S->stopped) return;
   __shared__ vreal R[SH + 2][SW + 2];
   unsigned bx, by, bz;
-  ops_block_coords(bx, by, bz);
+  ops_block_coords_dir(bx, by, bz, rev);
   const int x0 = 1 + (int)bx * SW, y0 = 1 + (int)by * SH;
   const int tid = threadIdx.x + BX * threadIdx.y;
   const int s = L.s, n = L.n;
@@ -289,11 +296,12 @@ __global__ __launch_bounds__(CT) void k_coarse(Folded f, const Rec *S) {
 struct Fine {
   int n, s;
   double rx, ry;
+  int rev = 0;  // tile walk direction of the launch
 };
 
 __device__ __forceinline__ bool fcell(const Fine &g, long &i) {
   int x, y;
-  const bool ok = cell2(g.n, g.n, x, y);
+  const bool ok = cell2(g.n, g.n, x, y, g.rev);
   i = x + (long)y * g.s;
   return ok;
 }
@@ -308,7 +316,7 @@ __global__ __launch_bounds__(NT) void k_start(const double *__restrict__ r, vrea
     b32[i] = (vreal)ri;
     v = ri * ri;
   }
-  write_partial(v, part);
+  write_partial(v, part, g.rev);
 }
 
 __global__ __launch_bounds__(FT) void k_fold_start(const double *part, unsigned n, Rec *S, double thresh) {
@@ -339,8 +347,8 @@ __global__ __launch_bounds__(NT) void k_dots(const double *__restrict__ r, const
   const double sa = ops_block_reduce<ops_red_inc<double>>(a, lds);
   const double sb = ops_block_reduce<ops_red_inc<double>>(b, lds);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
-    part[2 * lin_block()] = sa;
-    part[2 * lin_block() + 1] = sb;
+    part[2 * tile_id(g.rev)] = sa;
+    part[2 * tile_id(g.rev) + 1] = sb;
   }
 }
 
@@ -392,7 +400,7 @@ __global__ __launch_bounds__(NT) void k_pw(const double *__restrict__ Kx, const 
     w[i] = wi;
     v = wi * pc;
   }
-  write_partial(v, part);
+  write_partial(v, part, g.rev);
 }
 
 __global__ __launch_bounds__(FT) void k_fold_pw(const double *part, unsigned n, Rec *S) {
@@ -419,7 +427,7 @@ __global__ __launch_bounds__(NT) void k_ur(double *__restrict__ u, double *__res
     b32[i] = (vreal)ri;
     v = ri * ri;
   }
-  write_partial(v, part);
+  write_partial(v, part, g.rev);
 }
 
 __global__ __launch_bounds__(FT) void k_fold_rr(const double *part, unsigned n, Rec *S) {
@@ -455,7 +463,7 @@ __global__ __launch_bounds__(NT) void k_defl_dad(const double *__restrict__ Kx, 
                       g.rx * (kx1 * de + kx0 * dw);
     v = dc * ad;
   }
-  write_partial(v, part);
+  write_partial(v, part, g.rev);
 }
 
 __global__ __launch_bounds__(FT) void k_fold_dad(const double *part, unsigned n, Rec *S) {
@@ -471,7 +479,7 @@ __global__ __launch_bounds__(NT) void k_defl_dot(const double *__restrict__ d, d
   long i;
   double v = 0.0;
   if (fcell(g, i)) v = (d[i] - m) * r[i];
-  write_partial(v, part);
+  write_partial(v, part, g.rev);
 }
 
 __global__ __launch_bounds__(FT) void k_fold_defl(const double *part, unsigned n, Rec *S) {
@@ -521,7 +529,7 @@ char *carve(char *&p, size_t bytes) {
 // V-cycle on the levels; level 0's b must hold the right-hand side, x0 receives z. Levels
 // of more than FOLD_N cells per side are swept by one launch per pass; the rest (the
 // coarsest included) by one k_coarse launch.
-void vcycle(hipStream_t st, vreal *x0) {
+void vcycle(hipStream_t st, vreal *x0, int &dir) {
   const dim3 blk(BX, BY);
   const int L = (int)G.lv.size();
   std::vector<Level> lv = G.lv;
@@ -537,8 +545,8 @@ void vcycle(hipStream_t st, vreal *x0) {
   for (int l = 0; l < l0; l++) {
     Level a = lv[l];
     a.x = G.xb[l];
-    hipLaunchKernelGGL(k_smooth2, s2grid(a), blk, 0, st, a, (const vreal *)nullptr, 0, 1, G.rec);
-    hipLaunchKernelGGL(k_restrict, grid2(lv[l + 1].n, lv[l + 1].n), blk, 0, st, a, lv[l + 1], G.rec);
+    hipLaunchKernelGGL(k_smooth2, s2grid(a), blk, 0, st, a, (const vreal *)nullptr, 0, 1, G.rec, dir++ & 1);
+    hipLaunchKernelGGL(k_restrict, grid2(lv[l + 1].n, lv[l + 1].n), blk, 0, st, a, lv[l + 1], G.rec, dir++ & 1);
   }
   Folded f;
   f.n = L - l0;
@@ -547,8 +555,9 @@ void vcycle(hipStream_t st, vreal *x0) {
   for (int l = l0 - 1; l >= 0; l--) {
     Level a = lv[l];
     a.x = G.xb[l];
-    hipLaunchKernelGGL(k_prolong, grid2(lv[l].n, lv[l].n), blk, 0, st, a, lv[l + 1], G.rec);
-    hipLaunchKernelGGL(k_smooth2, s2grid(lv[l]), blk, 0, st, lv[l], (const vreal *)G.xb[l], 1, 0, G.rec);
+    hipLaunchKernelGGL(k_prolong, grid2(lv[l].n, lv[l].n), blk, 0, st, a, lv[l + 1], G.rec, dir++ & 1);
+    hipLaunchKernelGGL(k_smooth2, s2grid(lv[l]), blk, 0, st, lv[l], (const vreal *)G.xb[l], 1, 0, G.rec,
+                       dir++ & 1);
   }
 }
 
@@ -634,17 +643,22 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
     vreal *z = zbuf[par], *zp = zbuf[1 - par];
     const double *pin = par ? pa : pb;
     double *pout = par ? pb : pa;
-    vcycle(st, z);
+    int dir = 0;  // alternating tile walks, launch by launch
+    vcycle(st, z, dir);
+    Fine gd = g;
     if (defl) {
       hipLaunchKernelGGL(k_defl_dot, gf, blk, 0, st, defl, dmean, r64, g, part, G.rec);
       hipLaunchKernelGGL(k_fold_defl, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
       hipLaunchKernelGGL(k_defl_add, gf, blk, 0, st, defl, dmean, z, g, G.rec);
     }
-    hipLaunchKernelGGL(k_dots, gf, blk, 0, st, r64, z, zp, g, part, G.rec);
+    gd.rev = dir++ & 1;
+    hipLaunchKernelGGL(k_dots, gf, blk, 0, st, r64, z, zp, gd, part, G.rec);
     hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
-    hipLaunchKernelGGL(k_pw, gf, blk, 0, st, kx, ky, z, pin, pout, w64, g, part, G.rec);
+    gd.rev = dir++ & 1;
+    hipLaunchKernelGGL(k_pw, gf, blk, 0, st, kx, ky, z, pin, pout, w64, gd, part, G.rec);
     hipLaunchKernelGGL(k_fold_pw, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
-    hipLaunchKernelGGL(k_ur, gf, blk, 0, st, u64, r64, pout, w64, G.lv[0].b, g, part, G.rec);
+    gd.rev = dir++ & 1;
+    hipLaunchKernelGGL(k_ur, gf, blk, 0, st, u64, r64, pout, w64, G.lv[0].b, gd, part, G.rec);
     hipLaunchKernelGGL(k_fold_rr, dim3(1), dim3(FT), 0, st, part, nb, G.rec);
   };
   // capture the two parities as graphs (rebuilt when the arrays move)

@@ -58,6 +58,7 @@ template <typename T> struct Batch {
   int nsys;
   int solvedim;
   unsigned group;  // partitioned solve: workgroups per XCD run (trid_group)
+  int rev;         // partitioned solve: groups in reverse order (every other launch)
 };
 
 // line l of system S: its origin (elements from the box origin) and stride
@@ -282,11 +283,17 @@ constexpr int PL = 16;  // lines per workgroup
 // straddle them (a row of a group is 128-byte aligned only where the row pitch allows),
 // and on one XCD the second reader finds them in that XCD's L2; the XCDs still work on
 // neighbouring spans at any moment. A partial last span keeps its order.
-__device__ __forceinline__ unsigned trid_group(unsigned b, unsigned n, unsigned nb) {
+// With rev set the groups are taken in reverse order (launches alternate, so each starts on
+// the lines the previous one touched last, which the memory-side cache still holds;
+// neighbouring groups stay on one XCD).
+__device__ __forceinline__ unsigned trid_group(unsigned b, unsigned n, unsigned nb, int rev) {
   const unsigned S = 8 * nb;
-  if (nb <= 1 || b >= n - n % S) return b;
-  const unsigned q = b / S, r = b % S;
-  return q * S + (r % 8) * nb + r / 8;
+  unsigned g = b;
+  if (nb > 1 && b < n - n % S) {
+    const unsigned q = b / S, r = b % S;
+    g = q * S + (r % 8) * nb + r / 8;
+  }
+  return rev ? n - 1 - g : g;
 }
 constexpr int PP = 16;  // partitions (lanes) per line
 constexpr int PM = 16;  // rows per partition at most: n <= PP * PM
@@ -312,7 +319,7 @@ __global__ __launch_bounds__(256, 3) void trid_part(const Batch<T> B) {
   T(*red)[PL][2 * PP][4] = (T(*)[PL][2 * PP][4])smem;
   T(*tile)[PXP] = (T(*)[PXP])smem;
   const int tid = threadIdx.x, li = tid % PL, p = tid / PL;
-  const long long g0 = (long long)trid_group(blockIdx.x, gridDim.x, B.group) * PL;
+  const long long g0 = (long long)trid_group(blockIdx.x, gridDim.x, B.group, B.rev) * PL;
   const int k = find_sys(B, g0);
   const Sys<T> &S = B.sys[k];
   const int sd = XT ? 0 : B.solvedim;
@@ -648,6 +655,8 @@ void launch_chunk(const Call *calls, size_t first, size_t count, T *fixed = null
   // runs of 4 neighbouring line groups per XCD: ADI's 48-block step 12.2 -> 11.8 ms (runs of
   // 2 and 8 measure the same; one XCD band per launch, ops_xcd_remap, 12.5 ms)
   B.group = 4;
+  static unsigned launches = 0;
+  B.rev = (int)(launches++ & 1u);
   const int sd = B.solvedim;
   bool part = fixed == nullptr && !use_thomas();
   for (size_t k = first; k < first + count; k++) {

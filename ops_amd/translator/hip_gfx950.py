@@ -401,7 +401,7 @@ def batch_kernel(loop: Loop, params: List[Param], ndim: int, v1: bool, pointwise
         "  const unsigned bdx = B.b0[j], bdy = B.b1[j], bdz = B.b2[j];",
         "  const unsigned lb = blockIdx.x - B.first[j];",
         "  if (lb >= B.nb[j]) return;",
-        "  const unsigned lt = ops_xcd_remap(lb, B.nb[j]);",
+        "  const unsigned lt = ops_xcd_remap_dir(lb, B.nb[j], p.rev);",
         "  const unsigned bx = lt % B.g0[j], by = (lt / B.g0[j]) % B.g1[j], bz = lt / (B.g0[j] * B.g1[j]);",
         "  const unsigned tx = threadIdx.x % bdx, ty = (threadIdx.x / bdx) % bdy, tz = threadIdx.x / (bdx * bdy);",
         "  if (tz >= bdz) return;"]))
