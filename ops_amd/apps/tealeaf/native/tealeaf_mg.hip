@@ -189,13 +189,6 @@ __global__ __launch_bounds__(NT) void k_restrict(Level F, Level C, const Rec *S,
   restrict_cell(F, C, X, Y);
 }
 
-__global__ __launch_bounds__(NT) void k_prolong(Level F, Level C, const Rec *S, int rev) {
-  if (S->stopped) return;
-  int x, y;
-  if (!cell2(F.n, F.n, x, y, rev)) return;
-  prolong_cell(F, C, x, y);
-}
-
 // Both half sweeps of one Gauss-Seidel step in one launch: colour `first`, then the other
 // colour, over a tile of SW x SH cells per workgroup, from the iterate xin into L.x (a
 // different array: a neighbouring workgroup may still read xin's cells that this one
@@ -207,8 +200,12 @@ __global__ __launch_bounds__(NT) void k_prolong(Level F, Level C, const Rec *S, 
 // the other colour's old values are never used); each array is read about once per step
 // instead of twice.
 constexpr int SW = 64, SH = 16;
-__global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict__ xin, int first, int init,
-                                                const Rec *S, int rev) {
+// With xc set (the up-sweep), the iterate is xin plus the coarse correction prolonged on
+// the fly: x_j = xin_j + ALPHA_C * xc(parent of j) for interior j, the arithmetic of
+// k_prolong, so the prolonged field is neither written nor read back (only the other
+// colour's values of it are ever used: a Gauss-Seidel update does not read its own cell).
+__global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict__ xin, const vreal *__restrict__ xc,
+                                                int cs, int first, int init, const Rec *S, int rev) {
   if (S->stopped) return;
   __shared__ vreal R[SH + 2][SW + 2];
   unsigned bx, by, bz;
@@ -225,7 +222,17 @@ __global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict
       const vreal c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
       const vreal diag = L.m + ((c0 + c1) + (c2 + c3));
       vreal v = L.b[i];
-      if (!init) v += (c0 * xin[i - 1] + c1 * xin[i + 1]) + (c2 * xin[i - s] + c3 * xin[i + s]);
+      if (!init) {
+        vreal xw = xin[i - 1], xe = xin[i + 1], xs = xin[i - s], xn = xin[i + s];
+        if (xc) {
+          // the neighbours' prolonged values (halo cells get no correction)
+          if (x > 1) xw = xw + ALPHA_C * xc[x / 2 + (long)((y + 1) / 2) * cs];
+          if (x < n) xe = xe + ALPHA_C * xc[(x + 2) / 2 + (long)((y + 1) / 2) * cs];
+          if (y > 1) xs = xs + ALPHA_C * xc[(x + 1) / 2 + (long)(y / 2) * cs];
+          if (y < n) xn = xn + ALPHA_C * xc[(x + 1) / 2 + (long)((y + 2) / 2) * cs];
+        }
+        v += (c0 * xw + c1 * xe) + (c2 * xs + c3 * xn);
+      }
       r = v / diag;
       if (lx >= 1 && lx <= SW && ly >= 1 && ly <= SH) L.x[i] = r;
     }
@@ -545,20 +552,18 @@ void vcycle(hipStream_t st, vreal *x0, int &dir) {
   for (int l = 0; l < l0; l++) {
     Level a = lv[l];
     a.x = G.xb[l];
-    hipLaunchKernelGGL(k_smooth2, s2grid(a), blk, 0, st, a, (const vreal *)nullptr, 0, 1, G.rec, dir++ & 1);
+    hipLaunchKernelGGL(k_smooth2, s2grid(a), blk, 0, st, a, (const vreal *)nullptr, (const vreal *)nullptr, 0, 0,
+                       1, G.rec, dir++ & 1);
     hipLaunchKernelGGL(k_restrict, grid2(lv[l + 1].n, lv[l + 1].n), blk, 0, st, a, lv[l + 1], G.rec, dir++ & 1);
   }
   Folded f;
   f.n = L - l0;
   for (int k = 0; k < f.n; k++) f.lv[k] = lv[l0 + k];
   hipLaunchKernelGGL(k_coarse, dim3(1), dim3(CT), 0, st, f, G.rec);
-  for (int l = l0 - 1; l >= 0; l--) {
-    Level a = lv[l];
-    a.x = G.xb[l];
-    hipLaunchKernelGGL(k_prolong, grid2(lv[l].n, lv[l].n), blk, 0, st, a, lv[l + 1], G.rec, dir++ & 1);
-    hipLaunchKernelGGL(k_smooth2, s2grid(lv[l]), blk, 0, st, lv[l], (const vreal *)G.xb[l], 1, 0, G.rec,
-                       dir++ & 1);
-  }
+  // up-sweep: the coarse correction prolonged inside the smoothing (k_smooth2 with xc)
+  for (int l = l0 - 1; l >= 0; l--)
+    hipLaunchKernelGGL(k_smooth2, s2grid(lv[l]), blk, 0, st, lv[l], (const vreal *)G.xb[l],
+                       (const vreal *)lv[l + 1].x, lv[l + 1].s, 1, 0, G.rec, dir++ & 1);
 }
 
 }  // namespace
