@@ -92,8 +92,9 @@ struct ConvArgs {
   const bf16 *w;     // [cout][9][c1 + c2]
   const float *bias; // [cout] or null
   bf16 *out;         // [H][W][cout] or null
-  float *out32;      // [H][W][cout] or null
+  float *out32;      // [H][W][cout] (o32c0 = 0) or [H][W], channel 0 only (o32c0 = 1), or null
   int relu;
+  int o32c0 = 0;
 };
 
 // forward convolution (and the input-gradient convolution with transformed weights) as an
@@ -168,7 +169,8 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         if (a.relu && v < 0.f) v = 0.f;
         const long o = ((long)b * npix + q) * cout + co;
         if (a.out) a.out[o] = f2bf(v);
-        if (a.out32) a.out32[o] = v;
+        if (a.out32 && !a.o32c0) a.out32[o] = v;
+        if (a.out32 && a.o32c0 && co == 0) a.out32[(long)b * npix + q] = v;
       }
   }
 }
@@ -248,39 +250,45 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
         if (a.relu && v < 0.f) v = 0.f;
         const long o = (((long)b * H + y) * W + x) * cout + co;
         if (a.out) a.out[o] = f2bf(v);
-        if (a.out32) a.out32[o] = v;
+        if (a.out32 && !a.o32c0) a.out32[o] = v;
+        if (a.out32 && a.o32c0 && co == 0) a.out32[((long)b * H + y) * W + x] = v;
       }
   }
 }
 
-// row-chunked copy of a layer input (or of the masked output gradient) for the weight
-// gradient: dst[b][y][xc][c][8] holds pixels 8 xc .. 8 xc + 7 of channel c of row y, so a
-// lane's 8 consecutive pixels of one channel are one 16-byte load and the 16 channels of an
-// operand tile are 256 contiguous bytes; rows are padded with zeros to whole chunks
-// (Wb = ceil(W / 8)). Input: the concatenation [up(src1), src2] (src1 read at (y/2, x/2)).
-// A workgroup moves 64 pixels of one row through LDS, so both the NHWC reads and the
-// row-chunked writes are contiguous.
+// row-chunked copy of a layer input for the weight gradient: dst[b][y][xc][c][8] holds
+// pixels 8 xc .. 8 xc + 7 of channel c of row y, so a lane's 8 consecutive pixels of one
+// channel are one 16-byte load and the 16 channels of an operand tile are 256 contiguous
+// bytes; rows are padded with zeros to whole chunks (Wb = ceil(W / 8)). Input: the
+// concatenation [up(src1), src2] (src1 read at (y/2, x/2)). A workgroup moves 64 pixels of
+// one row through LDS: 16-byte loads of 8 channels of a pixel into a pixel-major tile, then
+// each 16-byte store gathers 8 pixels of one channel from it.
 constexpr int TP = 64;
+constexpr int RC_MAXC = 112;  // the widest layer input
 __global__ __launch_bounds__(256) void k_to_rc(const bf16 *src1, const bf16 *src2, int c1, int c2, int H, int W,
                                                bf16 *dst) {
-  __shared__ bf16 t[112][TP + 2];
-  const int C = c1 + c2, x0 = blockIdx.x * TP, y = blockIdx.y, b = blockIdx.z;
+  __shared__ __align__(16) bf16 t[TP * RC_MAXC];  // t[px * C + c]
+  const int C = c1 + c2, G = C >> 3, x0 = blockIdx.x * TP, y = blockIdx.y, b = blockIdx.z;
   const int Hc = H >> 1, Wc = W >> 1, Wb = (W + 7) >> 3;
-  for (int e = threadIdx.x; e < TP * C; e += 256) {
-    const int px = e / C, c = e % C, x = x0 + px;
-    bf16 v = 0;
+  for (int e = threadIdx.x; e < TP * G; e += 256) {
+    const int px = e / G, c = 8 * (e - px * G), x = x0 + px;
+    u4 v = {0u, 0u, 0u, 0u};
     if (x < W)
-      v = c < c1 ? src1[(((long)b * Hc + (y >> 1)) * Wc + (x >> 1)) * c1 + c]
-                 : src2[(((long)b * H + y) * W + x) * c2 + (c - c1)];
-    t[c][px] = v;
+      v = c < c1 ? *(const u4 *)(src1 + (((long)b * Hc + (y >> 1)) * Wc + (x >> 1)) * c1 + c)
+                 : *(const u4 *)(src2 + (((long)b * H + y) * W + x) * c2 + (c - c1));
+    *(u4 *)(t + px * C + c) = v;
   }
   __syncthreads();
-  // chunks x0 / 8 .. of this row: one contiguous block of nch * C * 8 values
+  // chunks x0 / 8 .. of this row: one contiguous block of nch * C chunks of 8 values
   const int xc0 = x0 >> 3, nch = min(TP / 8, Wb - xc0);
-  bf16 *out = dst + (((long)b * H + y) * Wb + xc0) * C * 8;
-  for (int e = threadIdx.x; e < nch * C * 8; e += 256) {
-    const int j = e & 7, c = (e >> 3) % C, xl = (e >> 3) / C;
-    out[e] = t[c][xl * 8 + j];
+  u4 *out = (u4 *)(dst + (((long)b * H + y) * Wb + xc0) * C * 8);
+  for (int e = threadIdx.x; e < nch * C; e += 256) {
+    const int xl = e / C, c = e - xl * C;
+    const bf16 *col = t + xl * 8 * C + c;
+    u4 v;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = (unsigned)col[2 * k * C] | ((unsigned)col[(2 * k + 1) * C] << 16);
+    out[e] = v;
   }
 }
 
@@ -474,41 +482,80 @@ __global__ __launch_bounds__(256) void k_wgrad_sum(const float *part, int nbands
   dw[((long)co * 9 + tap) * cin + ci] = s;
 }
 
-// masked gradient in BF16 for the MFMA operands and the first pass of the bias gradient:
-// each of BIAS_WG workgroups walks its slice of pixels (thread t owns channel t % cout and
-// every (256 / cout)-th pixel, so each pass of the workgroup covers consecutive elements)
-// and writes its per-channel sums, bpart[wg][co]
+// Masked output gradient of a layer (ReLU: zero where the output was not positive), in
+// BF16, and the first pass of the bias gradient (per-channel sums of the masked gradient).
 // The masked gradient is written twice: NHWC (gm, the input-gradient convolution's operand)
 // and row-chunked (gmT, the weight gradient's: [b][y][x / 8][c][8], rows of H x W images
-// padded with zeros to whole chunks).
+// padded with zeros to whole chunks). Each of BIAS_WG workgroups walks tiles of 64 pixels
+// of a row: lanes take 4 channels of a pixel (16-byte loads of g, 8-byte loads of y and
+// stores of gm, all contiguous across the wavefront) and put the masked values into a
+// pixel-major LDS tile, from which each 16-byte gmT store gathers 8 pixels of one channel.
+// A lane keeps one channel quad, so its sums and the workgroup's per-channel partials,
+// bpart[wg][co], are added in a fixed order. With g_c0 set, g holds channel 0 alone
+// ([pixels]; the other channels are zero): the output layer's gradient.
 constexpr int BIAS_WG = 2048;
-__global__ __launch_bounds__(256) void k_mask_bias(const float *g, const bf16 *y, int relu, long npix,
+constexpr int MB_PX = 64;
+__global__ __launch_bounds__(256) void k_mask_bias(const float *g, int g_c0, const bf16 *y, int relu, long rows,
                                                    int cout, int W, bf16 *gm, bf16 *gmT, float *bpart) {
-  __shared__ float lds[256];
-  const int lanes = 256 / cout, t = threadIdx.x, c = t % cout, q = t / cout;
-  const long per = (npix + BIAS_WG - 1) / BIAS_WG, p0 = blockIdx.x * per, p1 = min(npix, p0 + per);
-  const int Wb = (W + 7) >> 3;
-  float s = 0.f;
-  if (q < lanes)
-    for (long p = p0 + q; p < p1; p += lanes) {
-      const long i = p * cout + c;
-      float v = g[i];
-      if (relu && bf2f(y[i]) <= 0.f) v = 0.f;
-      const bf16 hv = f2bf(v);
-      gm[i] = hv;
-      const long row = p / W;  // b * H + y
-      const int x = (int)(p - row * W);
-      gmT[((row * Wb + (x >> 3)) * cout + c) * 8 + (x & 7)] = hv;
-      if (x == W - 1)  // the row's padding (the buffer is shared by layers of other widths)
-        for (int xp = W; xp < 8 * Wb; xp++) gmT[((row * Wb + (xp >> 3)) * cout + c) * 8 + (xp & 7)] = 0;
-      s += v;
+  __shared__ __align__(16) bf16 t[MB_PX * 64];  // t[px * cout + c], cout <= 64
+  __shared__ float lds[256 * 4];
+  const int Q = cout >> 2, lanes = 256 - 256 % Q, tid = threadIdx.x;
+  const int Wb = (W + 7) >> 3, tpr = (W + MB_PX - 1) / MB_PX;
+  const long ntiles = rows * tpr;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long row = tile / tpr;
+    const int x0 = (int)(tile - row * tpr) * MB_PX, npx = min(MB_PX, W - x0);
+    const int nchk = min(MB_PX / 8, Wb - (x0 >> 3));  // chunks, the row's padding included
+    if (tid < lanes)
+      for (int e = tid; e < nchk * 8 * Q; e += lanes) {
+        const int px = e / Q, c = 4 * (e - px * Q);
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        if (px < npx) {
+          const long p = row * W + x0 + px, i = p * cout + c;
+          if (!g_c0) {
+            const f4 a = *(const f4 *)(g + i);
+            v[0] = a[0];
+            v[1] = a[1];
+            v[2] = a[2];
+            v[3] = a[3];
+          } else if (c == 0) {
+            v[0] = g[p];
+          }
+          if (relu) {
+            const uint2 yy = *(const uint2 *)(y + i);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+              if (bf2f((bf16)((k < 2 ? yy.x : yy.y) >> (16 * (k & 1)))) <= 0.f) v[k] = 0.f;
+          }
+#pragma unroll
+          for (int k = 0; k < 4; k++) s[k] += v[k];
+        }
+        const uint2 o = {(unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16),
+                         (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16)};
+        if (px < npx) *(uint2 *)(gm + (row * W + x0 + px) * cout + c) = o;
+        *(uint2 *)(t + px * cout + c) = o;
+      }
+    __syncthreads();
+    u4 *out = (u4 *)(gmT + (row * Wb + (x0 >> 3)) * cout * 8);
+    for (int e = tid; e < nchk * cout; e += 256) {
+      const int xl = e / cout, c = e - xl * cout;
+      const bf16 *col = t + xl * 8 * cout + c;
+      u4 v;
+#pragma unroll
+      for (int k = 0; k < 4; k++) v[k] = (unsigned)col[2 * k * cout] | ((unsigned)col[(2 * k + 1) * cout] << 16);
+      out[e] = v;
     }
-  lds[t] = s;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) lds[tid * 4 + k] = s[k];
   __syncthreads();
-  if (t < cout) {
+  if (tid < cout) {
+    const int q = tid >> 2, k = tid & 3;
     float v = 0.f;
-    for (int k = 0; k < lanes; k++) v += lds[k * cout + t];
-    bpart[blockIdx.x * cout + t] = v;
+    for (int l = q; l < lanes; l += Q) v += lds[l * 4 + k];
+    bpart[blockIdx.x * cout + tid] = v;
   }
 }
 
@@ -545,35 +592,45 @@ __global__ __launch_bounds__(256) void k_pool(const bf16 *in, int B, int H, int 
   out[i] = f2bf(0.25f * s);
 }
 
-// adjoint of the pooling: gfine += 0.25 * gcoarse(parent)
+// adjoint of the pooling: gfine += 0.25 * gcoarse(parent), four channels per lane (C is a
+// multiple of 4)
 __global__ __launch_bounds__(256) void k_unpool_add(const float *gc, int B, int H, int W, int C, float *gf) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  const long n = (long)B * H * W * C;
+  const int C4 = C >> 2;
+  const long n = (long)B * H * W * C4;
   if (i >= n) return;
-  const int c = (int)(i % C);
-  const long p = i / C;
+  const int c = (int)(i % C4) * 4;
+  const long p = i / C4;
   const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((long)W * H));
-  gf[i] += 0.25f * gc[(((long)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+  f4 *d = (f4 *)(gf + p * C + c);
+  *d = *d + 0.25f * *(const f4 *)(gc + (((long)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c);
 }
 
-// split the input gradient of a decoder layer ([up(src1), src2] at H x W):
-// gsrc2 += g[..., c1:], gsrc1(coarse) = sum of the 2 x 2 children of g[..., :c1]
+// split the input gradient of a decoder layer ([up(src1), src2] at H x W): gsrc2 =
+// g[..., c1:], gsrc1(coarse) = sum of the 2 x 2 children of g[..., :c1]. A lane takes 4
+// channels of one coarse pixel's four children, so every fine pixel is read once and a
+// wavefront's loads cover contiguous runs of two rows. Both results are stored, not added:
+// the decoder layer is the first (skip source) or only (coarse source) layer whose input
+// gradient reaches them.
 __global__ __launch_bounds__(256) void k_split_up(const float *g, int B, int H, int W, int c1, int c2,
                                                   float *g1, float *g2) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  const int cin = c1 + c2;
-  const long n2 = (long)B * H * W * c2, n1 = (long)B * (H / 2) * (W / 2) * c1;
-  if (i < n2) {
-    const int c = (int)(i % c2);
-    const long p = i / c2;
-    g2[i] += g[p * cin + c1 + c];
-  }
-  if (i < n1) {
-    const int c = (int)(i % c1);
-    const long p = i / c1;
-    const int X = (int)(p % (W / 2)), Y = (int)((p / (W / 2)) % (H / 2)), b = (int)(p / ((long)(W / 2) * (H / 2)));
-    const long f = (((long)b * H + 2 * Y) * W + 2 * X) * cin + c;
-    g1[i] = (g[f] + g[f + cin]) + (g[f + (long)W * cin] + g[f + (long)W * cin + cin]);
+  const int cin = c1 + c2, Q = cin >> 2, Hc = H >> 1, Wc = W >> 1;
+  if (i >= (long)B * Hc * Wc * Q) return;
+  const int c = (int)(i % Q) * 4;
+  const long P = i / Q;
+  const int X = (int)(P % Wc), Y = (int)((P / Wc) % Hc), b = (int)(P / ((long)Wc * Hc));
+  const long p0 = ((long)b * H + 2 * Y) * W + 2 * X, p2 = p0 + W;
+  const f4 a0 = *(const f4 *)(g + p0 * cin + c), a1 = *(const f4 *)(g + (p0 + 1) * cin + c);
+  const f4 a2 = *(const f4 *)(g + p2 * cin + c), a3 = *(const f4 *)(g + (p2 + 1) * cin + c);
+  if (c < c1) {
+    *(f4 *)(g1 + P * c1 + c) = (a0 + a1) + (a2 + a3);
+  } else {
+    const int d = c - c1;
+    *(f4 *)(g2 + p0 * c2 + d) = a0;
+    *(f4 *)(g2 + (p0 + 1) * c2 + d) = a1;
+    *(f4 *)(g2 + p2 * c2 + d) = a2;
+    *(f4 *)(g2 + (p2 + 1) * c2 + d) = a3;
   }
 }
 
@@ -584,10 +641,11 @@ __global__ __launch_bounds__(256) void k_split_up(const float *g, int B, int H, 
 // on, which weights high-frequency noise by the operator's large eigenvalues).
 // coef[pixel] = {rx Kx, ry Ky} of the faces on the low side of the pixel (faces on the
 // mesh boundary carry zero); norm[2 b] = t^T t, norm[2 b + 1] = t^T C t. Gradient
-// 2 (I + beta C) e / (norm_b * batch) on channel 0, zero on channels 1..15; partial losses
-// per workgroup.
+// 2 (I + beta C) e / (norm_b * batch) on channel 0 (zero on the others); partial losses per
+// workgroup. The prediction and the gradient are the output layer's channel 0 alone
+// ([B][H][W]).
 __global__ __launch_bounds__(256) void k_loss(const float *pred, const float *target, const float *coef,
-                                              const double *norm, float beta, int B, int H, int W, int cout,
+                                              const double *norm, float beta, int B, int H, int W,
                                               float *grad, float *part) {
   __shared__ float lds[4];
   const long npix = (long)B * H * W;
@@ -595,7 +653,7 @@ __global__ __launch_bounds__(256) void k_loss(const float *pred, const float *ta
   float l = 0.f;
   if (p < npix) {
     const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((long)W * H));
-    auto err = [&](long q) { return pred[q * cout] - target[q]; };
+    auto err = [&](long q) { return pred[q] - target[q]; };
     const float e = err(p);
     float ce = 0.f;
     if (beta != 0.f) {
@@ -607,8 +665,7 @@ __global__ __launch_bounds__(256) void k_loss(const float *pred, const float *ta
     const float me = e + beta * ce;
     const float inv = (float)(1.0 / (norm[2 * b] + (double)beta * norm[2 * b + 1]));
     l = e * me * inv;
-    grad[p * cout] = 2.f * me * inv / (float)B;
-    for (int c = 1; c < cout; c++) grad[p * cout + c] = 0.f;
+    grad[p] = 2.f * me * inv / (float)B;
   }
   l = ops_block_reduce<ops_red_inc<float>>(l, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = l;
@@ -629,19 +686,9 @@ __global__ __launch_bounds__(256) void k_adam(float *w, const float *g, float *m
   if (wb) wb[i] = f2bf(wn);
 }
 
-__global__ __launch_bounds__(256) void k_add(const float *a, long n, float *b) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) b[i] += a[i];
-}
-
 __global__ __launch_bounds__(256) void k_to_bf16(const float *a, bf16 *b, long n) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i < n) b[i] = f2bf(a[i]);
-}
-
-__global__ __launch_bounds__(256) void k_fill(float *a, long n, float v) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) a[i] = v;
 }
 
 unsigned g1d(long n) { return (unsigned)((n + 255) / 256); }
@@ -660,11 +707,11 @@ struct Net {
   bf16 *act[NL] = {};   // layer outputs (BF16)
   bf16 *pooled[4] = {}; // pooled inputs of enc1..enc3 (index level)
   bf16 *in = nullptr;   // input, 16 channels
-  float *out32 = nullptr, *target = nullptr, *coef = nullptr;
+  float *out32 = nullptr;  // the output layer's channel 0 (FP32), [B][H][W]
+  float *target = nullptr, *coef = nullptr;
   double *norm = nullptr;  // per image of the batch, {t^T t, t^T C t}
   float *grad[NL] = {}; // gradients w.r.t. layer outputs (FP32)
   float *gin = nullptr; // input gradient of a layer (largest)
-  float *gpool = nullptr;
   bf16 *gm = nullptr;   // masked output gradient, BF16
   bf16 *gmT = nullptr;  // the same, row-chunked (weight gradient)
   bf16 *xT = nullptr;   // a layer's input, row-chunked (weight gradient)
@@ -705,7 +752,6 @@ void free_acts() {
   dfree(N.coef);
   dfree(N.norm);
   dfree(N.gin);
-  dfree(N.gpool);
   dfree(N.gm);
   dfree(N.gmT);
   dfree(N.xT);
@@ -728,7 +774,8 @@ void ensure_acts(int B, int H, int W) {
   long maxg = 0, maxrc = 0;
   for (int l = 0; l < NL; l++) {
     N.act[l] = dalloc<bf16>(pix(SPEC[l].level) * SPEC[l].cout);
-    N.grad[l] = dalloc<float>(pix(SPEC[l].level) * SPEC[l].cout);
+    // the output layer's gradient: channel 0 alone (the prediction's)
+    N.grad[l] = dalloc<float>(pix(SPEC[l].level) * (l == NL - 1 ? 1 : SPEC[l].cout));
     maxg = std::max(maxg, pix(SPEC[l].level) * (long)std::max(cin_of(l), SPEC[l].cout));
     // row-chunked copies: rows padded to whole 8-pixel chunks
     const long rows = (long)B * (H >> SPEC[l].level), wb = ((W >> SPEC[l].level) + 7) / 8;
@@ -736,12 +783,11 @@ void ensure_acts(int B, int H, int W) {
   }
   for (int lv = 1; lv < 4; lv++) N.pooled[lv] = dalloc<bf16>(pix(lv) * SPEC[lv].c2);
   N.in = dalloc<bf16>(pix(0) * 16);
-  N.out32 = dalloc<float>(pix(0) * 16);
+  N.out32 = dalloc<float>(pix(0));
   N.target = dalloc<float>(pix(0));
   N.coef = dalloc<float>(pix(0) * 2);
   N.norm = dalloc<double>(2 * B);
   N.gin = dalloc<float>(maxg);
-  N.gpool = dalloc<float>(maxg);
   N.gm = dalloc<bf16>(maxg);
   N.gmT = dalloc<bf16>(maxrc);
   N.xT = dalloc<bf16>(maxrc);
@@ -839,31 +885,33 @@ void forward() {
     a.bias = N.p + N.boff[l];
     a.out = N.act[l];
     a.out32 = (l == NL - 1) ? N.out32 : nullptr;
+    a.o32c0 = 1;  // the prediction is channel 0 of the output layer
     a.relu = SPEC[l].relu;
     launch_conv(a, B);
   }
   ops_hip_check_launch("tea_unet forward");
 }
 
-// gradients of every parameter from N.grad[NL-1] (output gradient)
+// gradients of every parameter from N.grad[NL-1] (output gradient). The layers run from
+// the output down, and every layer output's gradient is first stored, never cleared and
+// added to: the output layer's input gradient is dec0's; a decoder layer's stores its
+// coarse source's (that layer's only consumer) and its skip source's (the encoder layer
+// whose gradient the pooled path of the next encoder level adds to later in the pass).
 void backward() {
   hipStream_t st = stream();
   const int B = N.B, H = N.H, W = N.W;
-  // zero the gradients that accumulate (skip connections, pooled paths)
-  for (int l = 0; l < NL - 1; l++) {
-    const long n = (long)B * (H >> SPEC[l].level) * (W >> SPEC[l].level) * SPEC[l].cout;
-    hipLaunchKernelGGL(k_fill, dim3(g1d(n)), dim3(256), 0, st, N.grad[l], n, 0.f);
-  }
   for (int l = NL - 1; l >= 0; l--) {
     const int lv = SPEC[l].level, h = H >> lv, w = W >> lv, cout = SPEC[l].cout, cin = cin_of(l);
     const long npix = (long)B * h * w, n = npix * cout;
-    // masked gradient
-    hipLaunchKernelGGL(k_mask_bias, dim3(BIAS_WG), dim3(256), 0, st, N.grad[l], N.act[l], SPEC[l].relu, npix,
-                       cout, w, N.gm, N.gmT, N.bpart);
+    // masked gradient (the kernel's LDS tile holds at most 64 channels)
+    if (cout > 64 || cout % 8) throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: layer output too wide");
+    hipLaunchKernelGGL(k_mask_bias, dim3(BIAS_WG), dim3(256), 0, st, N.grad[l], l == NL - 1 ? 1 : 0, N.act[l],
+                       SPEC[l].relu, (long)B * h, cout, w, N.gm, N.gmT, N.bpart);
     hipLaunchKernelGGL(k_bias_fin, dim3((unsigned)cout), dim3(256), 0, st, N.bpart, cout, N.g + N.boff[l]);
     // weight gradient from row-chunked copies of the masked gradient and the layer input
     const bf16 *s1, *s2;
     sources(l, s1, s2);
+    if (cin > RC_MAXC || cin % 8) throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: layer input too wide");
     const dim3 tgrid((unsigned)((w + TP - 1) / TP), (unsigned)h, (unsigned)B);
     hipLaunchKernelGGL(k_to_rc, tgrid, dim3(256), 0, st, s1, s2, SPEC[l].c1, SPEC[l].c2, h, w, N.xT);
     WgradArgs wa;
@@ -917,26 +965,21 @@ void backward() {
     a.w = N.wt;
     a.bias = nullptr;
     a.out = nullptr;
-    a.out32 = N.gin;
+    a.out32 = l == NL - 1 ? N.grad[NL - 2] : N.gin;  // the output layer's input is dec0's output
     a.relu = 0;
     launch_conv(a, B);
     // route it to the layers that produced the input
     if (l >= 1 && l <= 3) {
       // input = pool(act[l-1]): back through the average
-      const long nf = (long)B * (2 * h) * (2 * w) * SPEC[l].c2;
+      const long nf = (long)B * (2 * h) * (2 * w) * SPEC[l].c2 / 4;
       hipLaunchKernelGGL(k_unpool_add, dim3(g1d(nf)), dim3(256), 0, st, N.gin, B, 2 * h, 2 * w, SPEC[l].c2,
                          N.grad[l - 1]);
     } else if (l >= 4 && l <= 6) {
       // input = [up(act[s1]), act[s2]]
       const int s1 = (l == 4) ? 3 : l - 1, s2 = (l == 4) ? 2 : (l == 5 ? 1 : 0);
-      const long n2 = npix * SPEC[l].c2;
-      const long n1 = (long)B * (h / 2) * (w / 2) * SPEC[l].c1;
-      hipLaunchKernelGGL(k_split_up, dim3(g1d(std::max(n1, n2))), dim3(256), 0, st, N.gin, B, h, w, SPEC[l].c1,
-                         SPEC[l].c2, N.gpool, N.grad[s2]);
-      hipLaunchKernelGGL(k_add, dim3(g1d(n1)), dim3(256), 0, st, N.gpool, n1, N.grad[s1]);
-    } else {
-      // output layer: input = act[6]
-      hipLaunchKernelGGL(k_add, dim3(g1d(npix * cin)), dim3(256), 0, st, N.gin, npix * cin, N.grad[6]);
+      const long nq = (long)B * (h / 2) * (w / 2) * (cin / 4);
+      hipLaunchKernelGGL(k_split_up, dim3(g1d(nq)), dim3(256), 0, st, N.gin, B, h, w, SPEC[l].c1, SPEC[l].c2,
+                         N.grad[s1], N.grad[s2]);
     }
   }
   ops_hip_check_launch("tea_unet backward");
@@ -1375,7 +1418,7 @@ void tea_unet_guess_dats(ops_dat rho, ops_dat u0, ops_dat r, ops_dat Kx, ops_dat
   forward();
   hipStream_t st = stream();
   const double *kx = dev_of(Kx), *ky = dev_of(Ky);
-  hipLaunchKernelGGL(k_apply, dim3(g1d((long)n * n)), dim3(256), 0, st, m, N.out32, 16, SS.stats, SS.pf);
+  hipLaunchKernelGGL(k_apply, dim3(g1d((long)n * n)), dim3(256), 0, st, m, N.out32, 1, SS.stats, SS.pf);
   hipLaunchKernelGGL(k_galerkin_part, dim3(STAT_BLOCKS), dim3(256), 0, st, m, SS.pf, kx, ky, rx, ry, SS.part);
   hipLaunchKernelGGL(k_galerkin_fin, dim3(1), dim3(256), 0, st, SS.part, STAT_BLOCKS, (long)n * n, SS.stats);
   ops_memspace ms = OPS_DEVICE;
@@ -1476,10 +1519,8 @@ extern "C" int tea_unet_forward_host(int B, int H, int W, const float *in, float
     CK(hipMemcpy(tmp, in, npix * 16 * sizeof(float), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_to_bf16, dim3(g1d(npix * 16)), dim3(256), 0, stream(), tmp, N.in, npix * 16);
     forward();
-    std::vector<float> o(npix * 16);
     sync();
-    CK(hipMemcpy(o.data(), N.out32, npix * 16 * sizeof(float), hipMemcpyDeviceToHost));
-    for (long p = 0; p < npix; p++) out[p] = o[p * 16];
+    CK(hipMemcpy(out, N.out32, npix * sizeof(float), hipMemcpyDeviceToHost));
   });
 }
 
@@ -1494,10 +1535,8 @@ extern "C" int tea_unet_backward_host(int B, int H, int W, const float *in, cons
     CK(hipMemcpy(tmp, in, npix * 16 * sizeof(float), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_to_bf16, dim3(g1d(npix * 16)), dim3(256), 0, stream(), tmp, N.in, npix * 16);
     forward();
-    std::vector<float> g(npix * 16, 0.f);
-    for (long p = 0; p < npix; p++) g[p * 16] = grad_out[p];
     sync();
-    CK(hipMemcpy(N.grad[NL - 1], g.data(), npix * 16 * sizeof(float), hipMemcpyHostToDevice));
+    CK(hipMemcpy(N.grad[NL - 1], grad_out, npix * sizeof(float), hipMemcpyHostToDevice));
     backward();
     sync();
     CK(hipMemcpy(gparams, N.g, N.nparam * sizeof(float), hipMemcpyDeviceToHost));
@@ -1570,7 +1609,7 @@ extern "C" int tea_unet_train(int epochs, int batch, float lr, float beta, unsig
         }
         forward();
         hipLaunchKernelGGL(k_loss, dim3(g1d(npix)), dim3(256), 0, st, N.out32, N.target, N.coef, N.norm, beta, batch, D.H,
-                           D.W, 16, N.grad[NL - 1], N.lpart);
+                           D.W, N.grad[NL - 1], N.lpart);
         backward();
         adam_step(lr);
         lsum += read_loss(npix, batch);
