@@ -95,7 +95,30 @@ struct ConvArgs {
   float *out32;      // [H][W][cout] (o32c0 = 0) or [H][W], channel 0 only (o32c0 = 1), or null
   int relu;
   int o32c0 = 0;
+  // mode 1, the input gradient of a decoder layer routed by its [up(src1), src2] split (out
+  // and out32 unused, the values stored are those out32 would hold): channels < sc1 summed
+  // over each 2 x 2 block of pixels into g1 ([H/2][W/2][sc1], (a00 + a01) + (a10 + a11)), the
+  // others stored into g2 ([H][W][cout - sc1]); a wavefront covers two rows x 32 pixels, so
+  // one lane holds all four pixels of a block (k_conv only). (The pooling adjoint of the
+  // encoder layers stays a separate pass: its read-modify-writes at the end of the
+  // convolution measured slower than the streaming kernel.)
+  int mode = 0, sc1 = 0;
+  float *g1 = nullptr, *g2 = nullptr;
 };
+
+// the two-row pixel blocks of mode 1: wavefront wq of an image covers rows 2 Y2, 2 Y2 + 1,
+// pixels xb .. xb + 31 (M tiles 0, 1 on the first row, 2, 3 on the second)
+__host__ __device__ inline int split_segs(int W) { return (W + 31) / 32; }
+
+// one output value of a plain convolution, pixel q = y W + x of image b
+__device__ __forceinline__ void conv_store(const ConvArgs &a, int b, int q, int co, float v) {
+  const int H = a.H, W = a.W, cout = a.cout;
+  if (a.relu && v < 0.f) v = 0.f;
+  const long p = (long)b * H * W + q, o = p * cout + co;
+  if (a.out) a.out[o] = f2bf(v);
+  if (a.out32 && !a.o32c0) a.out32[o] = v;
+  if (a.out32 && a.o32c0 && co == 0) a.out32[p] = v;
+}
 
 // forward convolution (and the input-gradient convolution with transformed weights) as an
 // implicit GEMM on v_mfma_f32_16x16x32_bf16: M = pixels, N = output channels, K = (tap,
@@ -104,7 +127,7 @@ struct ConvArgs {
 // load) are 8 channels of one pixel at one tap. A wavefront computes four 16-pixel M tiles
 // (64 consecutive pixels of the image, row-major; any width) by NT 16-channel N tiles, so
 // every weight fragment it loads feeds four products and every input fragment NT.
-template <int NT>
+template <int NT, int MODE>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
@@ -112,15 +135,22 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   const int H = a.H, W = a.W, c1 = a.c1, c2 = a.c2, cin = c1 + c2;
   const int K = 9 * cin, nkc = (K + 31) / 32;
   const int npix = H * W;
-  const int q0 = (blockIdx.x * 4 + wv) * 64;  // the wavefront's first pixel
+  const int q0 = (blockIdx.x * 4 + wv) * 64;  // the wavefront's first pixel (mode 0)
+  const int wq = blockIdx.x * 4 + wv, nseg = split_segs(W), Y2 = wq / nseg, xb = (wq - Y2 * nseg) * 32;
   int py[4], px[4];
   bool pv[4];
 #pragma unroll
   for (int t = 0; t < 4; t++) {
-    const int q = q0 + 16 * t + m;
-    pv[t] = q < npix;
-    py[t] = q / W;
-    px[t] = q - py[t] * W;
+    if (MODE == 1) {
+      py[t] = 2 * Y2 + (t >> 1);
+      px[t] = xb + 16 * (t & 1) + m;
+      pv[t] = 2 * Y2 < H && px[t] < W;
+    } else {
+      const int q = q0 + 16 * t + m;
+      pv[t] = q < npix;
+      py[t] = q / W;
+      px[t] = q - py[t] * W;
+    }
   }
   f4 acc[4][NT];
 #pragma unroll
@@ -155,6 +185,37 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   }
   // D[pixel 4 kq + j of the tile][channel m of the N tile]
   const int cout = a.cout;
+  if (MODE == 1) {
+    if (2 * Y2 >= H) return;
+    const int sc1 = a.sc1, sc2 = cout - sc1, Hc = H >> 1, Wc = W >> 1;
+#pragma unroll
+    for (int u = 0; u < NT; u++) {
+      const int co = 16 * u + m;  // sc1 is a multiple of 16: a tile is all g1 or all g2
+      const float bia = a.bias ? a.bias[co] : 0.f;
+      if (16 * u < sc1) {
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+          for (int jj = 0; jj < 2; jj++) {
+            const int x = xb + 16 * t + 4 * kq + 2 * jj;  // even; W is even
+            if (x >= W) continue;
+            const float v = ((acc[t][u][2 * jj] + bia) + (acc[t][u][2 * jj + 1] + bia)) +
+                            ((acc[t + 2][u][2 * jj] + bia) + (acc[t + 2][u][2 * jj + 1] + bia));
+            a.g1[(((long)b * Hc + Y2) * Wc + (x >> 1)) * sc1 + co] = v;
+          }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int x = xb + 16 * (t & 1) + 4 * kq + j;
+            if (x >= W) continue;
+            a.g2[(((long)b * H + 2 * Y2 + (t >> 1)) * W + x) * sc2 + (co - sc1)] = acc[t][u][j] + bia;
+          }
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < NT; u++) {
     const int co = 16 * u + m;
@@ -165,12 +226,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
       for (int j = 0; j < 4; j++) {
         const int q = q0 + 16 * t + 4 * kq + j;
         if (q >= npix) continue;
-        float v = acc[t][u][j] + bia;
-        if (a.relu && v < 0.f) v = 0.f;
-        const long o = ((long)b * npix + q) * cout + co;
-        if (a.out) a.out[o] = f2bf(v);
-        if (a.out32 && !a.o32c0) a.out32[o] = v;
-        if (a.out32 && a.o32c0 && co == 0) a.out32[(long)b * npix + q] = v;
+        conv_store(a, b, q, co, acc[t][u][j] + bia);
       }
   }
 }
@@ -235,7 +291,6 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
   // D[pixel 4 kq + j of the tile][channel m of the N tile]
   const int y = y0 + wv;
   if (y >= H) return;
-  const int cout = a.cout;
 #pragma unroll
   for (int u = 0; u < NT; u++) {
     const int co = 16 * u + m;
@@ -246,12 +301,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
       for (int j = 0; j < 4; j++) {
         const int x = x0 + 16 * t + 4 * kq + j;
         if (x >= W) continue;
-        float v = acc[t][u][j] + bia;
-        if (a.relu && v < 0.f) v = 0.f;
-        const long o = (((long)b * H + y) * W + x) * cout + co;
-        if (a.out) a.out[o] = f2bf(v);
-        if (a.out32 && !a.o32c0) a.out32[o] = v;
-        if (a.out32 && a.o32c0 && co == 0) a.out32[((long)b * H + y) * W + x] = v;
+        conv_store(a, b, y * W + x, co, acc[t][u][j] + bia);
       }
   }
 }
@@ -606,34 +656,6 @@ __global__ __launch_bounds__(256) void k_unpool_add(const float *gc, int B, int 
   *d = *d + 0.25f * *(const f4 *)(gc + (((long)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c);
 }
 
-// split the input gradient of a decoder layer ([up(src1), src2] at H x W): gsrc2 =
-// g[..., c1:], gsrc1(coarse) = sum of the 2 x 2 children of g[..., :c1]. A lane takes 4
-// channels of one coarse pixel's four children, so every fine pixel is read once and a
-// wavefront's loads cover contiguous runs of two rows. Both results are stored, not added:
-// the decoder layer is the first (skip source) or only (coarse source) layer whose input
-// gradient reaches them.
-__global__ __launch_bounds__(256) void k_split_up(const float *g, int B, int H, int W, int c1, int c2,
-                                                  float *g1, float *g2) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  const int cin = c1 + c2, Q = cin >> 2, Hc = H >> 1, Wc = W >> 1;
-  if (i >= (long)B * Hc * Wc * Q) return;
-  const int c = (int)(i % Q) * 4;
-  const long P = i / Q;
-  const int X = (int)(P % Wc), Y = (int)((P / Wc) % Hc), b = (int)(P / ((long)Wc * Hc));
-  const long p0 = ((long)b * H + 2 * Y) * W + 2 * X, p2 = p0 + W;
-  const f4 a0 = *(const f4 *)(g + p0 * cin + c), a1 = *(const f4 *)(g + (p0 + 1) * cin + c);
-  const f4 a2 = *(const f4 *)(g + p2 * cin + c), a3 = *(const f4 *)(g + (p2 + 1) * cin + c);
-  if (c < c1) {
-    *(f4 *)(g1 + P * c1 + c) = (a0 + a1) + (a2 + a3);
-  } else {
-    const int d = c - c1;
-    *(f4 *)(g2 + p0 * c2 + d) = a0;
-    *(f4 *)(g2 + (p0 + 1) * c2 + d) = a1;
-    *(f4 *)(g2 + p2 * c2 + d) = a2;
-    *(f4 *)(g2 + (p2 + 1) * c2 + d) = a3;
-  }
-}
-
 // Loss on channel 0 of the output: the error e = prediction - target in the norm
 // e^T (I + beta C) e, C the conduction part of the sample's operator (A = I + C), relative
 // to the target's, averaged over the batch. beta = 0 gives the relative mean squared
@@ -711,7 +733,7 @@ struct Net {
   float *target = nullptr, *coef = nullptr;
   double *norm = nullptr;  // per image of the batch, {t^T t, t^T C t}
   float *grad[NL] = {}; // gradients w.r.t. layer outputs (FP32)
-  float *gin = nullptr; // input gradient of a layer (largest)
+  float *gin = nullptr;  // input gradient of an encoder layer (largest)
   bf16 *gm = nullptr;   // masked output gradient, BF16
   bf16 *gmT = nullptr;  // the same, row-chunked (weight gradient)
   bf16 *xT = nullptr;   // a layer's input, row-chunked (weight gradient)
@@ -821,14 +843,9 @@ void launch_wgrad(const WgradArgs &a, int nbands) {
   }
 }
 
-// k_conv assumes channel counts in multiples of 16, at most 7 output tiles, and src1 present
-// exactly when c1 > 0
-void launch_conv(const ConvArgs &a, int B) {
-  if (a.c1 % 16 || a.c2 % 16 || a.c2 <= 0 || a.cout % 16 || a.cout <= 0 || a.cout > 112 ||
-      (a.c1 > 0) != (a.src1 != nullptr) || !a.src2 || !a.w || a.H <= 0 || a.W <= 0 || (a.c1 && (a.H % 2 || a.W % 2)))
-    throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: convolution shape outside the kernel's limits");
-  const int cin = a.c1 + a.c2;
-  if (cin <= 32 && a.cout <= 32) {
+template <int MODE>
+void launch_conv_mode(const ConvArgs &a, int B, int cin) {
+  if (MODE != 1 && cin <= 32 && a.cout <= 32) {
     // thin layers: the LDS-tiled kernel (at most 19 KB of LDS per workgroup)
     const dim3 g((unsigned)((a.W + 63) / 64), (unsigned)((a.H + CT_ROWS - 1) / CT_ROWS), (unsigned)B);
     const size_t lds = ct_lds_bytes(cin);
@@ -836,17 +853,33 @@ void launch_conv(const ConvArgs &a, int B) {
     else hipLaunchKernelGGL(k_conv_lds<2>, g, dim3(256), lds, stream(), a);
     return;
   }
-  // four wavefronts of 64 pixels per workgroup, pixels of one image row-major
-  const dim3 grid((unsigned)(((long)a.H * a.W + 255) / 256), (unsigned)B);
+  // four wavefronts of 64 pixels per workgroup, pixels of one image row-major (mode 1: the
+  // two-row blocks of 2 x 32 pixels)
+  const long waves = MODE == 1 ? (long)(a.H / 2) * split_segs(a.W) : ((long)a.H * a.W + 63) / 64;
+  const dim3 grid((unsigned)((waves + 3) / 4), (unsigned)B);
   switch (a.cout / 16) {
-    case 1: hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), 0, stream(), a); break;
-    case 2: hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, stream(), a); break;
-    case 3: hipLaunchKernelGGL(k_conv<3>, grid, dim3(256), 0, stream(), a); break;
-    case 4: hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, stream(), a); break;
-    case 5: hipLaunchKernelGGL(k_conv<5>, grid, dim3(256), 0, stream(), a); break;
-    case 6: hipLaunchKernelGGL(k_conv<6>, grid, dim3(256), 0, stream(), a); break;
-    default: hipLaunchKernelGGL(k_conv<7>, grid, dim3(256), 0, stream(), a); break;
+    case 1: hipLaunchKernelGGL((k_conv<1, MODE>), grid, dim3(256), 0, stream(), a); break;
+    case 2: hipLaunchKernelGGL((k_conv<2, MODE>), grid, dim3(256), 0, stream(), a); break;
+    case 3: hipLaunchKernelGGL((k_conv<3, MODE>), grid, dim3(256), 0, stream(), a); break;
+    case 4: hipLaunchKernelGGL((k_conv<4, MODE>), grid, dim3(256), 0, stream(), a); break;
+    case 5: hipLaunchKernelGGL((k_conv<5, MODE>), grid, dim3(256), 0, stream(), a); break;
+    case 6: hipLaunchKernelGGL((k_conv<6, MODE>), grid, dim3(256), 0, stream(), a); break;
+    default: hipLaunchKernelGGL((k_conv<7, MODE>), grid, dim3(256), 0, stream(), a); break;
   }
+}
+
+// k_conv assumes channel counts in multiples of 16, at most 7 output tiles, and src1 present
+// exactly when c1 > 0
+void launch_conv(const ConvArgs &a, int B) {
+  if (a.c1 % 16 || a.c2 % 16 || a.c2 <= 0 || a.cout % 16 || a.cout <= 0 || a.cout > 112 ||
+      (a.c1 > 0) != (a.src1 != nullptr) || !a.src2 || !a.w || a.H <= 0 || a.W <= 0 || (a.c1 && (a.H % 2 || a.W % 2)))
+    throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: convolution shape outside the kernel's limits");
+  if ((a.mode == 1 && (a.H % 2 || a.W % 2 || a.sc1 <= 0 || a.sc1 % 16 || a.sc1 >= a.cout || !a.g1 || !a.g2)) ||
+      a.mode < 0 || a.mode > 1)
+    throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: input-gradient routing outside the kernel's limits");
+  const int cin = a.c1 + a.c2;
+  if (a.mode == 0) launch_conv_mode<0>(a, B, cin);
+  else launch_conv_mode<1>(a, B, cin);
 }
 
 // sources of layer l
@@ -965,21 +998,27 @@ void backward() {
     a.w = N.wt;
     a.bias = nullptr;
     a.out = nullptr;
-    a.out32 = l == NL - 1 ? N.grad[NL - 2] : N.gin;  // the output layer's input is dec0's output
+    a.out32 = nullptr;
     a.relu = 0;
-    launch_conv(a, B);
-    // route it to the layers that produced the input
+    // routed to the layers that produced the input
     if (l >= 1 && l <= 3) {
-      // input = pool(act[l-1]): back through the average
-      const long nf = (long)B * (2 * h) * (2 * w) * SPEC[l].c2 / 4;
-      hipLaunchKernelGGL(k_unpool_add, dim3(g1d(nf)), dim3(256), 0, st, N.gin, B, 2 * h, 2 * w, SPEC[l].c2,
-                         N.grad[l - 1]);
+      a.out32 = N.gin;  // then back through the pooling below
     } else if (l >= 4 && l <= 6) {
       // input = [up(act[s1]), act[s2]]
       const int s1 = (l == 4) ? 3 : l - 1, s2 = (l == 4) ? 2 : (l == 5 ? 1 : 0);
-      const long nq = (long)B * (h / 2) * (w / 2) * (cin / 4);
-      hipLaunchKernelGGL(k_split_up, dim3(g1d(nq)), dim3(256), 0, st, N.gin, B, h, w, SPEC[l].c1, SPEC[l].c2,
-                         N.grad[s1], N.grad[s2]);
+      a.mode = 1;
+      a.sc1 = SPEC[l].c1;
+      a.g1 = N.grad[s1];
+      a.g2 = N.grad[s2];
+    } else {
+      a.out32 = N.grad[NL - 2];  // the output layer's input is dec0's output
+    }
+    launch_conv(a, B);
+    if (l >= 1 && l <= 3) {
+      // input = pool(act[l-1]): the average's adjoint, added to the skip path's part
+      const long nf = (long)B * (2 * h) * (2 * w) * SPEC[l].c2 / 4;
+      hipLaunchKernelGGL(k_unpool_add, dim3(g1d(nf)), dim3(256), 0, st, N.gin, B, 2 * h, 2 * w, SPEC[l].c2,
+                         N.grad[l - 1]);
     }
   }
   ops_hip_check_launch("tea_unet backward");
