@@ -536,86 +536,100 @@ __global__ __launch_bounds__(256) void k_wgrad_sum(const float *part, int nbands
 // BF16, and the first pass of the bias gradient (per-channel sums of the masked gradient).
 // The masked gradient is written twice: NHWC (gm, the input-gradient convolution's operand)
 // and row-chunked (gmT, the weight gradient's: [b][y][x / 8][c][8], rows of H x W images
-// padded with zeros to whole chunks). Each of BIAS_WG workgroups walks tiles of 64 pixels
-// of a row: lanes take 4 channels of a pixel (16-byte loads of g, 8-byte loads of y and
-// stores of gm, all contiguous across the wavefront) and put the masked values into a
-// pixel-major LDS tile, from which each 16-byte gmT store gathers 8 pixels of one channel.
-// A lane keeps one channel quad, so its sums and the workgroup's per-channel partials,
-// bpart[wg][co], are added in a fixed order. With g_c0 set, g holds channel 0 alone
-// ([pixels]; the other channels are zero): the output layer's gradient.
-constexpr int BIAS_WG = 2048;
+// padded with zeros to whole chunks). A workgroup takes a tile of 64 pixels of a row:
+// lanes take 4 channels of a pixel (16-byte loads of g, 8-byte loads of y and stores of gm,
+// all contiguous across the wavefront) and put the masked values into a pixel-major LDS
+// tile, from which each 16-byte gmT store gathers 8 pixels of one channel. A lane keeps one
+// channel quad, so the tile's per-channel sums, bpart[tile][co], are added in a fixed
+// order. With g_c0 set, g holds channel 0 alone ([pixels]; the other channels are zero):
+// the output layer's gradient.
 constexpr int MB_PX = 64;
-__global__ __launch_bounds__(256) void k_mask_bias(const float *g, int g_c0, const bf16 *y, int relu, long rows,
-                                                   int cout, int W, bf16 *gm, bf16 *gmT, float *bpart) {
+__global__ __launch_bounds__(256) void k_mask_bias(const float *g, int g_c0, const bf16 *y, int relu, int cout,
+                                                   int W, bf16 *gm, bf16 *gmT, float *bpart) {
   __shared__ __align__(16) bf16 t[MB_PX * 64];  // t[px * cout + c], cout <= 64
   __shared__ float lds[256 * 4];
   const int Q = cout >> 2, lanes = 256 - 256 % Q, tid = threadIdx.x;
   const int Wb = (W + 7) >> 3, tpr = (W + MB_PX - 1) / MB_PX;
-  const long ntiles = rows * tpr;
+  const long tile = blockIdx.x, row = tile / tpr;
+  const int x0 = (int)(tile - row * tpr) * MB_PX, npx = min(MB_PX, W - x0);
+  const int nchk = min(MB_PX / 8, Wb - (x0 >> 3));  // chunks, the row's padding included
   float s[4] = {0.f, 0.f, 0.f, 0.f};
-  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const long row = tile / tpr;
-    const int x0 = (int)(tile - row * tpr) * MB_PX, npx = min(MB_PX, W - x0);
-    const int nchk = min(MB_PX / 8, Wb - (x0 >> 3));  // chunks, the row's padding included
-    if (tid < lanes)
-      for (int e = tid; e < nchk * 8 * Q; e += lanes) {
-        const int px = e / Q, c = 4 * (e - px * Q);
-        float v[4] = {0.f, 0.f, 0.f, 0.f};
-        if (px < npx) {
-          const long p = row * W + x0 + px, i = p * cout + c;
-          if (!g_c0) {
-            const f4 a = *(const f4 *)(g + i);
-            v[0] = a[0];
-            v[1] = a[1];
-            v[2] = a[2];
-            v[3] = a[3];
-          } else if (c == 0) {
-            v[0] = g[p];
-          }
-          if (relu) {
-            const uint2 yy = *(const uint2 *)(y + i);
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-              if (bf2f((bf16)((k < 2 ? yy.x : yy.y) >> (16 * (k & 1)))) <= 0.f) v[k] = 0.f;
-          }
-#pragma unroll
-          for (int k = 0; k < 4; k++) s[k] += v[k];
+  if (tid < lanes)
+    for (int e = tid; e < nchk * 8 * Q; e += lanes) {
+      const int px = e / Q, c = 4 * (e - px * Q);
+      float v[4] = {0.f, 0.f, 0.f, 0.f};
+      if (px < npx) {
+        const long p = row * W + x0 + px, i = p * cout + c;
+        if (!g_c0) {
+          const f4 a = *(const f4 *)(g + i);
+          v[0] = a[0];
+          v[1] = a[1];
+          v[2] = a[2];
+          v[3] = a[3];
+        } else if (c == 0) {
+          v[0] = g[p];
         }
-        const uint2 o = {(unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16),
-                         (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16)};
-        if (px < npx) *(uint2 *)(gm + (row * W + x0 + px) * cout + c) = o;
-        *(uint2 *)(t + px * cout + c) = o;
-      }
-    __syncthreads();
-    u4 *out = (u4 *)(gmT + (row * Wb + (x0 >> 3)) * cout * 8);
-    for (int e = tid; e < nchk * cout; e += 256) {
-      const int xl = e / cout, c = e - xl * cout;
-      const bf16 *col = t + xl * 8 * cout + c;
-      u4 v;
+        if (relu) {
+          const uint2 yy = *(const uint2 *)(y + i);
 #pragma unroll
-      for (int k = 0; k < 4; k++) v[k] = (unsigned)col[2 * k * cout] | ((unsigned)col[(2 * k + 1) * cout] << 16);
-      out[e] = v;
+          for (int k = 0; k < 4; k++)
+            if (bf2f((bf16)((k < 2 ? yy.x : yy.y) >> (16 * (k & 1)))) <= 0.f) v[k] = 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] += v[k];
+      }
+      const uint2 o = {(unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16),
+                       (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16)};
+      if (px < npx) *(uint2 *)(gm + (row * W + x0 + px) * cout + c) = o;
+      *(uint2 *)(t + px * cout + c) = o;
     }
-    __syncthreads();
-  }
 #pragma unroll
   for (int k = 0; k < 4; k++) lds[tid * 4 + k] = s[k];
   __syncthreads();
+  u4 *out = (u4 *)(gmT + (row * Wb + (x0 >> 3)) * cout * 8);
+  for (int e = tid; e < nchk * cout; e += 256) {
+    const int xl = e / cout, c = e - xl * cout;
+    const bf16 *col = t + xl * 8 * cout + c;
+    u4 v;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = (unsigned)col[2 * k * cout] | ((unsigned)col[(2 * k + 1) * cout] << 16);
+    out[e] = v;
+  }
   if (tid < cout) {
     const int q = tid >> 2, k = tid & 3;
     float v = 0.f;
     for (int l = q; l < lanes; l += Q) v += lds[l * 4 + k];
-    bpart[blockIdx.x * cout + tid] = v;
+    bpart[tile * cout + tid] = v;
   }
 }
 
-// second pass: db[co] = sum over the workgroups' partials, one workgroup per channel (each
-// thread a strided share in order, then the workgroup tree)
-__global__ __launch_bounds__(256) void k_bias_fin(const float *bpart, int cout, float *db) {
+// second pass: segment sums of the tiles' partials, seg[s][co] = sum over the tiles of
+// segment s in order; a workgroup reads its segment's partials contiguously (lane t takes
+// channel t % cout of every (256 / cout)-th tile), then adds its lanes per channel in order
+constexpr int BIAS_SEGS = 256;
+__global__ __launch_bounds__(256) void k_bias_seg(const float *bpart, long ntiles, int cout, float *seg) {
+  __shared__ float lds[256];
+  const int t = threadIdx.x, lanes = 256 - 256 % cout, c = t % cout, nr = lanes / cout;
+  const long per = (ntiles + BIAS_SEGS - 1) / BIAS_SEGS, k0 = blockIdx.x * per, k1 = min(ntiles, k0 + per);
+  float v = 0.f;
+  if (t < lanes)
+    for (long k = k0 + t / cout; k < k1; k += nr) v += bpart[k * cout + c];
+  lds[t] = v;
+  __syncthreads();
+  if (t < cout) {
+    float r = 0.f;
+    for (int q = 0; q < nr; q++) r += lds[q * cout + t];
+    seg[blockIdx.x * cout + t] = r;
+  }
+}
+
+// third pass: db[co] = sum over the segments, one workgroup per channel (each thread a
+// strided share in order, then the workgroup tree)
+__global__ __launch_bounds__(256) void k_bias_fin(const float *bpart, long ntiles, int cout, float *db) {
   __shared__ float lds[8];
   const int c = blockIdx.x;
   float v = 0.f;
-  for (int k = threadIdx.x; k < BIAS_WG; k += 256) v += bpart[k * cout + c];
+  for (long k = threadIdx.x; k < ntiles; k += 256) v += bpart[k * cout + c];
   v = ops_block_reduce<ops_red_inc<float>>(v, lds);
   if (threadIdx.x == 0) db[c] = v;
 }
@@ -741,7 +755,8 @@ struct Net {
   size_t wpart_cap = 0;
   float *wseg = nullptr;  // segment partials of the two-pass band sum (at most 2048 x 64 values)
   float *lpart = nullptr;
-  float *bpart = nullptr;  // bias-gradient partials [BIAS_WG][64]
+  float *bpart = nullptr;  // bias-gradient partials [tile][cout] (tiles of 64 pixels of a row)
+  float *bseg = nullptr;   // their segment sums [BIAS_SEGS][cout]
   char *pinned = nullptr;
 };
 Net N;
@@ -779,6 +794,7 @@ void free_acts() {
   dfree(N.xT);
   dfree(N.lpart);
   dfree(N.bpart);
+  dfree(N.bseg);
   N.B = N.H = N.W = 0;
 }
 
@@ -814,7 +830,13 @@ void ensure_acts(int B, int H, int W) {
   N.gmT = dalloc<bf16>(maxrc);
   N.xT = dalloc<bf16>(maxrc);
   N.lpart = dalloc<float>(g1d(pix(0)) + 1);
-  N.bpart = dalloc<float>((size_t)BIAS_WG * 64);
+  long maxbp = 0;
+  for (int l = 0; l < NL; l++) {
+    const long w = W >> SPEC[l].level;
+    maxbp = std::max(maxbp, (long)B * (H >> SPEC[l].level) * ((w + MB_PX - 1) / MB_PX) * SPEC[l].cout);
+  }
+  N.bpart = dalloc<float>(maxbp);
+  N.bseg = dalloc<float>((size_t)BIAS_SEGS * 64);
 }
 
 
@@ -938,9 +960,12 @@ void backward() {
     const long npix = (long)B * h * w, n = npix * cout;
     // masked gradient (the kernel's LDS tile holds at most 64 channels)
     if (cout > 64 || cout % 8) throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: layer output too wide");
-    hipLaunchKernelGGL(k_mask_bias, dim3(BIAS_WG), dim3(256), 0, st, N.grad[l], l == NL - 1 ? 1 : 0, N.act[l],
-                       SPEC[l].relu, (long)B * h, cout, w, N.gm, N.gmT, N.bpart);
-    hipLaunchKernelGGL(k_bias_fin, dim3((unsigned)cout), dim3(256), 0, st, N.bpart, cout, N.g + N.boff[l]);
+    const long mtiles = (long)B * h * ((w + MB_PX - 1) / MB_PX);
+    hipLaunchKernelGGL(k_mask_bias, dim3((unsigned)mtiles), dim3(256), 0, st, N.grad[l], l == NL - 1 ? 1 : 0,
+                       N.act[l], SPEC[l].relu, cout, w, N.gm, N.gmT, N.bpart);
+    hipLaunchKernelGGL(k_bias_seg, dim3(BIAS_SEGS), dim3(256), 0, st, N.bpart, mtiles, cout, N.bseg);
+    hipLaunchKernelGGL(k_bias_fin, dim3((unsigned)cout), dim3(256), 0, st, N.bseg, (long)BIAS_SEGS, cout,
+                       N.g + N.boff[l]);
     // weight gradient from row-chunked copies of the masked gradient and the layer input
     const bf16 *s1, *s2;
     sources(l, s1, s2);
