@@ -980,12 +980,14 @@ void backward() {
     wa.H = h;
     wa.W = w;
     wa.rows = B * h;
-    // bands of rows: partial sums of at most ~16 MB (~64 MB for the nine-tap kernel, whose
-    // wavefronts each take a band), at least 64 and at most 1024 (4096) bands
+    // bands of rows: partial sums of ~16 MB (~64 MB for the nine-tap kernel, whose
+    // wavefronts each take a band), at least 64 and at most 4096 bands for the nine-tap
+    // kernel; the per-tap kernel (one workgroup per band) takes at least 512 bands so that
+    // its workgroups fill the GPU (256 and 1024 measured slower), at most 1024
     const long per_band = 9L * cout * cin;
     const bool nine = (cout / 16) * (cin / 16) <= 3;
     const long cap = nine ? 4096L : 1024L;
-    const int want = (int)std::min(cap, std::max(64L, (nine ? (16L << 20) : (4L << 20)) / per_band));
+    const int want = (int)std::min(cap, std::max(nine ? 64L : 512L, (nine ? (16L << 20) : (4L << 20)) / per_band));
     wa.band = std::max(1, (B * h + want - 1) / want);
     const int nbands = (B * h + wa.band - 1) / wa.band;
     const size_t need = (size_t)nbands * per_band;
