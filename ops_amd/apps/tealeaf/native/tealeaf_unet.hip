@@ -983,7 +983,8 @@ void backward() {
     // bands of rows: partial sums of ~16 MB (~64 MB for the nine-tap kernel, whose
     // wavefronts each take a band), at least 64 and at most 4096 bands for the nine-tap
     // kernel; the per-tap kernel (one workgroup per band) takes at least 512 bands so that
-    // its workgroups fill the GPU (256 and 1024 measured slower), at most 1024
+    // its workgroups fill the GPU (the ~100-200 bands of a 16 MB bound and a 1024-band floor
+    // measured slower), at most 1024
     const long per_band = 9L * cout * cin;
     const bool nine = (cout / 16) * (cin / 16) <= 3;
     const long cap = nine ? 4096L : 1024L;
