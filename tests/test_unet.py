@@ -56,11 +56,13 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
+# (2, 32, 24): widths that are not multiples of a wavefront's row segment at any level;
+# (1, 64, 512): widths 512 .. 64, multiples of it at every level
 @pytest.mark.gpu
-def test_gpu_forward_and_gradients_match_oracle():
+@pytest.mark.parametrize("B,H,W", [(2, 32, 24), (1, 64, 512)])
+def test_gpu_forward_and_gradients_match_oracle(B, H, W):
     import ops_amd
     L = ops_amd.app("tealeaf")
-    B, H, W = 2, 32, 24
     p = random_params(5).astype(np.float32)
     x = random_input(6, B, H, W).astype(np.float32)
     go = np.random.default_rng(7).normal(0, 1, (B, H, W)).astype(np.float32)
