@@ -306,22 +306,40 @@ struct Fine {
   int rev = 0;  // tile walk direction of the launch
 };
 
-__device__ __forceinline__ bool fcell(const Fine &g, long &i) {
-  int x, y;
-  const bool ok = cell2(g.n, g.n, x, y, g.rev);
-  i = x + (long)y * g.s;
-  return ok;
+// The outer kernels' lanes take FR cells each, one column of the workgroup's tile of BX x
+// (BY FR) cells, rows BY apart (row by BY FR + ty + BY r for r < FR), so a wavefront's loads
+// stay contiguous along x; the tiles (and partials, one per tile) are FR times fewer than a
+// one-cell-per-lane grid's, which keeps the single-workgroup folds short.
+constexpr int FR = 4;
+dim3 gridf(int n) { return dim3((unsigned)((n + BX - 1) / BX), (unsigned)((n + BY * FR - 1) / (BY * FR))); }
+
+struct FCol {
+  int x, y0;  // the lane's column and first row (1-based interior coordinates)
+};
+__device__ __forceinline__ FCol fcol(const Fine &g) {
+  unsigned bx, by, bz;
+  ops_block_coords_dir(bx, by, bz, g.rev);
+  return FCol{1 + (int)(bx * BX + threadIdx.x), 1 + (int)(by * BY * FR + threadIdx.y)};
+}
+// cell r of the lane: false outside the mesh
+__device__ __forceinline__ bool fcell(const Fine &g, const FCol &c, int r, long &i) {
+  const int y = c.y0 + BY * r;
+  i = c.x + (long)y * g.s;
+  return c.x <= g.n && y <= g.n;
 }
 
 // start: b32 = (vreal) r, r.r partials
 __global__ __launch_bounds__(NT) void k_start(const double *__restrict__ r, vreal *__restrict__ b32, Fine g,
                                               double *part) {
-  long i;
+  const FCol c = fcol(g);
   double v = 0.0;
-  if (fcell(g, i)) {
+#pragma unroll
+  for (int q = 0; q < FR; q++) {
+    long i;
+    if (!fcell(g, c, q, i)) continue;
     const double ri = r[i];
     b32[i] = (vreal)ri;
-    v = ri * ri;
+    v += ri * ri;
   }
   write_partial(v, part, g.rev);
 }
@@ -343,12 +361,15 @@ __global__ __launch_bounds__(NT) void k_dots(const double *__restrict__ r, const
                                              const vreal *__restrict__ zp, Fine g, double *part,
                                              const Rec *S) {
   if (S->stopped) return;
-  long i;
+  const FCol c = fcol(g);
   double a = 0.0, b = 0.0;
-  if (fcell(g, i)) {
+#pragma unroll
+  for (int q = 0; q < FR; q++) {
+    long i;
+    if (!fcell(g, c, q, i)) continue;
     const double ri = r[i];
-    a = ri * (double)z[i];
-    b = ri * (double)zp[i];
+    a += ri * (double)z[i];
+    b += ri * (double)zp[i];
   }
   __shared__ double lds[NT / OPS_WAVE];
   const double sa = ops_block_reduce<ops_red_inc<double>>(a, lds);
@@ -391,9 +412,12 @@ __global__ __launch_bounds__(NT) void k_pw(const double *__restrict__ Kx, const 
                                            double *part, const Rec *S) {
   if (S->stopped) return;
   const double beta = S->beta;
-  long i;
+  const FCol c = fcol(g);
   double v = 0.0;
-  if (fcell(g, i)) {
+#pragma unroll
+  for (int q = 0; q < FR; q++) {
+    long i;
+    if (!fcell(g, c, q, i)) continue;
     const int s = g.s;
     const double pc = (double)z[i] + beta * pin[i];
     const double pe = (double)z[i + 1] + beta * pin[i + 1];
@@ -405,7 +429,7 @@ __global__ __launch_bounds__(NT) void k_pw(const double *__restrict__ Kx, const 
                       g.rx * (kx1 * pe + kx0 * pwst);
     pout[i] = pc;
     w[i] = wi;
-    v = wi * pc;
+    v += wi * pc;
   }
   write_partial(v, part, g.rev);
 }
@@ -425,14 +449,17 @@ __global__ __launch_bounds__(NT) void k_ur(double *__restrict__ u, double *__res
                                            vreal *__restrict__ b32, Fine g, double *part, const Rec *S) {
   if (S->stopped) return;
   const double alpha = S->alpha;
-  long i;
+  const FCol c = fcol(g);
   double v = 0.0;
-  if (fcell(g, i)) {
+#pragma unroll
+  for (int q = 0; q < FR; q++) {
+    long i;
+    if (!fcell(g, c, q, i)) continue;
     u[i] = u[i] + alpha * p[i];
     const double ri = r[i] - alpha * w[i];
     r[i] = ri;
     b32[i] = (vreal)ri;
-    v = ri * ri;
+    v += ri * ri;
   }
   write_partial(v, part, g.rev);
 }
@@ -456,9 +483,12 @@ __global__ __launch_bounds__(FT) void k_fold_rr(const double *part, unsigned n, 
 // (d - m)^T A (d - m) partials (once per solve)
 __global__ __launch_bounds__(NT) void k_defl_dad(const double *__restrict__ Kx, const double *__restrict__ Ky,
                                                  const double *__restrict__ d, double m, Fine g, double *part) {
-  long i;
+  const FCol c = fcol(g);
   double v = 0.0;
-  if (fcell(g, i)) {
+#pragma unroll
+  for (int q = 0; q < FR; q++) {
+    long i;
+    if (!fcell(g, c, q, i)) continue;
     const int s = g.s, n = g.n;
     const int x = (int)(i % s), y = (int)(i / s);
     // d outside the mesh is zero (not d - m): the operator's boundary faces see x = 0
@@ -468,7 +498,7 @@ __global__ __launch_bounds__(NT) void k_defl_dad(const double *__restrict__ Kx, 
     const double kx0 = Kx[i], kx1 = Kx[i + 1], ky0 = Ky[i], ky1 = Ky[i + s];
     const double ad = (1.0 + g.ry * (ky1 + ky0) + g.rx * (kx1 + kx0)) * dc - g.ry * (ky1 * dn + ky0 * ds) -
                       g.rx * (kx1 * de + kx0 * dw);
-    v = dc * ad;
+    v += dc * ad;
   }
   write_partial(v, part, g.rev);
 }
@@ -483,9 +513,13 @@ __global__ __launch_bounds__(NT) void k_defl_dot(const double *__restrict__ d, d
                                                  const double *__restrict__ r, Fine g, double *part,
                                                  const Rec *S) {
   if (S->stopped) return;
-  long i;
+  const FCol c = fcol(g);
   double v = 0.0;
-  if (fcell(g, i)) v = (d[i] - m) * r[i];
+#pragma unroll
+  for (int q = 0; q < FR; q++) {
+    long i;
+    if (fcell(g, c, q, i)) v += (d[i] - m) * r[i];
+  }
   write_partial(v, part, g.rev);
 }
 
@@ -499,8 +533,12 @@ __global__ __launch_bounds__(FT) void k_fold_defl(const double *part, unsigned n
 __global__ __launch_bounds__(NT) void k_defl_add(const double *__restrict__ d, double m, vreal *__restrict__ z,
                                                  Fine g, const Rec *S) {
   if (S->stopped) return;
-  long i;
-  if (fcell(g, i)) z[i] = (vreal)((double)z[i] + S->dgam * (d[i] - m));
+  const FCol c = fcol(g);
+#pragma unroll
+  for (int q = 0; q < FR; q++) {
+    long i;
+    if (fcell(g, c, q, i)) z[i] = (vreal)((double)z[i] + S->dgam * (d[i] - m));
+  }
 }
 
 // ---------------------------------------------------------------- host side
@@ -582,7 +620,7 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
   hipStream_t st = (hipStream_t)ops_hip_get_stream();
   const int n = P->nx;
   const Fine g{n, n + 2, P->rx, P->ry};
-  const dim3 blk(BX, BY), gf = grid2(n, n);
+  const dim3 blk(BX, BY), gf = gridf(n);
   const unsigned nb = gf.x * gf.y;
   // levels
   std::vector<int> sizes{n};
