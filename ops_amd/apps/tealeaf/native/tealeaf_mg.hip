@@ -213,11 +213,17 @@ __global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict
   const int x0 = 1 + (int)bx * SW, y0 = 1 + (int)by * SH;
   const int tid = threadIdx.x + BX * threadIdx.y;
   const int s = L.s, n = L.n;
-  for (int e = tid; e < (SH + 2) * (SW + 2); e += NT) {
-    const int lx = e % (SW + 2), ly = e / (SW + 2);
+  // lanes take the cells of the phase's own colour only: in window row ly those at lx of
+  // parity (first + x0 + y0 + ly) (the first colour, phase 1; 33 of the ring window's 66) and
+  // lx - 1 of that parity (the second, phase 2; 32 of the tile's 64). Entries of R at the
+  // second colour are never read.
+  constexpr int H1 = (SW + 2) / 2, H2 = SW / 2;
+  const int par0 = first + x0 + y0;
+  for (int e = tid; e < (SH + 2) * H1; e += NT) {
+    const int ly = e / H1, lx = 2 * (e - ly * H1) + ((par0 + ly) & 1);
     const int x = x0 - 1 + lx, y = y0 - 1 + ly;
     vreal r = vreal(0);
-    if (x >= 1 && x <= n && y >= 1 && y <= n && (x + y) % 2 == first) {
+    if (x >= 1 && x <= n && y >= 1 && y <= n) {
       const long i = x + (long)y * s;
       const vreal c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
       const vreal diag = L.m + ((c0 + c1) + (c2 + c3));
@@ -239,10 +245,10 @@ __global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict
     R[ly][lx] = r;
   }
   __syncthreads();
-  for (int e = tid; e < SH * SW; e += NT) {
-    const int lx = 1 + e % SW, ly = 1 + e / SW;
+  for (int e = tid; e < SH * H2; e += NT) {
+    const int ly = 1 + e / H2, lx = 1 + 2 * (e - (ly - 1) * H2) + ((par0 + ly) & 1);
     const int x = x0 - 1 + lx, y = y0 - 1 + ly;
-    if (x > n || y > n || (x + y) % 2 == first) continue;
+    if (x > n || y > n) continue;
     const long i = x + (long)y * s;
     const vreal c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
     const vreal diag = L.m + ((c0 + c1) + (c2 + c3));
