@@ -194,16 +194,41 @@ __global__ __launch_bounds__(NT) void k_restrict(Level F, Level C, const Rec *S,
 // different array: a neighbouring workgroup may still read xin's cells that this one
 // updates). Phase one computes colour `first` on the tile and a one-cell ring around it
 // (the ring's cells are also computed by the neighbouring workgroups, with the same inputs
-// and arithmetic, so the values agree), keeps them in LDS and stores the tile's own; phase
-// two computes the other colour of the tile from those LDS values. The result is the two
-// separate half sweeps' (with init, xin is not read: the first colour starts from zero and
-// the other colour's old values are never used); each array is read about once per step
-// instead of twice.
+// and arithmetic, so the values agree) and keeps them in LDS; phase two computes the other
+// colour of the tile from those LDS values. The result is the two separate half sweeps'
+// (with init, xin is not read: the first colour starts from zero and the other colour's old
+// values are never used), from one launch instead of two.
 constexpr int SW = 64, SH = 16;
 // With xc set (the up-sweep), the iterate is xin plus the coarse correction prolonged on
-// the fly: x_j = xin_j + ALPHA_C * xc(parent of j) for interior j, the arithmetic of
-// k_prolong, so the prolonged field is neither written nor read back (only the other
-// colour's values of it are ever used: a Gauss-Seidel update does not read its own cell).
+// the fly: x_j = xin_j + ALPHA_C * xc(parent of j) for interior j, so the prolonged field is
+// neither written nor read back (only the other colour's values of it are ever used: a
+// Gauss-Seidel update does not read its own cell).
+// Two kernels do this step with the same arithmetic per cell: k_smooth2 (a lane per cell of
+// the phase's colour) and k_smooth2p (a lane per pair of cells, for the widest levels).
+
+// the value of first-colour cell (x, y) of level L in a smoothing step from the iterate xin
+// (init: from zero), the neighbours prolonged-corrected when xc is set
+__device__ __forceinline__ vreal first_value(const Level &L, const vreal *__restrict__ xin,
+                                             const vreal *__restrict__ xc, int cs, int init, int x, int y) {
+  const int s = L.s, n = L.n;
+  const long i = x + (long)y * s;
+  const vreal c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
+  const vreal diag = L.m + ((c0 + c1) + (c2 + c3));
+  vreal v = L.b[i];
+  if (!init) {
+    vreal xw = xin[i - 1], xe = xin[i + 1], xs = xin[i - s], xn = xin[i + s];
+    if (xc) {
+      // the neighbours' prolonged values (halo cells get no correction)
+      if (x > 1) xw = xw + ALPHA_C * xc[x / 2 + (long)((y + 1) / 2) * cs];
+      if (x < n) xe = xe + ALPHA_C * xc[(x + 2) / 2 + (long)((y + 1) / 2) * cs];
+      if (y > 1) xs = xs + ALPHA_C * xc[(x + 1) / 2 + (long)(y / 2) * cs];
+      if (y < n) xn = xn + ALPHA_C * xc[(x + 1) / 2 + (long)((y + 2) / 2) * cs];
+    }
+    v += (c0 * xw + c1 * xe) + (c2 * xs + c3 * xn);
+  }
+  return v / diag;
+}
+
 __global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict__ xin, const vreal *__restrict__ xc,
                                                 int cs, int first, int init, const Rec *S, int rev) {
   if (S->stopped) return;
@@ -224,23 +249,8 @@ __global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict
     const int x = x0 - 1 + lx, y = y0 - 1 + ly;
     vreal r = vreal(0);
     if (x >= 1 && x <= n && y >= 1 && y <= n) {
-      const long i = x + (long)y * s;
-      const vreal c0 = L.cx[i], c1 = L.cx[i + 1], c2 = L.cy[i], c3 = L.cy[i + s];
-      const vreal diag = L.m + ((c0 + c1) + (c2 + c3));
-      vreal v = L.b[i];
-      if (!init) {
-        vreal xw = xin[i - 1], xe = xin[i + 1], xs = xin[i - s], xn = xin[i + s];
-        if (xc) {
-          // the neighbours' prolonged values (halo cells get no correction)
-          if (x > 1) xw = xw + ALPHA_C * xc[x / 2 + (long)((y + 1) / 2) * cs];
-          if (x < n) xe = xe + ALPHA_C * xc[(x + 2) / 2 + (long)((y + 1) / 2) * cs];
-          if (y > 1) xs = xs + ALPHA_C * xc[(x + 1) / 2 + (long)(y / 2) * cs];
-          if (y < n) xn = xn + ALPHA_C * xc[(x + 1) / 2 + (long)((y + 2) / 2) * cs];
-        }
-        v += (c0 * xw + c1 * xe) + (c2 * xs + c3 * xn);
-      }
-      r = v / diag;
-      if (lx >= 1 && lx <= SW && ly >= 1 && ly <= SH) L.x[i] = r;
+      r = first_value(L, xin, xc, cs, init, x, y);
+      if (lx >= 1 && lx <= SW && ly >= 1 && ly <= SH) L.x[x + (long)y * s] = r;
     }
     R[ly][lx] = r;
   }
@@ -255,6 +265,119 @@ __global__ __launch_bounds__(NT) void k_smooth2(Level L, const vreal *__restrict
     vreal v = L.b[i];
     v += (c0 * R[ly][lx - 1] + c1 * R[ly][lx + 1]) + (c2 * R[ly - 1][lx] + c3 * R[ly + 1][lx]);
     L.x[i] = v / diag;
+  }
+}
+
+// The same step with lanes taking horizontal pairs of cells, one of each colour (the
+// window's pairs start at even x, so with an even pitch a pair is one aligned 2-vector), 33
+// pairs per window row: phase one loads each array once with pair loads, computes the pair's
+// first-colour cell and keeps the other cell's inputs in registers, so phase two reads LDS
+// alone and stores the tile's pairs as one 2-vector. Pairs with a cell outside the mesh (and
+// every pair when the pitch is odd) take the per-cell path. Used for levels wider than
+// pair_min cells: at 4096^2 its launches take 78 and 114 us against 101 and 122 us; at
+// 2048^2 it is no faster, and on smaller levels its longer per-workgroup chain makes the
+// latency-bound launches slower.
+typedef vreal vpair __attribute__((ext_vector_type(2)));
+constexpr int NPAIR = (SW + 2) / 2, PPT = ((SH + 2) * NPAIR + NT - 1) / NT;
+
+__global__ __launch_bounds__(NT) void k_smooth2p(Level L, const vreal *__restrict__ xin, const vreal *__restrict__ xc,
+                                                int cs, int first, int init, const Rec *S, int rev) {
+  if (S->stopped) return;
+  __shared__ vreal R[SH + 2][SW + 2];
+  unsigned bx, by, bz;
+  ops_block_coords_dir(bx, by, bz, rev);
+  const int x0 = 1 + (int)bx * SW, y0 = 1 + (int)by * SH;
+  const int tid = threadIdx.x + BX * threadIdx.y;
+  const int s = L.s, n = L.n;
+  // per pair of the lane: the second-colour cell's b and coefficients, the first colour's value
+  vreal sb[PPT], sc0[PPT], sc1[PPT], sc2[PPT], sc3[PPT], rf[PPT];
+  bool sok[PPT];  // the second-colour cell is in the mesh
+#pragma unroll
+  for (int k = 0; k < PPT; k++) {
+    const int e = tid + k * NT;
+    sok[k] = false;
+    rf[k] = vreal(0);
+    if (e >= (SH + 2) * NPAIR) continue;
+    const int ly = e / NPAIR, lx = 2 * (e - ly * NPAIR);
+    const int x = x0 - 1 + lx, y = y0 - 1 + ly;  // cells (x, y) and (x + 1, y); x is even
+    const int fo = ((x + y) & 1) == first ? 0 : 1;  // which of the two is the first colour
+    const bool yin = y >= 1 && y <= n;
+    if (yin && x >= 1 && x + 1 <= n && !(s & 1)) {
+      const long i = x + (long)y * s;
+      const vpair bb = *(const vpair *)(L.b + i), cxa = *(const vpair *)(L.cx + i);
+      const vpair cya = *(const vpair *)(L.cy + i), cyb = *(const vpair *)(L.cy + i + s);
+      const vreal cx2 = L.cx[i + 2];
+      // the left cell's coefficients {cx[i], cx[i + 1], cy[i], cy[i + s]}, the right's shifted by one
+      const vreal l0 = cxa.x, l1 = cxa.y, l2 = cya.x, l3 = cyb.x;
+      const vreal r0 = cxa.y, r1 = cx2, r2 = cya.y, r3 = cyb.y;
+      const vreal c0 = fo ? r0 : l0, c1 = fo ? r1 : l1, c2 = fo ? r2 : l2, c3 = fo ? r3 : l3;
+      const vreal diag = L.m + ((c0 + c1) + (c2 + c3));
+      vreal v = fo ? bb.y : bb.x;
+      if (!init) {
+        const int xf = x + fo;
+        const vpair xm = *(const vpair *)(xin + i);
+        const vpair xsv = *(const vpair *)(xin + i - s), xnv = *(const vpair *)(xin + i + s);
+        vreal xw = fo ? xm.x : xin[i - 1], xe = fo ? xin[i + 2] : xm.y;
+        vreal xs = fo ? xsv.y : xsv.x, xn = fo ? xnv.y : xnv.x;
+        if (xc) {
+          if (xf > 1) xw = xw + ALPHA_C * xc[xf / 2 + (long)((y + 1) / 2) * cs];
+          if (xf < n) xe = xe + ALPHA_C * xc[(xf + 2) / 2 + (long)((y + 1) / 2) * cs];
+          if (y > 1) xs = xs + ALPHA_C * xc[(xf + 1) / 2 + (long)(y / 2) * cs];
+          if (y < n) xn = xn + ALPHA_C * xc[(xf + 1) / 2 + (long)((y + 2) / 2) * cs];
+        }
+        v += (c0 * xw + c1 * xe) + (c2 * xs + c3 * xn);
+      }
+      rf[k] = v / diag;
+      sb[k] = fo ? bb.x : bb.y;
+      sc0[k] = fo ? l0 : r0;
+      sc1[k] = fo ? l1 : r1;
+      sc2[k] = fo ? l2 : r2;
+      sc3[k] = fo ? l3 : r3;
+      sok[k] = true;
+    } else if (yin) {
+      // a pair at the mesh edge (or an odd pitch): cell by cell
+      const int xf = x + fo, xs2 = x + 1 - fo;
+      if (xf >= 1 && xf <= n) rf[k] = first_value(L, xin, xc, cs, init, xf, y);
+      if (xs2 >= 1 && xs2 <= n) {
+        const long i = xs2 + (long)y * s;
+        sb[k] = L.b[i];
+        sc0[k] = L.cx[i];
+        sc1[k] = L.cx[i + 1];
+        sc2[k] = L.cy[i];
+        sc3[k] = L.cy[i + s];
+        sok[k] = true;
+      }
+    }
+    R[ly][lx + fo] = rf[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PPT; k++) {
+    const int e = tid + k * NT;
+    if (e >= (SH + 2) * NPAIR) continue;
+    const int ly = e / NPAIR, lx = 2 * (e - ly * NPAIR);
+    if (ly < 1 || ly > SH) continue;
+    const int x = x0 - 1 + lx, y = y0 - 1 + ly;
+    const int fo = ((x + y) & 1) == first ? 0 : 1, lf = lx + fo, ls = lx + 1 - fo;
+    const bool fin = lf >= 1 && lf <= SW && x + fo >= 1 && x + fo <= n && y <= n;  // tile cell in the mesh
+    const bool sin = sok[k] && ls >= 1 && ls <= SW;
+    vreal rs = vreal(0);
+    if (sin) {
+      const vreal diag = L.m + ((sc0[k] + sc1[k]) + (sc2[k] + sc3[k]));
+      vreal v = sb[k];
+      v += (sc0[k] * R[ly][ls - 1] + sc1[k] * R[ly][ls + 1]) + (sc2[k] * R[ly - 1][ls] + sc3[k] * R[ly + 1][ls]);
+      rs = v / diag;
+    }
+    const long i = x + (long)y * s;
+    if (fin && sin && !(s & 1)) {
+      vpair o;
+      o.x = fo ? rs : rf[k];
+      o.y = fo ? rf[k] : rs;
+      *(vpair *)(L.x + i) = o;
+    } else {
+      if (fin) L.x[i + fo] = rf[k];
+      if (sin) L.x[i + 1 - fo] = rs;
+    }
   }
 }
 
@@ -558,7 +681,8 @@ struct MG {
   Rec *rec = nullptr, *host = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int graph_n = -1;
-  const void *graph_key[10] = {};
+  const void *graph_key[11] = {};
+  int pair_min = 2049;  // smoothing levels of at least this many cells per side use k_smooth2p
 };
 MG G;
 
@@ -596,8 +720,12 @@ void vcycle(hipStream_t st, vreal *x0, int &dir) {
   for (int l = 0; l < l0; l++) {
     Level a = lv[l];
     a.x = G.xb[l];
-    hipLaunchKernelGGL(k_smooth2, s2grid(a), blk, 0, st, a, (const vreal *)nullptr, (const vreal *)nullptr, 0, 0,
-                       1, G.rec, dir++ & 1);
+    if (a.n >= G.pair_min)
+      hipLaunchKernelGGL(k_smooth2p, s2grid(a), blk, 0, st, a, (const vreal *)nullptr, (const vreal *)nullptr, 0, 0,
+                         1, G.rec, dir++ & 1);
+    else
+      hipLaunchKernelGGL(k_smooth2, s2grid(a), blk, 0, st, a, (const vreal *)nullptr, (const vreal *)nullptr, 0, 0,
+                         1, G.rec, dir++ & 1);
     hipLaunchKernelGGL(k_restrict, grid2(lv[l + 1].n, lv[l + 1].n), blk, 0, st, a, lv[l + 1], G.rec, dir++ & 1);
   }
   Folded f;
@@ -605,9 +733,14 @@ void vcycle(hipStream_t st, vreal *x0, int &dir) {
   for (int k = 0; k < f.n; k++) f.lv[k] = lv[l0 + k];
   hipLaunchKernelGGL(k_coarse, dim3(1), dim3(CT), 0, st, f, G.rec);
   // up-sweep: the coarse correction prolonged inside the smoothing (k_smooth2 with xc)
-  for (int l = l0 - 1; l >= 0; l--)
-    hipLaunchKernelGGL(k_smooth2, s2grid(lv[l]), blk, 0, st, lv[l], (const vreal *)G.xb[l],
-                       (const vreal *)lv[l + 1].x, lv[l + 1].s, 1, 0, G.rec, dir++ & 1);
+  for (int l = l0 - 1; l >= 0; l--) {
+    if (lv[l].n >= G.pair_min)
+      hipLaunchKernelGGL(k_smooth2p, s2grid(lv[l]), blk, 0, st, lv[l], (const vreal *)G.xb[l],
+                         (const vreal *)lv[l + 1].x, lv[l + 1].s, 1, 0, G.rec, dir++ & 1);
+    else
+      hipLaunchKernelGGL(k_smooth2, s2grid(lv[l]), blk, 0, st, lv[l], (const vreal *)G.xb[l],
+                         (const vreal *)lv[l + 1].x, lv[l + 1].s, 1, 0, G.rec, dir++ & 1);
+  }
 }
 
 }  // namespace
@@ -624,6 +757,10 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
     ops_dat_release_raw_data(d, 0, OPS_RW);
   }
   hipStream_t st = (hipStream_t)ops_hip_get_stream();
+  // OPS_TEA_PAIR_MIN: the narrowest level smoothed by the pair kernel (default 2049; the
+  // kernels give the same values, so this moves time, not results)
+  const char *pm = getenv("OPS_TEA_PAIR_MIN");
+  G.pair_min = pm && atoi(pm) > 0 ? atoi(pm) : 2049;
   const int n = P->nx;
   const Fine g{n, n + 2, P->rx, P->ry};
   const dim3 blk(BX, BY), gf = gridf(n);
@@ -716,9 +853,10 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
   double dm_bits = dmean;
   long dm_key;
   memcpy(&dm_key, &dm_bits, sizeof dm_key);
-  const void *key[10] = {G.mem, u64, r64, pa, w64, kx, ky, (const void *)(long)n, defl, (const void *)dm_key};
+  const void *key[11] = {G.mem, u64, r64, pa, w64, kx, ky, (const void *)(long)n, defl, (const void *)dm_key,
+                         (const void *)(long)G.pair_min};
   bool same = G.graph[0] != nullptr;
-  for (int k = 0; k < 10 && same; k++) same = key[k] == G.graph_key[k];
+  for (int k = 0; k < 11 && same; k++) same = key[k] == G.graph_key[k];
   // OPS_TEA_GRAPH=0: launch every iteration's kernels eagerly (the same kernels and order;
   // for tools that do not follow graph launches)
   const char *ge = getenv("OPS_TEA_GRAPH");
@@ -733,7 +871,7 @@ void TEA_MGCG_ENTRY(ops_dat u, ops_dat u0, ops_dat r, ops_dat p, ops_dat w, ops_
       CK(hipGraphInstantiate(&G.graph[par], gr, nullptr, nullptr, 0));
       CK(hipGraphDestroy(gr));
     }
-    for (int k = 0; k < 10; k++) G.graph_key[k] = key[k];
+    for (int k = 0; k < 11; k++) G.graph_key[k] = key[k];
   }
   // iterations 1, 2, ...: odd iterations use parity 1 (p from pa into pb)
   int issued = 0;
