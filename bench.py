@@ -110,17 +110,23 @@ def kernel_short_name(name):
     return short
 
 
-def ingest_pmc(d):
+def ingest_pmc(d, match=None):
     """rocprofv3 --pmc CSVs (one run with FETCH_SIZE, one with WRITE_SIZE; values in KiB per
     dispatch) -> profiles/pmc_traffic.json. Each counter is scaled by what it reads for the
     runtime's copy probe kernel, whose true traffic is known (n bytes read, n written).
-    Kernels measured here replace their entries; the file's other kernels are kept."""
+    Kernels measured here replace their entries; the file's other kernels are kept. With
+    `match` (a regular expression), only kernels whose full name it finds are taken (the copy
+    probe always is)."""
     import csv
     import glob
+    import re
     rows = []
     for f in glob.glob(str(d / "**" / "*counter_collection.csv"), recursive=True):
         with open(f) as fh:
             rows += list(csv.DictReader(fh))
+    if match:
+        pat = re.compile(match)
+        rows = [r for r in rows if "copy_probe_kernel" in r["Kernel_Name"] or pat.search(r["Kernel_Name"])]
     per = {}
     for r in rows:
         short = kernel_short_name(r["Kernel_Name"])
@@ -149,7 +155,7 @@ def ingest_pmc(d):
         old = json.loads(p.read_text())
         merged = dict(old.get("kernels", {}))
         merged.update(out["kernels"])
-        srcs = old.get("sources", [old["source"]] if "source" in old else [])
+        srcs = [x for x in old.get("sources", [old["source"]] if "source" in old else []) if x != str(d)]
         out = {"sources": srcs + [str(d)], "unit": out["unit"], "calibration": out["calibration"],
                "kernels": dict(sorted(merged.items()))}
     p.write_text(json.dumps(out, indent=1) + "\n")
@@ -810,9 +816,11 @@ def main():
     ap.add_argument("--ingest-pmc", metavar="DIR", default=None,
                     help="read rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE CSV output under DIR "
                          "and write profiles/pmc_traffic.json, then exit")
+    ap.add_argument("--ingest-match", metavar="REGEX", default=None,
+                    help="with --ingest-pmc: take only the kernels whose full name matches REGEX")
     args = ap.parse_args()
     if args.ingest_pmc:
-        ingest_pmc(Path(args.ingest_pmc))
+        ingest_pmc(Path(args.ingest_pmc), args.ingest_match)
         return
     if not args.no_autotune:
         # the runtime measures candidate workgroup shapes (and tridiagonal tile heights) on
