@@ -146,6 +146,9 @@ typedef struct ops_reduction_core {
   OPS_instance *instance;
   char *data_d;     /* device-resident running value (this backend) */
   int dirty_d;      /* 1: device value is newer than data */
+  char *data_m;     /* pinned host copy of data_d, written by the same kernel (this backend) */
+  unsigned seq_expect; /* finisher launches of the handle outside graph capture */
+  int seq_valid;    /* 1: the last finisher launch stores seq_expect after data_m */
 } ops_reduction_core;
 typedef ops_reduction_core *ops_reduction;
 

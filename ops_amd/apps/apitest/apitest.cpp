@@ -9,6 +9,8 @@
 //   api_tridiag    ops_tridMultiDimBatch[_Inc] on caller-provided coefficients
 //   api_reduce     INC/MIN/MAX reductions on double and int, a 2-component reduction,
 //                  accumulation across two loops before ops_reduction_result
+//   api_reduce_graph  a sum reduction read on the host after every pass of v += 1, sum(v):
+//                  eager passes, then passes replayed from a captured graph, then eager again
 //   api_random[_normal]  ops_randomgen_init + ops_fill_random_uniform / _normal on a dat, read back with
 //                  its halos
 //   api_tridiag2   two solves in a row (the second on a smaller box) through the queueing
@@ -309,6 +311,52 @@ extern "C" int api_reduce(int argc, const char **argv, int nx, int ny, const dou
     return 0;
   } catch (std::exception &e) {
     return fail("api_reduce", e);
+  }
+}
+
+// out[k] = the sum read after pass k of {v += 1; sum(v)}: passes 0 and 1 eager, 2 .. passes - 3
+// replayed from a graph captured from that pair of loops (eager where the build has no
+// graphs), the last two eager again
+extern "C" int api_reduce_graph(int argc, const char **argv, int nx, int ny, const double *vals, int passes,
+                                double *out) {
+  try {
+    ops_init(argc, argv, 1);
+    ops_block blk = ops_decl_block(2, "redg");
+    int size[2] = {nx, ny}, base[2] = {0, 0}, dm[2] = {-1, -1}, dp[2] = {1, 1};
+    double *none = nullptr;
+    ops_dat V = ops_decl_dat(blk, 1, size, base, dm, dp, none, "double", "v");
+    int full[4] = {0, nx, 0, ny};
+    ops_dat_set_data_slab_memspace(V, 0, (char *)vals, full, OPS_HOST);
+    int s00[] = {0, 0};
+    ops_stencil S00 = ops_decl_stencil(2, 1, s00, "00");
+    ops_reduction hs = ops_decl_reduction_handle(sizeof(double), "double", "vsum");
+    auto pass = [&]() {
+      ops_par_loop(api_addone, "api_addone", blk, 2, full, ops_arg_dat(V, 1, S00, "double", OPS_RW));
+      ops_par_loop(api_vsum, "api_vsum", blk, 2, full, ops_arg_dat(V, 1, S00, "double", OPS_READ),
+                   ops_arg_reduce(hs, 1, "double", OPS_INC));
+    };
+    void *g = nullptr;
+    for (int k = 0; k < passes; k++) {
+      const bool replay = k >= 2 && k < passes - 2;
+      if (replay && !g) {
+        // the capture records the pair (it does not run it); the first replay is pass k
+        if (ops_hip_graph_begin() == 0) {
+          pass();
+          g = ops_hip_graph_end();
+        }
+      }
+      if (replay && g) {
+        if (ops_hip_graph_replay(g, 1) != 0) throw OPSException(OPS_RUNTIME_ERROR, "replay failed");
+      } else {
+        pass();
+      }
+      ops_reduction_result(hs, &out[k]);
+    }
+    if (g) ops_hip_graph_free(g);
+    ops_exit();
+    return 0;
+  } catch (std::exception &e) {
+    return fail("api_reduce_graph", e);
   }
 }
 

@@ -21,6 +21,10 @@ void api_reduce(const ACC<double> &v, const ACC<int> &w, double *sum, double *mn
   pair[1] = pair[1] + 1.0;
 }
 
+// v += 1, then the sum of v (the graph-replayed reduction sequence)
+void api_addone(ACC<double> &v) { v(0, 0) = v(0, 0) + 1.0; }
+void api_vsum(const ACC<double> &v, double *sum) { *sum = *sum + v(0, 0); }
+
 // one dat read and written at the same point through two arguments (the runtime then uses
 // the kernel variant without no-alias pointers)
 void api_twice_plus_one(const ACC<double> &in, ACC<double> &out) { out(0, 0) = 2.0 * in(0, 0) + 1.0; }

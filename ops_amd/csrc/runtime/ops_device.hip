@@ -144,6 +144,19 @@ void ops_dev_free(void *p) {
     (void)hipFree(p);
   }
 }
+void *ops_dev_host_alloc(size_t bytes) {
+  void *p = nullptr;
+  HIPCHECK(hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocCoherent | hipHostMallocMapped));
+  memset(p, 0, bytes ? bytes : 8);
+  return p;
+}
+void ops_dev_host_free(void *p) {
+  if (p) {
+    ops_trid_flush();
+    (void)hipStreamSynchronize(g_dev.stream);
+    (void)hipHostFree(p);
+  }
+}
 void ops_dev_memset(void *p, int v, size_t bytes) {
   ops_trid_flush();
   HIPCHECK(hipMemsetAsync(p, v, bytes, g_dev.stream));
@@ -935,6 +948,9 @@ extern "C" int ops_hip_loop_begin(ops_hip_loop *lp, int kernel_id, const char *n
 struct RedEntry {
   const void *part;
   void *value;
+  void *mirror;  // the handle's pinned host copy (written with value), or null
+  unsigned *seq_ptr;  // pinned word that receives seq after the copy (release, system scope), or null
+  unsigned seq;
   unsigned nb;  // partials (workgroups of the loop)
   int dim;
   int op;       // OPS_INC, OPS_MIN, OPS_MAX
@@ -954,7 +970,11 @@ __device__ __forceinline__ void red_fold(const RedEntry &E, T *lds) {
     T a = OP::identity();
     for (unsigned i = tid; i < E.nb; i += 256) a = OP::op(a, part[(size_t)i * E.dim + d]);
     const T r = ops_block_reduce<OP>(a, lds);
-    if (tid == 0) value[d] = E.init ? r : OP::op(value[d], r);
+    if (tid == 0) {
+      const T v = E.init ? r : OP::op(value[d], r);
+      value[d] = v;
+      if (E.mirror) ((T *)E.mirror)[d] = v;
+    }
   }
 }
 
@@ -970,6 +990,36 @@ __global__ __launch_bounds__(256) void red_finish_kernel(const RedTable t) {
   if (E.type == 0) red_fold_op<double>(E, lds);
   else if (E.type == 1) red_fold_op<float>(E, (float *)lds);
   else red_fold_op<int>(E, (int *)lds);
+  // the host may read the pinned copy as soon as it sees this launch's number
+  if (threadIdx.x == 0 && E.seq_ptr)
+    __hip_atomic_store(E.seq_ptr, E.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// the pinned sequence word of a handle: after the value's bytes, on its own 64-byte line
+static unsigned *red_seq_word(ops_reduction h) {
+  return (unsigned *)(h->data_m + (((size_t)std::max(h->size, 8) + 63) & ~(size_t)63));
+}
+
+void ops_dev_red_wait(ops_reduction h) {
+  ops_trid_flush();  // pending batches and chains first: the finisher may be among them
+  if (h->seq_valid) {
+    // spin up to 200 us on the finisher's number, then block on the stream
+    const unsigned *w = red_seq_word(h);
+    const auto t0 = std::chrono::steady_clock::now();
+    while (__atomic_load_n(w, __ATOMIC_ACQUIRE) != h->seq_expect) {
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) break;
+    }
+    if (__atomic_load_n(w, __ATOMIC_ACQUIRE) == h->seq_expect) return;
+  }
+  HIPCHECK(hipStreamSynchronize(g_dev.stream));
+}
+
+// reduction handles finished inside the current graph capture (a replay gives them a newer
+// device value, as the captured loops would)
+static std::vector<ops_reduction> g_capture_reds;
+static void capture_note_reduction(ops_reduction h) {
+  if (std::find(g_capture_reds.begin(), g_capture_reds.end(), h) == g_capture_reds.end())
+    g_capture_reds.push_back(h);
 }
 
 static void finish_reductions(const ops_hip_loop *lp, const ops_arg *args, int nargs) {
@@ -986,7 +1036,20 @@ static void finish_reductions(const ops_hip_loop *lp, const ops_arg *args, int n
     else if (!strcmp(ty, "float")) type = 1;
     else if (!strcmp(ty, "int")) type = 2;
     else ops_throw(OPS_NOT_IMPLEMENTED, std::string("reduction of type ") + ty);
-    t.e[n++] = {lp->red_part[i], lp->red_value[i], lp->nblocks, a.dim, a.acc, type, lp->red_init[i]};
+    ops_reduction h = (ops_reduction)a.data;
+    // a captured launch replays with its arguments fixed: it stores no number, and reads of
+    // the handle after it wait for the stream
+    unsigned *seq_ptr = nullptr;
+    if (h->data_m && !g_capturing) {
+      seq_ptr = red_seq_word(h);
+      h->seq_valid = 1;
+      h->seq_expect++;
+    } else {
+      h->seq_valid = 0;
+      if (g_capturing) capture_note_reduction(h);
+    }
+    t.e[n++] = {lp->red_part[i], lp->red_value[i], h->data_m, seq_ptr, h->seq_expect, lp->nblocks, a.dim, a.acc,
+                type, lp->red_init[i]};
   }
   if (n == 0) return;
   hipLaunchKernelGGL(red_finish_kernel, dim3(n), dim3(256), 0, g_dev.stream, t);
@@ -1255,8 +1318,16 @@ extern "C" int ops_hip_graph_begin(void) {
   if (g_timing) ops_throw(OPS_RUNTIME_ERROR, "ops_hip_graph_begin: per-kernel timing is on");
   if (hipStreamBeginCapture(g_dev.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return 1;
   g_capturing = true;
+  g_capture_reds.clear();
   return 0;
 }
+
+// the handle ops_hip_graph_end returns: the executable graph and the reduction handles its
+// loops finish
+struct GraphRec {
+  hipGraphExec_t exec;
+  std::vector<ops_reduction> reds;
+};
 
 extern "C" void *ops_hip_graph_end(void) {
   ops_trid_flush();
@@ -1266,15 +1337,27 @@ extern "C" void *ops_hip_graph_end(void) {
   hipGraphExec_t e = nullptr;
   const hipError_t st = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
-  return st == hipSuccess ? (void *)e : nullptr;
+  if (st != hipSuccess) return nullptr;
+  return new GraphRec{e, g_capture_reds};
 }
 
 extern "C" int ops_hip_graph_replay(void *graph, int times) {
+  GraphRec *r = (GraphRec *)graph;
   for (int i = 0; i < times; i++)
-    if (hipGraphLaunch((hipGraphExec_t)graph, g_dev.stream) != hipSuccess) return 1;
+    if (hipGraphLaunch(r->exec, g_dev.stream) != hipSuccess) return 1;
+  // the replayed finishers wrote these handles' device values and pinned copies (with no
+  // launch number: reads wait for the stream); the handles count as used in a reduction
+  for (ops_reduction h : r->reds) {
+    h->dirty_d = 1;
+    h->seq_valid = 0;
+    h->initialized = 1;
+  }
   return 0;
 }
 
 extern "C" void ops_hip_graph_free(void *graph) {
-  if (graph) (void)hipGraphExecDestroy((hipGraphExec_t)graph);
+  GraphRec *r = (GraphRec *)graph;
+  if (!r) return;
+  (void)hipGraphExecDestroy(r->exec);
+  delete r;
 }

@@ -75,6 +75,11 @@ void ops_dev_init(ops_runtime_instance *inst);
 void ops_dev_exit(ops_runtime_instance *inst);
 void *ops_dev_malloc(size_t bytes);
 void ops_dev_free(void *p);
+// pinned, coherent host memory that kernels may write (reduction results read on the host)
+void *ops_dev_host_alloc(size_t bytes);
+void ops_dev_host_free(void *p);
+// wait for a reduction handle's pinned copy (the finisher's sequence number, then the stream)
+void ops_dev_red_wait(ops_reduction h);
 void ops_dev_memset(void *p, int v, size_t bytes);
 void ops_dev_h2d(void *dst, const void *src, size_t bytes);   // synchronous
 void ops_dev_d2h(void *dst, const void *src, size_t bytes);   // synchronous (drains stream)

@@ -356,6 +356,25 @@ def reduce_run(kind, nx, ny, seed=3):
     return v, w, out
 
 
+def reduce_graph_run(kind, nx, ny, passes, seed=4):
+    L = lib("apitest", kind)
+    v = np.random.default_rng(seed).integers(-50, 50, nx * ny).astype(np.float64)
+    out = np.zeros(passes)
+    argc, av = argv()
+    assert L.api_reduce_graph(argc, av, nx, ny, p(v), passes, p(out)) == 0
+    return v, out
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_reduction_read_after_every_pass(kind):
+    # passes of {v += 1; sum(v)} read on the host after each: eagerly, replayed from a
+    # captured graph (on the GPU), and eagerly again; integer values, so every sum is exact
+    nx, ny, passes = 300, 201, 7
+    v, out = reduce_graph_run(kind, nx, ny, passes)
+    expect = [v.sum() + (k + 1) * nx * ny for k in range(passes)]
+    assert out.tolist() == expect
+
+
 @pytest.mark.parametrize("kind", KINDS)
 @pytest.mark.parametrize("shape", [(2, 3), (7, 5), (300, 201)])
 def test_reductions(kind, shape):
