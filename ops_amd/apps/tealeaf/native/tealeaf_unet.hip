@@ -642,18 +642,32 @@ __global__ __launch_bounds__(256) void k_wtrans(const bf16 *w, int cout, int cin
   wt[((long)ci * 9 + (8 - tap)) * cout + co] = w[i];
 }
 
-// 2 x 2 average pooling, BF16 -> BF16 (FP32 sum)
+// 2 x 2 average pooling, BF16 -> BF16 (FP32 sum), 8 channels per lane (C is a multiple of 8)
 __global__ __launch_bounds__(256) void k_pool(const bf16 *in, int B, int H, int W, int C, bf16 *out) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  const int Hc = H / 2, Wc = W / 2;
-  const long n = (long)B * Hc * Wc * C;
+  const int Hc = H / 2, Wc = W / 2, C8 = C >> 3;
+  const long n = (long)B * Hc * Wc * C8;
   if (i >= n) return;
-  const int c = (int)(i % C);
-  const long p = i / C;
+  const int c = (int)(i % C8) * 8;
+  const long p = i / C8;
   const int X = (int)(p % Wc), Y = (int)((p / Wc) % Hc), b = (int)(p / ((long)Wc * Hc));
-  const long base = (((long)b * H + 2 * Y) * W + 2 * X) * C + c;
-  const float s = (bf2f(in[base]) + bf2f(in[base + C])) + (bf2f(in[base + (long)W * C]) + bf2f(in[base + (long)W * C + C]));
-  out[i] = f2bf(0.25f * s);
+  const long base = (((long)b * H + 2 * Y) * W + 2 * X) * C + c, r = (long)W * C;
+  const u4 a0 = *(const u4 *)(in + base), a1 = *(const u4 *)(in + base + C);
+  const u4 a2 = *(const u4 *)(in + base + r), a3 = *(const u4 *)(in + base + r + C);
+  u4 o;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    unsigned h[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int sh = 16 * q;
+      const float s = (bf2f((bf16)(a0[k] >> sh)) + bf2f((bf16)(a1[k] >> sh))) +
+                      (bf2f((bf16)(a2[k] >> sh)) + bf2f((bf16)(a3[k] >> sh)));
+      h[q] = f2bf(0.25f * s);
+    }
+    o[k] = h[0] | (h[1] << 16);
+  }
+  *(u4 *)(out + p * C + c) = o;
 }
 
 // adjoint of the pooling: gfine += 0.25 * gcoarse(parent), four channels per lane (C is a
@@ -926,7 +940,7 @@ void forward() {
     const int lv = SPEC[l].level, h = H >> lv, w = W >> lv;
     if (l >= 1 && l <= 3) {  // pool the previous encoder output
       const long n = (long)B * h * w * SPEC[l].c2;
-      hipLaunchKernelGGL(k_pool, dim3(g1d(n)), dim3(256), 0, st, N.act[l - 1], B, 2 * h, 2 * w,
+      hipLaunchKernelGGL(k_pool, dim3(g1d(n / 8)), dim3(256), 0, st, N.act[l - 1], B, 2 * h, 2 * w,
                          SPEC[l].c2, N.pooled[lv]);
     }
     ConvArgs a;
@@ -1068,12 +1082,15 @@ struct Data {
 };
 Data D;
 
-// batch input: 16 channels (4 features, zeros) from the packed 4-channel samples
+// batch input: 16 channels (4 features, zeros) from the packed 4-channel samples, one pixel
+// per lane (an 8-byte load, two 16-byte stores)
 __global__ __launch_bounds__(256) void k_expand(const bf16 *src, long npix, bf16 *dst) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= npix * 16) return;
-  const int c = (int)(i & 15);
-  dst[i] = c < 4 ? src[(i >> 4) * 4 + c] : (bf16)0;
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  const uint2 f = *(const uint2 *)(src + p * 4);
+  u4 *d = (u4 *)(dst + p * 16);
+  d[0] = u4{f.x, f.y, 0u, 0u};
+  d[1] = u4{0u, 0u, 0u, 0u};
 }
 
 __global__ __launch_bounds__(256) void k_pack_in(const float *src, long npix, bf16 *dst) {
@@ -1666,7 +1683,7 @@ extern "C" int tea_unet_train(int epochs, int batch, float lr, float beta, unsig
       for (int i0 = 0; i0 + batch <= D.count; i0 += batch) {
         for (int k = 0; k < batch; k++) {
           const int idx = order[i0 + k];
-          hipLaunchKernelGGL(k_expand, dim3(g1d(spix * 16)), dim3(256), 0, st, D.in + (long)idx * spix * 4, spix,
+          hipLaunchKernelGGL(k_expand, dim3(g1d(spix)), dim3(256), 0, st, D.in + (long)idx * spix * 4, spix,
                              N.in + k * spix * 16);
           CK(hipMemcpyAsync(N.target + k * spix, D.target + (long)idx * spix, spix * sizeof(float),
                             hipMemcpyDeviceToDevice, st));
