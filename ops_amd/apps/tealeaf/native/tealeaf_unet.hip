@@ -48,6 +48,11 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned u4 __attribute__((ext_vector_type(4)));   // 8 BF16 values, one 16-byte load
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));    // the MFMA operand type
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));    // a v_dot2_f32_bf16 operand
+// a.lo b.lo + a.hi b.hi + c, the BF16 pairs packed in 32-bit words (gfx950's v_dot2c_f32_bf16)
+__device__ __forceinline__ float dot2(unsigned a, unsigned b, float c) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf2, a), __builtin_bit_cast(bf2, b), c, false);
+}
 typedef uint16_t bf16;
 __device__ __forceinline__ bf8 as_bf8(u4 v) { return __builtin_bit_cast(bf8, v); }
 
@@ -684,6 +689,111 @@ __global__ __launch_bounds__(256) void k_unpool_add(const float *gc, int B, int 
   *d = *d + 0.25f * *(const f4 *)(gc + (((long)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c);
 }
 
+// The output layer (16 input channels, the prediction its channel 0): only channel 0 of its
+// output is used, so it runs as a single-output-channel convolution with the same BF16
+// operands and FP32 sums as the layers above (channels 1..15 of its output gradient are zero,
+// so their weights' and biases' gradients are zero and are never written). w is its
+// BF16 weight row of channel 0, [9][16]; x its input (dec0), [B][H][W][16] BF16.
+constexpr int OUT_CIN = 16;
+static_assert(SPEC[NL - 1].c1 == 0 && SPEC[NL - 1].c2 == OUT_CIN && SPEC[NL - 1].relu == 0 && SPEC[NL - 1].level == 0,
+              "the output layer's kernels assume 16 input channels at full resolution and no ReLU");
+
+// prediction: out[p] = bias + sum over taps and channels of w x (zero outside the image)
+__global__ __launch_bounds__(256) void k_out_fwd(const bf16 *x, const bf16 *w, const float *bias, int B, int H,
+                                                 int W, float *out) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long npix = (long)B * H * W;
+  if (p >= npix) return;
+  const long img = p / ((long)H * W) * ((long)H * W);
+  const int q = (int)(p - img), yq = q / W, xq = q - yq * W;
+  const unsigned *w2 = (const unsigned *)w;  // channel pairs (2k, 2k + 1) of each tap
+  float acc = 0.f;
+#pragma unroll
+  for (int tap = 0; tap < 9; tap++) {
+    const int ys = yq + tap / 3 - 1, xs = xq + tap % 3 - 1;
+    if (ys < 0 || ys >= H || xs < 0 || xs >= W) continue;
+    const u4 *src = (const u4 *)(x + (img + (long)ys * W + xs) * OUT_CIN);
+    const u4 a0 = src[0], a1 = src[1];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      acc = dot2(a0[k], w2[tap * 8 + k], acc);
+      acc = dot2(a1[k], w2[tap * 8 + 4 + k], acc);
+    }
+  }
+  out[p] = acc + bias[0];
+}
+
+// input gradient, into dec0's output gradient: gx[q][c] = sum over taps of w[tap][c] G(q - offset(tap)),
+// G = the output gradient rounded to BF16 (the operand precision of the other layers)
+__global__ __launch_bounds__(256) void k_out_igrad(const float *g, const bf16 *w, int B, int H, int W, float *gx) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long npix = (long)B * H * W;
+  if (p >= npix) return;
+  const long img = p / ((long)H * W) * ((long)H * W);
+  const int q = (int)(p - img), yq = q / W, xq = q - yq * W;
+  float acc[OUT_CIN];
+#pragma unroll
+  for (int c = 0; c < OUT_CIN; c++) acc[c] = 0.f;
+#pragma unroll
+  for (int tap = 0; tap < 9; tap++) {
+    const int yo = yq - (tap / 3 - 1), xo = xq - (tap % 3 - 1);
+    if (yo < 0 || yo >= H || xo < 0 || xo >= W) continue;
+    const float G = bf2f(f2bf(g[img + (long)yo * W + xo]));
+#pragma unroll
+    for (int c = 0; c < OUT_CIN; c++) acc[c] = acc[c] + bf2f(w[tap * OUT_CIN + c]) * G;
+  }
+  f4 *d = (f4 *)(gx + p * OUT_CIN);
+#pragma unroll
+  for (int k = 0; k < OUT_CIN / 4; k++) d[k] = f4{acc[4 * k], acc[4 * k + 1], acc[4 * k + 2], acc[4 * k + 3]};
+}
+
+// weight gradient partials of one band of rows: part[band][tap][c] = sum over the band's
+// output pixels o of G(o) x(o + offset(tap))[c]; nine wavefronts, one per tap. A lane takes
+// pairs of horizontally adjacent output pixels (W is even, so a pair stays in its row) in a
+// pixel-strided order: the pair's two G values and, per channel, its two inputs are packed
+// into BF16 pairs, so one dot2 adds both pixels' products; the wavefront's lanes are then
+// added by a fixed butterfly.
+__global__ __launch_bounds__(576) void k_out_wgrad(const float *g, const bf16 *x, int B, int H, int W, int band,
+                                                   float *part) {
+  const int tap = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const long rows = (long)B * H, r0 = (long)blockIdx.x * band, r1 = min(rows, r0 + band);
+  float acc[OUT_CIN];
+#pragma unroll
+  for (int c = 0; c < OUT_CIN; c++) acc[c] = 0.f;
+  const int Wp = W >> 1;
+  const long n = (r1 - r0) * Wp;
+  const u4 zero = {0u, 0u, 0u, 0u};
+  for (long e = lane; e < n; e += 64) {
+    const long row = r0 + e / Wp;  // b * H + y
+    const int xo = 2 * (int)(e % Wp), yo = (int)(row % H), ys = yo + dy;
+    if (ys < 0 || ys >= H) continue;
+    const int xs0 = xo + dx, xs1 = xo + 1 + dx;
+    const bool v0 = xs0 >= 0 && xs0 < W, v1 = xs1 >= 0 && xs1 < W;
+    // the pair's G (BF16, as the other layers' operands), zero where its input is outside
+    const unsigned gp = (v0 ? (unsigned)f2bf(g[row * W + xo]) : 0u) |
+                        ((v1 ? (unsigned)f2bf(g[row * W + xo + 1]) : 0u) << 16);
+    const bf16 *base = x + (row + dy) * W * OUT_CIN;
+    const u4 p0 = v0 ? *(const u4 *)(base + (long)xs0 * OUT_CIN) : zero;
+    const u4 p1 = v0 ? *(const u4 *)(base + (long)xs0 * OUT_CIN + 8) : zero;
+    const u4 q0 = v1 ? *(const u4 *)(base + (long)xs1 * OUT_CIN) : zero;
+    const u4 q1 = v1 ? *(const u4 *)(base + (long)xs1 * OUT_CIN + 8) : zero;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      // channels 2k, 2k + 1 (and 8 + 2k, 9 + 2k) of the two pixels, as pixel pairs
+      acc[2 * k] = dot2((p0[k] & 0xffffu) | (q0[k] << 16), gp, acc[2 * k]);
+      acc[2 * k + 1] = dot2((p0[k] >> 16) | (q0[k] & 0xffff0000u), gp, acc[2 * k + 1]);
+      acc[8 + 2 * k] = dot2((p1[k] & 0xffffu) | (q1[k] << 16), gp, acc[8 + 2 * k]);
+      acc[9 + 2 * k] = dot2((p1[k] >> 16) | (q1[k] & 0xffff0000u), gp, acc[9 + 2 * k]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < OUT_CIN; c++) {
+    const float v = ops_wave_reduce<ops_red_inc<float>>(acc[c]);
+    if (lane == 0) part[((long)blockIdx.x * 9 + tap) * OUT_CIN + c] = v;
+  }
+}
+
 // Loss on channel 0 of the output: the error e = prediction - target in the norm
 // e^T (I + beta C) e, C the conduction part of the sample's operator (A = I + C), relative
 // to the target's, averaged over the batch. beta = 0 gives the relative mean squared
@@ -943,6 +1053,14 @@ void forward() {
       hipLaunchKernelGGL(k_pool, dim3(g1d(n / 8)), dim3(256), 0, st, N.act[l - 1], B, 2 * h, 2 * w,
                          SPEC[l].c2, N.pooled[lv]);
     }
+    if (l == NL - 1) {
+      // the output layer: channel 0 alone (the prediction); its weights are read as BF16
+      // pairs (32-bit words)
+      if (N.woff[l] % 2) throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: output weights not word aligned");
+      hipLaunchKernelGGL(k_out_fwd, dim3(g1d((long)B * h * w)), dim3(256), 0, st, N.act[l - 1], N.wb + N.woff[l],
+                         N.p + N.boff[l], B, h, w, N.out32);
+      continue;
+    }
     ConvArgs a;
     sources(l, a.src1, a.src2);
     a.c1 = SPEC[l].c1;
@@ -972,6 +1090,37 @@ void backward() {
   for (int l = NL - 1; l >= 0; l--) {
     const int lv = SPEC[l].level, h = H >> lv, w = W >> lv, cout = SPEC[l].cout, cin = cin_of(l);
     const long npix = (long)B * h * w, n = npix * cout;
+    if (l == NL - 1) {
+      // the output layer, channel 0 alone: bias gradient (the sum of the gradient), weight
+      // gradient in bands of 8 rows (partials folded like the other layers'), input gradient
+      // straight into dec0's output gradient
+      hipLaunchKernelGGL(k_bias_seg, dim3(BIAS_SEGS), dim3(256), 0, st, N.grad[l], npix, 1, N.bseg);
+      hipLaunchKernelGGL(k_bias_fin, dim3(1), dim3(256), 0, st, N.bseg, (long)BIAS_SEGS, 1, N.g + N.boff[l]);
+      const int band = 8, nbands = (B * h + band - 1) / band;
+      const size_t need = (size_t)nbands * 9 * OUT_CIN;
+      if (need > N.wpart_cap) {
+        sync();
+        dfree(N.wpart);
+        N.wpart = dalloc<float>(need);
+        N.wpart_cap = need;
+      }
+      hipLaunchKernelGGL(k_out_wgrad, dim3((unsigned)nbands), dim3(576), 0, st, N.grad[l], N.act[l - 1], B, h, w,
+                         band, N.wpart);
+      const unsigned gx = (unsigned)((9 * OUT_CIN + 63) / 64);
+      const int nseg = std::max(1, std::min(nbands / 16, (int)(2048 / gx)));
+      if (nseg > 1) {
+        hipLaunchKernelGGL(k_wgrad_sum, dim3(gx, (unsigned)nseg), dim3(256), 0, st, N.wpart, nbands, 1, OUT_CIN,
+                           N.wseg, nullptr);
+        hipLaunchKernelGGL(k_wgrad_sum, dim3(gx, 1), dim3(256), 0, st, N.wseg, nseg, 1, OUT_CIN, nullptr,
+                           N.g + N.woff[l]);
+      } else {
+        hipLaunchKernelGGL(k_wgrad_sum, dim3(gx, 1), dim3(256), 0, st, N.wpart, nbands, 1, OUT_CIN, nullptr,
+                           N.g + N.woff[l]);
+      }
+      hipLaunchKernelGGL(k_out_igrad, dim3(g1d(npix)), dim3(256), 0, st, N.grad[l], N.wb + N.woff[l], B, h, w,
+                         N.grad[NL - 2]);
+      continue;
+    }
     // masked gradient (the kernel's LDS tile holds at most 64 channels)
     if (cout > 64 || cout % 8) throw OPSException(OPS_INTERNAL_ERROR, "tea_unet: layer output too wide");
     const long mtiles = (long)B * h * ((w + MB_PX - 1) / MB_PX);
